@@ -1,0 +1,292 @@
+#!/usr/bin/env python3
+"""Sequential gread bandwidth on one MI355X, plus the grep scan.
+
+Workload (the sequential-read configuration): one file of seeded uniform
+random bytes (default 8 GiB), read by 256 workgroups that each gopen the file
+and gread a disjoint 32 MiB range into a device buffer, through a page cache
+of 256 KiB pages (default 16 GiB pool).  One "step" is one such launch over the
+whole file.  `value` is the warm-cache rate (the file's pages already resident
+in HBM, inputs in HBM when the timed region starts); the cold rate (page cache
+emptied, host page cache warm, every byte crosses the host link) and the
+measured host->device link peak are reported beside it.  Each range's
+FNV-1a-64 is checked bit-exact against the CPU path (host pread + FNV).
+
+Also measured: grep over 64 text files of 4 MiB (gopen + gread 64 KiB windows
+into LDS + counts of 1024 query words), cold and warm, bit-exact against the
+CPU path.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import ctypes
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "gread GB/s (cold, warm); grep GB/s; files/sec + p50/p99 latency served"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--file-gib", type=float, default=8.0)
+    ap.add_argument("--workgroups", type=int, default=256)
+    ap.add_argument("--threads", type=int, default=1024)
+    ap.add_argument("--page-kib", type=int, default=256)
+    ap.add_argument("--cache-gib", type=float, default=16.0)
+    ap.add_argument("--ring-depth", type=int, default=1024)
+    ap.add_argument("--host-threads", type=int, default=16)
+    ap.add_argument("--cold-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-grep", action="store_true")
+    ap.add_argument("--workdir", default=os.environ.get("GPUFS_BENCH_DIR", "/tmp/gpufs_bench"))
+    return ap.parse_args()
+
+
+def link_peak_gbs(torch, nbytes=1 << 30, trials=5):
+    """Pinned host -> device copy of 1 GiB, best of `trials` (the role of the
+    reference's pure-H2D option, samples/memcopy/cp.cpp:189-199)."""
+    src = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+    src.fill_(1)
+    dst = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    s = torch.cuda.Stream()
+    best = 0.0
+    for _ in range(trials):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(s):
+            e0.record(s)
+            dst.copy_(src, non_blocking=True)
+            e1.record(s)
+        e1.synchronize()
+        best = max(best, nbytes / (e0.elapsed_time(e1) / 1e3) / 1e9)
+    del src, dst
+    return best
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the read path shards per GPU with no exchange, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from gpufs_amd import _lib, datagen as dg
+    L = _lib.lib()
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    os.makedirs(args.workdir, exist_ok=True)
+
+    size = int(args.file_gib * (1 << 30))
+    n = args.workgroups
+    rng_bytes = size // n
+    assert rng_bytes * n == size
+    path = os.path.join(args.workdir, f"seq_{size}.bin")
+    if not os.path.exists(path) or os.path.getsize(path) != size:
+        t = time.time()
+        dg.random_file(path, size, dg.SEED_C1, threads=16)
+        log(f"generated {path} ({size / 2**30:.1f} GiB) in {time.time() - t:.1f}s")
+    subprocess.run(["cat", path], stdout=subprocess.DEVNULL, check=True)   # host page cache warm
+
+    link = link_peak_gbs(torch)
+    log(f"link peak (pinned H2D 1 GiB, best of 5): {link:.1f} GB/s")
+
+    ctx = _lib.Context(cache_bytes=int(args.cache_gib * (1 << 30)), page_size=args.page_kib << 10,
+                       ring_depth=args.ring_depth, host_threads=args.host_threads, max_files=1024)
+    ctx.host_start()
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    name = torch.zeros(256, dtype=torch.uint8)
+    b = os.fsencode(path)
+    name[:len(b)] = torch.frombuffer(bytearray(b), dtype=torch.uint8)
+    name = name.cuda()
+    dst = torch.empty(size, dtype=torch.uint8, device="cuda")
+    out = torch.zeros(n, dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()
+
+    def launch():
+        rc = L.gpufs_launch_gread_ranges(ctx.h, ctypes.c_void_p(name.data_ptr()), n, rng_bytes,
+                                         ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                         args.threads, None)
+        _lib.check("gpufs_launch_gread_ranges", rc)
+
+    def timed_pass():
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        launch()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / 1e3
+
+    def gpu_fnv():
+        h = torch.zeros(n, dtype=torch.int64, device="cuda")
+        _lib.check("fnv", L.gpufs_fnv1a64_ranges(ctypes.c_void_p(dst.data_ptr()), n, rng_bytes,
+                                                 ctypes.c_void_p(h.data_ptr()), None))
+        torch.cuda.synchronize()
+        return h.cpu().numpy().view(np.uint64)
+
+    # ---- CPU path (pread + FNV-1a-64 per range), also the checksum reference
+    cpu = None
+    cpu_fnv = None
+    ncores = min(16, os.cpu_count() or 1)
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        t = time.perf_counter()
+        cpu_fnv, got = orc.pread_ranges(path, n, rng_bytes, fnv=True, threads=ncores)
+        dt = time.perf_counter() - t
+        assert (got == rng_bytes).all()
+        t = time.perf_counter()
+        orc.pread_ranges(path, n, rng_bytes, fnv=False, threads=ncores)
+        dt_copy = time.perf_counter() - t
+        cpu = {"value": round(size / dt / 1e9, 3), "unit": "GB/s", "cores": ncores, "kind": "port",
+               "sample": f"whole {size / 2**30:.0f} GiB file: pread 4 MiB pieces of each 32 MiB range + FNV-1a-64 per range",
+               "pread_only_GBps": round(size / dt_copy / 1e9, 3)}
+        log(f"CPU path: pread+FNV {cpu['value']} GB/s, pread only {cpu['pread_only_GBps']} GB/s on {ncores} threads")
+
+    # ---- cold passes: empty page cache, every page crosses the link
+    cold = []
+    for i in range(args.cold_steps):
+        ctx.cache_drop()
+        ctx.stats_reset()
+        cold.append(timed_pass())
+        o = out.cpu().numpy()
+        assert (o == rng_bytes).all(), o[:8]
+        s = ctx.stats()
+        log(f"cold pass {i}: {size / cold[-1] / 1e9:.1f} GB/s  misses={s['page_misses']} hits={s['page_hits']}")
+    cold_ok = None
+    if cpu_fnv is not None:
+        cold_ok = bool((gpu_fnv() == cpu_fnv).all())
+        assert cold_ok, "cold gread output differs from the file"
+
+    # ---- warm passes
+    ctx.stats_reset()
+    for _ in range(args.warmup):
+        timed_pass()
+    torch.cuda.synchronize()
+    kt = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        kt.append(timed_pass())
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    s = ctx.stats()
+    assert s["page_misses"] == 0, s
+    warm_ok = None
+    if cpu_fnv is not None:
+        warm_ok = bool((gpu_fnv() == cpu_fnv).all())
+        assert warm_ok, "warm gread output differs from the file"
+    ms_per_step = wall / args.steps * 1e3
+    value = size / (wall / args.steps) / 1e9
+    avg_k = sum(kt) / len(kt)
+    achieved = 2 * size / avg_k / 1e9
+    cold_best = size / min(cold) / 1e9 if cold else None
+    log(f"warm: {value:.1f} GB/s payload (kernel avg {avg_k * 1e3:.3f} ms, {achieved:.0f} GB/s HBM moved)")
+
+    grep_res = None
+    if not args.no_grep:
+        grep_res = bench_grep(args, ctx, torch, L, _lib, dg)
+
+    ctx.host_stop()
+    ctx.close()
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded uniform random bytes; seeded Zipf text for grep)",
+        "config": {"workload": "sequential gread: 1 file, 256 workgroups x 32 MiB ranges, 256 KiB pages, 16 GiB page cache, warm",
+                   "file_bytes": size, "workgroups": n, "range_bytes": rng_bytes, "page_bytes": args.page_kib << 10,
+                   "cache_bytes": int(args.cache_gib * (1 << 30)), "threads_per_workgroup": args.threads},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_gread_ranges", "kernel_avg_ms": round(avg_k * 1e3, 4),
+                     "algorithmic_bytes_per_launch": 2 * size},
+        "cold": {"gread_GBps_best": round(cold_best, 2) if cold_best else None,
+                 "gread_GBps_each": [round(size / c / 1e9, 2) for c in cold],
+                 "link_peak_GBps": round(link, 2),
+                 "frac_of_link": round(cold_best / link, 4) if cold_best else None},
+        "bit_exact": {"cold_fnv": cold_ok, "warm_fnv": warm_ok},
+        "cpu_baseline": cpu,
+        "grep": grep_res,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_grep(args, ctx, torch, L, _lib, dg):
+    """64 text files x 4 MiB, 1024 query words, 64 KiB windows; cold and warm."""
+    nfiles, fsize = 64, 4 << 20
+    voc = dg.vocab(50000, seed=dg.SEED_VOCAB)
+    qi = dg.query(50000, 1024, seed=dg.SEED_QUERY)
+    q = np.ascontiguousarray(voc[qi])
+    gdir = os.path.join(args.workdir, "grep")
+    os.makedirs(gdir, exist_ok=True)
+    paths = [os.path.join(gdir, f"text_{i:03d}.txt") for i in range(nfiles)]
+    if not all(os.path.exists(p) and os.path.getsize(p) == fsize for p in paths):
+        dg.text_files(paths, [fsize] * nfiles, dg.SEED_C0, voc, threads=16)
+    names = np.zeros((nfiles, 256), dtype=np.uint8)
+    for i, p in enumerate(paths):
+        b = os.fsencode(p)
+        names[i, :len(b)] = np.frombuffer(b, dtype=np.uint8)
+    names_d = torch.from_numpy(names).cuda()
+    dict_d = torch.from_numpy(q.reshape(-1)).cuda()
+    counts = torch.zeros(nfiles * 1024, dtype=torch.int32, device="cuda")
+    status = torch.zeros(nfiles, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    torch.cuda.synchronize()
+
+    def run():
+        counts.zero_()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        rc = L.gpufs_launch_grep(ctx.h, ctypes.c_void_p(names_d.data_ptr()), 256, nfiles,
+                                 ctypes.c_void_p(dict_d.data_ptr()), 1024, 65536,
+                                 ctypes.c_void_p(counts.data_ptr()), ctypes.c_void_p(status.data_ptr()), 1024, None)
+        _lib.check("gpufs_launch_grep", rc)
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / 1e3
+
+    ctx.cache_drop()
+    t_cold = run()
+    c_cold = counts.cpu().numpy().view(np.uint32).reshape(nfiles, 1024).copy()
+    assert (status.cpu().numpy() == 0).all()
+    t_warm = min(run() for _ in range(5))
+    c_warm = counts.cpu().numpy().view(np.uint32).reshape(nfiles, 1024)
+    total = nfiles * fsize
+    res = {"files": nfiles, "file_bytes": fsize, "words": 1024, "window": 65536,
+           "cold_GBps": round(total / t_cold / 1e9, 2), "warm_GBps": round(total / t_warm / 1e9, 2),
+           "warm_ms": round(t_warm * 1e3, 3)}
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        ncores = min(16, os.cpu_count() or 1)
+        t = time.perf_counter()
+        cpu, st = orc.scan_files(paths, q, 65536, threads=ncores)
+        dt = time.perf_counter() - t
+        res["bit_exact"] = bool((cpu == c_cold).all() and (cpu == c_warm).all() and (st == 0).all())
+        assert res["bit_exact"], "grep counts differ from the CPU path"
+        res["cpu_baseline"] = {"value": round(total / dt / 1e9, 3), "unit": "GB/s", "cores": ncores, "kind": "port"}
+    log(f"grep: cold {res['cold_GBps']} GB/s, warm {res['warm_GBps']} GB/s")
+    return res
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1278 @@
+// gpufs.h -- device-side file API for MI355X (gfx950), header-only HIP.
+//
+// Same calls, signatures and flag values as the reference's device API
+// (libgpufs/fs_calls.cu.h:41-65): gopen, gclose, gread, gwrite, gmmap, gmunmap,
+// gfsync, gftruncate, gmsync, fstat.  Every call is workgroup-collective: all
+// threads of the workgroup call it with the same arguments (the contract of
+// libgpufs/fs_calls.cu:685-687); calling it from a divergent subset of threads
+// is undefined.  fstat is an ordinary per-thread read.
+//
+// Design (not a translation of the reference):
+//  * page-cache index: one lock-free radix tree per open file, fan-out 64.  A
+//    batch of up to 64 consecutive pages is resolved by wave 0 in one step:
+//    lane i owns page i, the 64 leaf slots of a node are one coalesced 512-byte
+//    load, and hits / misses / in-flight fills are separated with ballots.
+//  * frames: a 64-bit state word per frame (pin count, state, dirty flag,
+//    generation).  Leaf slots hold (frame, generation); a recycled frame has a
+//    new generation, so a stale leaf is detected by the pinning atomic itself
+//    (no ABA, no lock held across a fill).
+//  * allocation: never-used frames first, then a wave-wide CLOCK sweep (64
+//    frames per atomic on the hand) that evicts clean, unpinned frames with a
+//    second-chance reference bit.
+//  * host RPC: slots claimed from a device bitmap, requests published into a
+//    submission queue in coherent pinned host memory with system-scope release
+//    stores; the host service pread()s into pinned staging and bumps a per-batch
+//    completion counter; the workgroup copies staging into HBM frames with
+//    16-byte loads and stores.
+//
+// Workgroups must have at least 64 threads (one full wave64): wave 0 resolves
+// up to 64 pages at a time, one page per lane.
+//
+// Errors: calls returning int return a negative code; gread/gwrite keep the
+// reference's size_t return type and report errors as (size_t)(-errno), so
+// `(ssize_t)ret < 0` tests for failure.  gread returns the bytes actually read
+// (short at end of file); the reference always returns `size`
+// (libgpufs/fs_calls.cu:720-736).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <errno.h>
+#include <stdint.h>
+#include "gfs_layout.h"
+#include "../../include/gpufs_amd.h"
+
+typedef unsigned char uchar;
+
+// Device state of the bound context, one copy per translation unit (no
+// relocatable device code is needed); gpufs_bind() sets it from the host.
+static __device__ gfs_dev* gfs_g;
+
+namespace gfs {
+
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) v4u gv4u;
+typedef __attribute__((address_space(3))) v4u lv4u;
+
+#define GFS_NOFRAME 0xFFFFFFFFu
+#define GFS_K_NONE 0
+#define GFS_K_HIT 1
+#define GFS_K_OWN 2
+#define GFS_K_WAIT 3
+#define GFS_K_MISS 4
+#define GFS_K_ERR 5
+#define GFS_K_DONE 6
+
+// Per-workgroup control block shared by all API calls (calls are collective
+// and sequential within a workgroup, so one block suffices).
+struct wg_ctl {
+    uint32_t frame[64];
+    uint32_t slot[64];
+    uint8_t  kind[64];
+    int64_t  ret;
+    int32_t  err;
+    uint32_t n_own;
+    uint32_t cq;
+    uint32_t flag;
+    uint64_t a, b;
+    uint32_t rot;
+};
+
+__device__ __forceinline__ uint32_t tid() {
+    return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+}
+__device__ __forceinline__ uint32_t nthreads() { return blockDim.x * blockDim.y * blockDim.z; }
+__device__ __forceinline__ uint32_t lane() { return __lane_id(); }
+__device__ __forceinline__ uint32_t wave() { return tid() >> 6; }
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+__device__ __forceinline__ uint32_t popc(uint64_t m) { return __popcll(m); }
+__device__ __forceinline__ uint32_t first_lane(uint64_t m) { return __ffsll((unsigned long long)m) - 1; }
+__device__ __forceinline__ uint32_t nth_set(uint64_t m, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) m &= m - 1;
+    return first_lane(m);
+}
+
+// ---- scoped atomics: agent scope for HBM state, system scope for host memory
+// (the value parameter is a non-deduced context so literals convert to T)
+template <class T> struct ident { typedef T type; };
+template <class T> __device__ __forceinline__ T ld_acq(T* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ T ld_rlx(T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ void st_rel(T* p, typename ident<T>::type v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ void st_rlx(T* p, typename ident<T>::type v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ T fadd_rlx(T* p, typename ident<T>::type v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ T fadd(T* p, typename ident<T>::type v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ bool cas(T* p, T& expected, typename ident<T>::type desired) {
+    return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_ACQ_REL,
+                                                __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ T sys_ld(T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+template <class T> __device__ __forceinline__ T sys_ld_acq(T* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+template <class T> __device__ __forceinline__ void sys_st(T* p, typename ident<T>::type v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+template <class T> __device__ __forceinline__ void sys_st_rel(T* p, typename ident<T>::type v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void fence_acq_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+__device__ __forceinline__ void fence_rel_agent() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+__device__ __forceinline__ void fence_acq_sys() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, ""); }
+__device__ __forceinline__ void fence_rel_sys() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, ""); }
+__device__ __forceinline__ void backoff() { __builtin_amdgcn_s_sleep(16); }
+__device__ __forceinline__ uint64_t now_ticks() { return wall_clock64(); }
+
+__device__ __forceinline__ uint32_t meta_state(uint64_t m) { return (uint32_t)((m >> GFS_ST_SHIFT) & 0xF); }
+__device__ __forceinline__ uint32_t meta_gen(uint64_t m) { return (uint32_t)(m >> GFS_GEN_SHIFT); }
+__device__ __forceinline__ uint32_t meta_ref(uint64_t m) { return (uint32_t)(m & GFS_REF_MASK); }
+__device__ __forceinline__ uint64_t leaf_pack(uint32_t f, uint32_t gen) { return ((uint64_t)gen << 32) | (uint64_t)(f + 1u); }
+__device__ __forceinline__ uint32_t leaf_frame(uint64_t v) { return (uint32_t)(v & 0xFFFFFFFFu) - 1u; }
+__device__ __forceinline__ uint32_t leaf_gen(uint64_t v) { return (uint32_t)(v >> 32); }
+
+__device__ __forceinline__ uint32_t fst_state(uint64_t s) { return (uint32_t)(s & 0xFF); }
+__device__ __forceinline__ uint32_t fst_opens(uint64_t s) { return (uint32_t)((s >> 8) & 0xFFFFFF); }
+__device__ __forceinline__ uint32_t fst_version(uint64_t s) { return (uint32_t)(s >> 32); }
+__device__ __forceinline__ uint64_t fst_make(uint32_t state, uint32_t opens, uint32_t version) {
+    return ((uint64_t)version << 32) | ((uint64_t)(opens & 0xFFFFFF) << 8) | state;
+}
+
+__device__ __forceinline__ void count(unsigned long long* c, unsigned long long v) {
+    fadd_rlx(c, v);
+}
+// Wave-aggregated counter bump: one atomic per wave (call with the whole wave).
+__device__ __forceinline__ void wave_count(unsigned long long* c, bool pred) {
+    uint64_t m = __ballot(pred);
+    if (m && lane() == first_lane(m)) fadd_rlx(c, (unsigned long long)popc(m));
+}
+
+// ======================================================================
+// Copy engine: workgroup-cooperative byte copies with 16-byte vector bodies.
+// `j` is the thread's rank in [0,T).  Handles any alignment: equal 16-byte
+// phase -> dwordx4; otherwise dst is aligned to 16 and each 16-byte unit is
+// assembled from five aligned source dwords with v_alignbyte.  Every source
+// dword loaded holds at least one byte of the segment.
+// ======================================================================
+// True when a generic pointer points into LDS.
+__device__ __forceinline__ bool is_lds(const void* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_is_shared(p);
+#else
+    (void)p;
+    return false;
+#endif
+}
+__device__ __forceinline__ v4u load16(const uint8_t* p, bool src_lds, bool nt) {
+    if (src_lds) return *(const lv4u*)(p);
+    if (nt) return __builtin_nontemporal_load((const gv4u*)p);
+    return *(const gv4u*)p;
+}
+__device__ __forceinline__ void store16(uint8_t* p, v4u v, bool dst_lds, bool nt) {
+    if (dst_lds) { *(lv4u*)p = v; return; }
+    if (nt) { __builtin_nontemporal_store(v, (gv4u*)p); return; }
+    *(gv4u*)p = v;
+}
+
+__device__ inline void copy_seg(uint8_t* dst, const uint8_t* src, uint64_t len, uint32_t j,
+                                uint32_t T, bool nt_load, bool nt_store) {
+    if (len == 0) return;
+    const bool dl = is_lds(dst);
+    const bool sl = is_lds(src);
+    const uintptr_t d = (uintptr_t)dst, s = (uintptr_t)src;
+    uint64_t head = (16 - (d & 15)) & 15;
+    if (head > len) head = len;
+    for (uint64_t k = j; k < head; k += T) dst[k] = src[k];
+    const uint64_t rest = len - head;
+    const uint64_t units = rest >> 4;
+    const uint64_t tail = rest & 15;
+    uint8_t* D = dst + head;
+    const uint8_t* S = src + head;
+    for (uint64_t k = j; k < tail; k += T) D[units * 16 + k] = S[units * 16 + k];
+    if (units == 0) return;
+    if (((s ^ d) & 15) == 0) {
+        uint64_t u = j;
+        const uint64_t st = T;
+        for (; u + 3 * st < units; u += 4 * st) {
+            v4u a = load16(S + 16 * u, sl, nt_load);
+            v4u b = load16(S + 16 * (u + st), sl, nt_load);
+            v4u c = load16(S + 16 * (u + 2 * st), sl, nt_load);
+            v4u e = load16(S + 16 * (u + 3 * st), sl, nt_load);
+            store16(D + 16 * u, a, dl, nt_store);
+            store16(D + 16 * (u + st), b, dl, nt_store);
+            store16(D + 16 * (u + 2 * st), c, dl, nt_store);
+            store16(D + 16 * (u + 3 * st), e, dl, nt_store);
+        }
+        for (; u < units; u += st) store16(D + 16 * u, load16(S + 16 * u, sl, nt_load), dl, nt_store);
+        return;
+    }
+    // misaligned source: S is shifted by sh bytes from a dword boundary
+    const uint32_t sh = (uint32_t)((uintptr_t)S & 3);
+    const uint32_t* W = (const uint32_t*)((uintptr_t)S - sh);
+    for (uint64_t u = j; u < units; u += T) {
+        const uint32_t* w = W + 4 * u;
+        v4u o;
+        if (sh == 0) {
+            o.x = w[0]; o.y = w[1]; o.z = w[2]; o.w = w[3];
+        } else {
+            uint32_t w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4];
+            o.x = __builtin_amdgcn_alignbyte(w1, w0, sh);
+            o.y = __builtin_amdgcn_alignbyte(w2, w1, sh);
+            o.z = __builtin_amdgcn_alignbyte(w3, w2, sh);
+            o.w = __builtin_amdgcn_alignbyte(w4, w3, sh);
+        }
+        store16(D + 16 * u, o, dl, nt_store);
+    }
+}
+
+// Whole-workgroup copy of one buffer.
+__device__ inline void wg_copy(uint8_t* dst, const uint8_t* src, uint64_t len, bool nt_load = false,
+                               bool nt_store = false) {
+    copy_seg(dst, src, len, tid(), nthreads(), nt_load, nt_store);
+}
+
+// ======================================================================
+// Radix tree (per file, fan-out 64).  root word = node | height << 40.
+// Node n occupies g->nodes[n*64 .. n*64+63]; node 0 is the null node.
+// ======================================================================
+__device__ __forceinline__ uint64_t tree_cap(uint32_t h) {
+    return h >= GFS_MAX_HEIGHT ? ~0ull : (1ull << (GFS_FANOUT_LOG * h));
+}
+__device__ inline uint64_t node_alloc(gfs_dev* g) {
+    unsigned long long n = fadd_rlx(&g->node_next, 1ull);
+    if (n >= g->node_cap) return 0;
+    return n;
+}
+
+// Returns the node index of the leaf covering pages [leaf*64, leaf*64+64), creating
+// interior nodes on the way when `create`.  0 = absent (or -ENOMEM in *err).
+__device__ inline uint64_t tree_leaf(gfs_dev* g, gfs_file* f, uint64_t leaf, bool create, int* err) {
+    const uint64_t page = leaf << GFS_FANOUT_LOG;
+    uint64_t rw = ld_acq(&f->root);
+    for (int guard = 0; guard < 16; ++guard) {
+        uint32_t h = (uint32_t)(rw >> 40);
+        if (rw != 0 && page < tree_cap(h)) break;
+        if (!create) return 0;
+        uint64_t n = node_alloc(g);
+        if (!n) { *err = -ENOMEM; return 0; }
+        uint64_t nw;
+        if (rw == 0) {
+            uint32_t nh = 1;
+            while (page >= tree_cap(nh)) ++nh;
+            nw = n | ((uint64_t)nh << 40);
+        } else {
+            st_rlx(&g->nodes[n * GFS_FANOUT], rw & ((1ull << 40) - 1));
+            nw = n | ((uint64_t)(h + 1) << 40);
+        }
+        uint64_t exp = rw;
+        if (cas(&f->root, exp, nw)) rw = nw;
+        else rw = exp;   // lost the race; the unused node stays zero
+    }
+    uint32_t h = (uint32_t)(rw >> 40);
+    uint64_t node = rw & ((1ull << 40) - 1);
+    for (int lvl = (int)h - 1; lvl >= 1; --lvl) {
+        uint64_t* sp = &g->nodes[node * GFS_FANOUT + ((page >> (GFS_FANOUT_LOG * lvl)) & (GFS_FANOUT - 1))];
+        uint64_t c = ld_acq(sp);
+        if (!c) {
+            if (!create) return 0;
+            uint64_t n = node_alloc(g);
+            if (!n) { *err = -ENOMEM; return 0; }
+            uint64_t exp = 0;
+            c = cas(sp, exp, n) ? n : exp;
+        }
+        node = c;
+    }
+    return node;
+}
+
+// ======================================================================
+// Frame allocation (wave-wide).  Lanes with `need` receive a frame whose meta
+// word is (new generation | FILLING | pin 1), or GFS_NOFRAME.
+// ======================================================================
+__device__ inline uint32_t frame_alloc_wave(gfs_dev* g, bool need, uint32_t sweeps) {
+    uint32_t my = GFS_NOFRAME;
+    uint64_t needm = __ballot(need);
+    if (!needm) return my;
+    const uint32_t ln = lane();
+    const uint32_t N = g->n_frames;
+    // never-used frames
+    {
+        uint32_t k = popc(needm);
+        uint32_t leader = first_lane(needm);
+        unsigned long long base = 0;
+        if (ln == leader) base = fadd_rlx(&g->fresh_next, (unsigned long long)k);
+        base = __shfl(base, leader);
+        if (need) {
+            unsigned long long f = base + mbcnt(needm);
+            if (f < N) {
+                my = (uint32_t)f;
+                st_rlx(&g->meta[my], (GFS_F_FILLING << GFS_ST_SHIFT) | 1ull);
+            }
+        }
+        needm = __ballot(need && my == GFS_NOFRAME);
+    }
+    // CLOCK sweep with second chance
+    const uint32_t rounds = sweeps * ((N + 63) / 64) + 1;
+    for (uint32_t r = 0; needm && r < rounds; ++r) {
+        unsigned long long h = 0;
+        if (ln == first_lane(needm)) h = fadd_rlx(&g->clock_hand, 64ull);
+        h = __shfl(h, first_lane(needm));
+        uint32_t f = (uint32_t)((h + ln) % N);
+        uint64_t m = ld_rlx(&g->meta[f]);
+        uint32_t st = meta_state(m);
+        bool cand = meta_ref(m) == 0 && !(m & GFS_FL_DIRTY) && (st == GFS_F_FREE || st == GFS_F_VALID);
+        if (cand && st == GFS_F_VALID && g->fref[f]) { g->fref[f] = 0; cand = false; }
+        uint64_t candm = __ballot(cand);
+        cand = cand && mbcnt(candm) < popc(needm);
+        bool got = false;
+        if (cand) {
+            uint64_t nm = ((uint64_t)(meta_gen(m) + 1u) << GFS_GEN_SHIFT) | (GFS_F_FILLING << GFS_ST_SHIFT) | 1ull;
+            uint64_t exp = m;
+            got = cas(&g->meta[f], exp, nm);
+            if (got && st == GFS_F_VALID) count(&g->ctr.evictions, 1);
+        }
+        uint64_t gotm = __ballot(got);
+        uint32_t rank = mbcnt(needm);
+        uint32_t src = 0;
+        bool take = need && my == GFS_NOFRAME && rank < popc(gotm);
+        if (take) src = nth_set(gotm, rank);
+        uint32_t fv = __shfl(f, src);
+        if (take) my = fv;
+        needm = __ballot(need && my == GFS_NOFRAME);
+        if (needm && r > 0 && (r % ((N + 63) / 64)) == 0) backoff();
+    }
+    if (need && my == GFS_NOFRAME) count(&g->ctr.alloc_fail, 1);
+    return my;
+}
+
+// Give back a frame that was never published (or whose fill failed): bump the
+// generation, state -> FREE, drop the owner's pin.  Transient pins of racing
+// readers survive the add and are undone by them.
+__device__ __forceinline__ void frame_release(gfs_dev* g, uint32_t f) {
+    const uint64_t delta = (1ull << GFS_GEN_SHIFT) - (GFS_F_FILLING << GFS_ST_SHIFT) - 1ull;
+    fadd(&g->meta[f], delta);
+}
+
+// ======================================================================
+// RPC client
+// ======================================================================
+__device__ inline int rpc_claim(gfs_dev* g, bool need, uint32_t hint) {
+    if (!need) return -1;
+    const uint32_t W = g->slot_words;
+    for (uint32_t a = 0; a < W; ++a) {
+        uint32_t w = (hint + a) % W;
+        uint64_t x = ld_rlx(&g->slot_free[w]);
+        while (x) {
+            uint32_t b = first_lane(x);
+            uint64_t bit = 1ull << b;
+            uint64_t old = __hip_atomic_fetch_and(&g->slot_free[w], ~bit, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (old & bit) return (int)(w * 64 + b);
+            x = old & ~bit;
+        }
+    }
+    return -1;
+}
+__device__ __forceinline__ void rpc_release(gfs_dev* g, int s) {
+    __hip_atomic_fetch_or(&g->slot_free[s >> 6], 1ull << (s & 63), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Fill the request fields of slot s and publish it to the host.
+__device__ inline void rpc_post(gfs_dev* g, int s, uint32_t op, int32_t host_fd, int32_t flags,
+                                uint64_t off, uint64_t len, uint32_t cq) {
+    gfs_rpc_slot* r = &g->slots[s];
+    sys_st(&r->op, op);
+    sys_st(&r->host_fd, host_fd);
+    sys_st(&r->flags, flags);
+    sys_st(&r->cq, cq);
+    sys_st(&r->offset, off);
+    sys_st(&r->length, len);
+    sys_st(&r->result, (int64_t)INT64_MIN);
+    fence_rel_sys();
+    unsigned long long t = fadd_rlx(&g->sq_tail, 1ull);
+    sys_st_rel(&g->sq[t & (g->sq_depth - 1)], (uint64_t)((t << 16) | (uint64_t)s));
+}
+// Wait until completion counter `cq` reaches n.  Returns 0 or -ETIMEDOUT.
+__device__ inline int rpc_wait(gfs_dev* g, uint32_t cq, uint32_t n) {
+    uint64_t t0 = now_ticks();
+    uint32_t spins = 0;
+    while (sys_ld_acq(&g->cq[cq]) < n) {
+        if ((++spins & 255) == 0 && now_ticks() - t0 > g->wait_ticks) return -ETIMEDOUT;
+        backoff();
+    }
+    fence_acq_sys();
+    return 0;
+}
+// One synchronous request from a single thread (open / close / truncate / write).
+__device__ inline int64_t rpc_call_single(gfs_dev* g, uint32_t op, int32_t host_fd, int32_t flags,
+                                          uint64_t off, uint64_t len, const char* name, int* slot_out) {
+    int s = -1;
+    uint64_t t0 = now_ticks();
+    uint32_t hint = blockIdx.x * 7u + blockIdx.y * 13u;
+    while ((s = rpc_claim(g, true, hint)) < 0) {
+        if (now_ticks() - t0 > g->wait_ticks) return -ETIMEDOUT;
+        backoff();
+    }
+    if (name) {
+        gfs_rpc_slot* r = &g->slots[s];
+        for (int i = 0; i < GFS_NAME_MAX; i += 8) {
+            uint64_t w = 0;
+            for (int b = 0; b < 8; ++b) w |= (uint64_t)(uint8_t)name[i + b] << (8 * b);
+            sys_st((uint64_t*)(r->name + i), w);
+        }
+    }
+    sys_st(&g->cq[s], 0u);
+    rpc_post(g, s, op, host_fd, flags, off, len, (uint32_t)s);
+    int w = rpc_wait(g, (uint32_t)s, 1);
+    int64_t res = w ? (int64_t)w : sys_ld(&g->slots[s].result);
+    if (slot_out) *slot_out = s;   // caller reads extra results, then releases
+    else rpc_release(g, s);
+    return res;
+}
+
+// ======================================================================
+// File table
+// ======================================================================
+__device__ __forceinline__ bool fd_valid(gfs_dev* g, int fd) {
+    return fd >= 0 && (unsigned long long)fd < ld_rlx(&g->files_next) && (uint32_t)fd < g->max_files;
+}
+__device__ inline uint32_t name_hash(const char* s, int* len) {
+    uint32_t h = 2166136261u;
+    int i = 0;
+    for (; i < GFS_NAME_MAX && s[i]; ++i) { h ^= (uint8_t)s[i]; h *= 16777619u; }
+    *len = i;
+    return h;
+}
+__device__ inline bool name_eq(const char* a, const char* b) {
+    for (int i = 0; i < GFS_NAME_MAX; ++i) {
+        if (a[i] != b[i]) return false;
+        if (!a[i]) return true;
+    }
+    return true;
+}
+// Host open of entry e (new or reopened).  Fills host_fd/size/stat fields.
+__device__ inline int open_host(gfs_dev* g, gfs_file* e, int flags, bool reopen) {
+    int s = -1;
+    int64_t r = rpc_call_single(g, GFS_OP_OPEN, -1, flags, 0, 0, e->name, &s);
+    count(&g->ctr.rpc_opens, 1);
+    if (s < 0) return (int)r;
+    gfs_rpc_slot* sl = &g->slots[s];
+    uint64_t size = sys_ld(&sl->size), ct = sys_ld(&sl->ctime_ns), mt = sys_ld(&sl->mtime_ns), ino = sys_ld(&sl->ino);
+    rpc_release(g, s);
+    if (r < 0) return (int)r;
+    uint64_t st = ld_rlx(&e->st);
+    uint32_t version = fst_version(st);
+    if (reopen && (e->ctime_ns != ct || e->mtime_ns != mt || e->size != size || e->ino != ino)) {
+        // the file changed on the host: drop every cached page of the old version
+        ++version;
+        st_rlx(&e->root, 0ull);
+    }
+    e->host_fd = (int32_t)r;
+    e->flags = flags;
+    e->size = size;
+    e->ctime_ns = ct;
+    e->mtime_ns = mt;
+    e->ino = ino;
+    e->dirty = 0;
+    st_rel(&e->st, fst_make(GFS_FS_OPEN, 1, version));
+    return 0;
+}
+
+// Single-thread open (reference: single_thread_open, libgpufs/fs_calls.cu:152-225).
+__device__ inline int open_single(const char* name, int flags) {
+    gfs_dev* g = gfs_g;
+    int len = 0;
+    uint32_t h = name_hash(name, &len);
+    if (len == 0) return -ENOENT;
+    if (len >= GFS_NAME_MAX) return -ENAMETOOLONG;
+    const uint32_t mask = g->name_cap - 1;
+    uint32_t pos = h & mask;
+    int idx = -1;
+    int mine = -1;
+    for (uint32_t probe = 0; probe <= mask; ++probe, pos = (pos + 1) & mask) {
+        uint32_t v = ld_acq(&g->name_index[pos]);
+        if (v == 0) {
+            if (mine < 0) {
+                unsigned long long m = fadd_rlx(&g->files_next, 1ull);
+                if (m >= g->max_files) return E_FSTABLE_FULL;
+                mine = (int)m;
+                gfs_file* e = &g->files[mine];
+                for (int i = 0; i < GFS_NAME_MAX; ++i) e->name[i] = i <= len ? name[i] : 0;
+                e->name[len] = 0;
+                e->name_hash = h;
+                e->root = 0;
+                e->host_fd = -1;
+                st_rlx(&e->st, fst_make(GFS_FS_OPENING, 1, 0));
+            }
+            uint32_t exp = 0;
+            if (cas(&g->name_index[pos], exp, (uint32_t)mine + 1u)) {
+                int r = open_host(g, &g->files[mine], flags, false);
+                if (r < 0) {
+                    st_rel(&g->files[mine].st, fst_make(GFS_FS_CLOSED, 0, 0));
+                    return E_IPC_OPEN_ERROR;
+                }
+                return mine;
+            }
+            v = exp;
+        }
+        gfs_file* e = &g->files[v - 1];
+        if (e->name_hash == h && name_eq(e->name, name)) { idx = (int)v - 1; break; }
+    }
+    if (mine >= 0 && idx != mine) st_rel(&g->files[mine].st, fst_make(GFS_FS_DEAD, 0, 0));
+    if (idx < 0) return E_FSTABLE_FULL;
+    gfs_file* e = &g->files[idx];
+    uint64_t t0 = now_ticks();
+    for (;;) {
+        uint64_t st = ld_acq(&e->st);
+        uint32_t s = fst_state(st);
+        if (s == GFS_FS_OPEN) {
+            uint64_t exp = st;
+            if (cas(&e->st, exp, fst_make(GFS_FS_OPEN, fst_opens(st) + 1, fst_version(st)))) return idx;
+            continue;
+        }
+        if (s == GFS_FS_CLOSED) {
+            uint64_t exp = st;
+            if (cas(&e->st, exp, fst_make(GFS_FS_OPENING, 1, fst_version(st)))) {
+                int r = open_host(g, e, flags, true);
+                if (r < 0) {
+                    st_rel(&e->st, fst_make(GFS_FS_CLOSED, 0, fst_version(st)));
+                    return E_IPC_OPEN_ERROR;
+                }
+                return idx;
+            }
+            continue;
+        }
+        if (now_ticks() - t0 > g->wait_ticks) return -ETIMEDOUT;
+        backoff();
+    }
+}
+
+} // namespace gfs
+
+static __shared__ gfs::wg_ctl gfs_ctl;
+
+namespace gfs {
+
+// ======================================================================
+// Page resolution for a batch of up to 64 consecutive pages (wave 0 only).
+// On return gfs_ctl.kind[i] is HIT (pinned VALID frame), OWN (pinned FILLING
+// frame this workgroup must fill; ring slot in slot[i]), WAIT (another
+// workgroup is filling it, or no slot/frame was free) or DONE (not pending).
+// `zero_fill_ok`: pages that will be fully overwritten or created zeroed do not
+// need host data; they come back OWN with slot[i] = 0xFFFFFFFF.
+// ======================================================================
+__device__ inline void resolve_batch(gfs_dev* g, int fd, uint64_t pb, uint32_t cnt, uint64_t pending,
+                                     uint64_t nohost_mask, bool for_write) {
+    const uint32_t ln = lane();
+    gfs_file* file = &g->files[fd];
+    const bool mine = ((pending >> ln) & 1ull) != 0 && ln < cnt;
+    const uint64_t page = pb + ln;
+    // leaf nodes: at most two distinct leaves in a 64-page window
+    uint64_t leaf0 = pb >> GFS_FANOUT_LOG, leaf1 = (pb + (cnt ? cnt - 1 : 0)) >> GFS_FANOUT_LOG;
+    uint64_t n0 = 0, n1 = 0;
+    int err = 0;
+    if (ln == 0) {
+        n0 = tree_leaf(g, file, leaf0, true, &err);
+        n1 = (leaf1 == leaf0) ? n0 : tree_leaf(g, file, leaf1, true, &err);
+    }
+    n0 = __shfl(n0, 0);
+    n1 = __shfl(n1, 0);
+    err = __shfl(err, 0);
+    if (err) {
+        if (ln == 0) gfs_ctl.err = err;
+        if (ln < 64) gfs_ctl.kind[ln] = GFS_K_DONE;
+        return;
+    }
+    uint64_t node = ((page >> GFS_FANOUT_LOG) == leaf0) ? n0 : n1;
+    uint64_t* sp = &g->nodes[node * GFS_FANOUT + (page & (GFS_FANOUT - 1))];
+    uint32_t kind = GFS_K_DONE, f = GFS_NOFRAME;
+    uint64_t v = 0;
+    if (mine) {
+        v = ld_rlx(sp);
+        kind = GFS_K_MISS;
+        if (v) {
+            uint32_t vf = leaf_frame(v), vg = leaf_gen(v);
+            uint64_t old = fadd_rlx(&g->meta[vf], 1ull);
+            if (meta_gen(old) == vg && meta_state(old) == GFS_F_VALID) {
+                kind = GFS_K_HIT;
+                f = vf;
+                if (!g->fref[vf]) g->fref[vf] = 1;
+                if (for_write) __hip_atomic_fetch_or(&g->meta[vf], GFS_FL_DIRTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                fadd_rlx(&g->meta[vf], ~0ull);
+                if (meta_gen(old) == vg && meta_state(old) == GFS_F_FILLING) kind = GFS_K_WAIT;
+            }
+        }
+    }
+    // misses that need host data claim a ring slot first
+    const bool nohost = ((nohost_mask >> ln) & 1ull) != 0;
+    int s = rpc_claim(g, kind == GFS_K_MISS && !nohost, blockIdx.x * 5u + ln);
+    bool need_frame = kind == GFS_K_MISS && (nohost || s >= 0);
+    if (kind == GFS_K_MISS && !need_frame) kind = GFS_K_WAIT;
+    uint32_t nf = frame_alloc_wave(g, need_frame, 2);
+    if (need_frame) {
+        if (nf == GFS_NOFRAME) {
+            if (s >= 0) rpc_release(g, s);
+            s = -1;
+            kind = GFS_K_WAIT;
+        } else {
+            g->fkey[nf] = gfs_key((uint32_t)fd, page);
+            g->fbytes[nf] = 0;
+            uint32_t gen = meta_gen(ld_rlx(&g->meta[nf]));
+            uint64_t exp = v;
+            if (cas(sp, exp, leaf_pack(nf, gen))) {
+                kind = GFS_K_OWN;
+                f = nf;
+                if (!g->fref[nf]) g->fref[nf] = 1;
+            } else {
+                frame_release(g, nf);
+                if (s >= 0) rpc_release(g, s);
+                s = -1;
+                kind = GFS_K_WAIT;
+            }
+        }
+    }
+    wave_count(&g->ctr.page_waits, kind == GFS_K_WAIT && mine);
+    wave_count(&g->ctr.page_hits, kind == GFS_K_HIT);
+    wave_count(&g->ctr.page_misses, kind == GFS_K_OWN);
+    fence_acq_agent();
+    if (ln < 64) {
+        gfs_ctl.kind[ln] = (uint8_t)kind;
+        gfs_ctl.frame[ln] = f;
+        gfs_ctl.slot[ln] = (kind == GFS_K_OWN && s >= 0) ? (uint32_t)s : 0xFFFFFFFFu;
+    }
+    // publish read requests for OWN pages that need host data
+    bool post = kind == GFS_K_OWN && s >= 0;
+    uint64_t postm = __ballot(post);
+    uint32_t cq = 0;
+    if (postm) {
+        uint32_t leader = first_lane(postm);
+        cq = (uint32_t)__shfl(s, leader);
+        if (ln == leader) sys_st(&g->cq[cq], 0u);
+        if (post) {
+            uint64_t poff = page << g->log_page;
+            uint64_t plen = g->page_size;
+            if (poff + plen > file->size) plen = file->size > poff ? file->size - poff : 0;
+            rpc_post(g, s, GFS_OP_READ, file->host_fd, 0, poff, plen, cq);
+        }
+        wave_count(&g->ctr.rpc_reads, post);
+    }
+    if (ln == 0) { gfs_ctl.n_own = popc(postm); gfs_ctl.cq = cq; }
+}
+
+// Drop the pins taken by resolve_batch; OWN frames are published VALID first
+// (or released when their fill failed).
+__device__ inline void finish_batch(gfs_dev* g, uint32_t cnt) {
+    const uint32_t ln = lane();
+    if (ln >= cnt) return;
+    uint32_t k = gfs_ctl.kind[ln];
+    uint32_t f = gfs_ctl.frame[ln];
+    if (k == GFS_K_HIT) {
+        fadd(&g->meta[f], ~0ull);
+    } else if (k == GFS_K_OWN) {
+        // FILLING -> VALID and drop the owner's pin in one release add
+        fadd(&g->meta[f], (GFS_F_VALID - GFS_F_FILLING) * (1ull << GFS_ST_SHIFT) - 1ull);
+    }
+    if (gfs_ctl.slot[ln] != 0xFFFFFFFFu) rpc_release(g, (int)gfs_ctl.slot[ln]);
+}
+
+// Undo OWN frames whose fill failed: clear the leaf and free the frame.
+__device__ inline void fail_own(gfs_dev* g, int fd, uint64_t pb, uint32_t cnt) {
+    const uint32_t ln = lane();
+    if (ln >= cnt || gfs_ctl.kind[ln] != GFS_K_ERR) return;
+    uint32_t f = gfs_ctl.frame[ln];
+    int err = 0;
+    uint64_t page = pb + ln;
+    uint64_t node = tree_leaf(g, &g->files[fd], page >> GFS_FANOUT_LOG, false, &err);
+    if (node) {
+        uint64_t* sp = &g->nodes[node * GFS_FANOUT + (page & (GFS_FANOUT - 1))];
+        uint64_t exp = leaf_pack(f, meta_gen(ld_rlx(&g->meta[f])));
+        cas(sp, exp, 0ull);
+    }
+    frame_release(g, f);
+    if (gfs_ctl.slot[ln] != 0xFFFFFFFFu) rpc_release(g, (int)gfs_ctl.slot[ln]);
+    gfs_ctl.slot[ln] = 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ uint8_t* frame_ptr(gfs_dev* g, uint32_t f) {
+    return g->pool + ((uint64_t)f << g->log_page);
+}
+__device__ __forceinline__ uint8_t* staging_ptr(gfs_dev* g, uint32_t s) {
+    return g->staging + ((uint64_t)s << g->log_page);
+}
+
+// Wait for the batch's host reads and copy staging -> frames (all threads).
+// Sets kind ERR for failed pages; returns the first error or 0.
+__device__ inline void fill_from_host(gfs_dev* g, uint32_t cnt, uint64_t pb, uint64_t fsize) {
+    if (wave() == 0 && lane() == 0) {
+        gfs_ctl.err = 0;
+        if (gfs_ctl.n_own) {
+            int w = rpc_wait(g, gfs_ctl.cq, gfs_ctl.n_own);
+            if (w) gfs_ctl.err = w;
+        }
+    }
+    __syncthreads();
+    if (gfs_ctl.n_own == 0) return;
+    const int werr = gfs_ctl.err;
+    __syncthreads();
+    if (wave() == 0) {
+        const uint32_t ln = lane();
+        if (ln < cnt && gfs_ctl.kind[ln] == GFS_K_OWN && gfs_ctl.slot[ln] != 0xFFFFFFFFu) {
+            int64_t r = werr ? (int64_t)werr : sys_ld(&g->slots[gfs_ctl.slot[ln]].result);
+            uint64_t poff = (pb + ln) << g->log_page;
+            uint64_t want = fsize > poff ? fsize - poff : 0;
+            if (want > g->page_size) want = g->page_size;
+            if (r < 0 || (uint64_t)r != want) {
+                gfs_ctl.kind[ln] = GFS_K_ERR;
+                atomicCAS(&gfs_ctl.err, 0, r < 0 ? (int)r : -EIO);
+            } else {
+                g->fbytes[gfs_ctl.frame[ln]] = (uint32_t)r;
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t T = nthreads(), t = tid();
+    uint32_t rot = 0;
+    for (uint32_t i = 0; i < cnt; ++i) {
+        if (gfs_ctl.kind[i] != GFS_K_OWN || gfs_ctl.slot[i] == 0xFFFFFFFFu) continue;
+        uint32_t f = gfs_ctl.frame[i];
+        uint64_t bytes = g->fbytes[f];
+        copy_seg(frame_ptr(g, f), staging_ptr(g, gfs_ctl.slot[i]), bytes, (t + T - rot) % T, T, true, false);
+        rot = (rot + (uint32_t)(bytes >> 4)) % T;
+        if (t == 0) count(&g->ctr.bytes_filled, bytes);
+    }
+    __threadfence();
+    __syncthreads();
+}
+
+} // namespace gfs
+
+// ======================================================================
+// Public device API (reference names and signatures)
+// ======================================================================
+
+// libgpufs/fs_calls.cu:227-236
+__device__ inline int gopen(const char* filename, int flags) {
+    __syncthreads();
+    if (gfs::tid() == 0) gfs_ctl.ret = gfs::open_single(filename, flags);
+    __syncthreads();
+    return (int)gfs_ctl.ret;
+}
+
+// libgpufs/fs_calls.cu:851-854 (per-thread, not collective)
+__device__ inline size_t fstat(int fd) {
+    gfs_dev* g = gfs_g;
+    if (!gfs::fd_valid(g, fd)) return (size_t)(-EBADF);
+    return (size_t)g->files[fd].size;
+}
+
+// libgpufs/fs_calls.cu:689-737.  Reads [offset, offset+size) clipped at end of
+// file into `buffer` (global or LDS); returns the bytes read.
+__device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer) {
+    using namespace gfs;
+    gfs_dev* g = gfs_g;
+    __syncthreads();
+    if (tid() == 0) {
+        int64_t r = 0;
+        if (!fd_valid(g, fd) || fst_state(ld_acq(&g->files[fd].st)) != GFS_FS_OPEN) r = -EBADF;
+        else {
+            uint64_t fs = g->files[fd].size;
+            r = offset >= fs ? 0 : (int64_t)(size < fs - offset ? size : fs - offset);
+        }
+        gfs_ctl.ret = r;
+        gfs_ctl.a = r > 0 ? g->files[fd].size : 0;
+    }
+    __syncthreads();
+    const int64_t n = gfs_ctl.ret;
+    const uint64_t fsize = gfs_ctl.a;
+    if (n <= 0) return (size_t)n;
+    const uint32_t L = g->log_page;
+    const uint64_t PS = g->page_size;
+    const uint64_t p_first = offset >> L, p_last = (offset + n - 1) >> L;
+    const uint32_t bmax = 64;
+    int err = 0;
+    for (uint64_t pb = p_first; pb <= p_last && !err; pb += bmax) {
+        const uint32_t cnt = (uint32_t)((p_last - pb + 1) < bmax ? (p_last - pb + 1) : bmax);
+        uint64_t pending = cnt == 64 ? ~0ull : ((1ull << cnt) - 1);
+        uint32_t idle = 0;
+        uint64_t t0 = now_ticks();
+        while (pending && !err) {
+            __syncthreads();
+            if (wave() == 0) {
+                if (lane() == 0) gfs_ctl.err = 0;
+                resolve_batch(g, fd, pb, cnt, pending, 0ull, false);
+            }
+            __syncthreads();
+            if (gfs_ctl.err) { err = gfs_ctl.err; break; }
+            fill_from_host(g, cnt, pb, fsize);
+            // copy every pinned page (hit or freshly filled) into the caller's buffer
+            const uint32_t T = nthreads(), t = tid();
+            uint32_t rot = 0;
+            for (uint32_t i = 0; i < cnt; ++i) {
+                uint32_t k = gfs_ctl.kind[i];
+                if (k != GFS_K_HIT && k != GFS_K_OWN) continue;
+                uint64_t poff = (pb + i) << L;
+                uint64_t lo = poff > offset ? poff : offset;
+                uint64_t hi = poff + PS < offset + n ? poff + PS : offset + n;
+                copy_seg(buffer + (lo - offset), frame_ptr(g, gfs_ctl.frame[i]) + (lo - poff), hi - lo,
+                         (t + T - rot) % T, T, true, true);
+                rot = (rot + (uint32_t)((hi - lo) >> 4)) % T;
+            }
+            __syncthreads();
+            if (wave() == 0) {
+                const uint32_t ln = lane();
+                if (gfs_ctl.err) fail_own(g, fd, pb, cnt);
+                finish_batch(g, cnt);
+                uint64_t waitm = __ballot(ln < cnt && gfs_ctl.kind[ln] == GFS_K_WAIT);
+                if (ln == 0) gfs_ctl.a = waitm;
+            }
+            __syncthreads();
+            if (gfs_ctl.err) { err = gfs_ctl.err; break; }
+            uint64_t nxt = gfs_ctl.a;
+            if (nxt == pending) {
+                if (++idle > 4) backoff();
+                if ((idle & 63) == 0 && now_ticks() - t0 > g->wait_ticks) { err = -ETIMEDOUT; break; }
+            } else {
+                idle = 0;
+            }
+            pending = nxt;
+        }
+    }
+    __syncthreads();
+    return err ? (size_t)(int64_t)err : (size_t)n;
+}
+
+// Writes [offset, offset+size) of `buffer` through the page cache
+// (libgpufs/fs_calls.cu:739-802).  Pages are fetched from the host first
+// unless the file is O_GWRONCE (zero-filled) or the page is fully overwritten.
+__device__ inline size_t gwrite(int fd, size_t offset, size_t size, uchar* buffer) {
+    using namespace gfs;
+    gfs_dev* g = gfs_g;
+    __syncthreads();
+    if (tid() == 0) {
+        int64_t r = (int64_t)size;
+        if (!fd_valid(g, fd) || fst_state(ld_acq(&g->files[fd].st)) != GFS_FS_OPEN) r = -EBADF;
+        else if ((g->files[fd].flags & O_ACCMODE) == O_RDONLY) r = -EBADF;
+        gfs_ctl.ret = r;
+        gfs_ctl.a = r > 0 ? g->files[fd].size : 0;
+        gfs_ctl.flag = r > 0 ? (uint32_t)g->files[fd].flags : 0;
+    }
+    __syncthreads();
+    const int64_t n = gfs_ctl.ret;
+    const uint64_t fsize0 = gfs_ctl.a;
+    const bool wronce = (gfs_ctl.flag & (O_CREAT | O_ACCMODE)) == (uint32_t)O_GWRONCE;
+    if (n <= 0) return (size_t)n;
+    const uint32_t L = g->log_page;
+    const uint64_t PS = g->page_size;
+    const uint64_t p_first = offset >> L, p_last = (offset + n - 1) >> L;
+    const uint32_t bmax = 64;
+    int err = 0;
+    for (uint64_t pb = p_first; pb <= p_last && !err; pb += bmax) {
+        const uint32_t cnt = (uint32_t)((p_last - pb + 1) < bmax ? (p_last - pb + 1) : bmax);
+        uint64_t pending = cnt == 64 ? ~0ull : ((1ull << cnt) - 1);
+        // pages that need no host data: O_GWRONCE, fully overwritten, or wholly past EOF
+        uint64_t nohost = 0;
+        for (uint32_t i = 0; i < cnt; ++i) {
+            uint64_t poff = (pb + i) << L;
+            bool full = offset <= poff && offset + n >= poff + PS;
+            if (wronce || full || poff >= fsize0) nohost |= 1ull << i;
+        }
+        uint32_t idle = 0;
+        uint64_t t0 = now_ticks();
+        while (pending && !err) {
+            __syncthreads();
+            if (wave() == 0) {
+                if (lane() == 0) gfs_ctl.err = 0;
+                resolve_batch(g, fd, pb, cnt, pending, nohost, true);
+            }
+            __syncthreads();
+            if (gfs_ctl.err) { err = gfs_ctl.err; break; }
+            fill_from_host(g, cnt, pb, fsize0);
+            const uint32_t T = nthreads(), t = tid();
+            // zero the never-filled part of host-less OWN pages, then copy the caller's bytes
+            for (uint32_t i = 0; i < cnt; ++i) {
+                if (gfs_ctl.kind[i] != GFS_K_OWN || gfs_ctl.slot[i] != 0xFFFFFFFFu) continue;
+                uint8_t* fp = frame_ptr(g, gfs_ctl.frame[i]);
+                for (uint64_t k = t; k < (PS >> 4); k += T) *(v4u*)(fp + 16 * k) = v4u{0, 0, 0, 0};
+            }
+            __syncthreads();
+            for (uint32_t i = 0; i < cnt; ++i) {
+                uint32_t k = gfs_ctl.kind[i];
+                if (k != GFS_K_HIT && k != GFS_K_OWN) continue;
+                uint64_t poff = (pb + i) << L;
+                uint64_t lo = poff > offset ? poff : offset;
+                uint64_t hi = poff + PS < offset + n ? poff + PS : offset + n;
+                copy_seg(frame_ptr(g, gfs_ctl.frame[i]) + (lo - poff), buffer + (lo - offset), hi - lo, t, T, false, false);
+                if (t == 0) atomicMax(&g->fbytes[gfs_ctl.frame[i]], (uint32_t)(hi - poff));
+            }
+            __threadfence();
+            __syncthreads();
+            if (wave() == 0) {
+                const uint32_t ln = lane();
+                if (ln < cnt) {
+                    uint32_t k = gfs_ctl.kind[ln];
+                    if (k == GFS_K_OWN || k == GFS_K_HIT)
+                        __hip_atomic_fetch_or(&g->meta[gfs_ctl.frame[ln]], GFS_FL_DIRTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                if (gfs_ctl.err) fail_own(g, fd, pb, cnt);
+                finish_batch(g, cnt);
+                uint64_t waitm = __ballot(ln < cnt && gfs_ctl.kind[ln] == GFS_K_WAIT);
+                if (ln == 0) gfs_ctl.a = waitm;
+            }
+            __syncthreads();
+            if (gfs_ctl.err) { err = gfs_ctl.err; break; }
+            uint64_t nxt = gfs_ctl.a;
+            if (nxt == pending) {
+                if (++idle > 4) backoff();
+                if ((idle & 63) == 0 && now_ticks() - t0 > g->wait_ticks) { err = -ETIMEDOUT; break; }
+            } else {
+                idle = 0;
+            }
+            pending = nxt;
+        }
+    }
+    if (tid() == 0 && !err) {
+        g->files[fd].dirty = 1;
+        atomicMax((unsigned long long*)&g->files[fd].size, (unsigned long long)(offset + n));
+    }
+    __syncthreads();
+    return err ? (size_t)(int64_t)err : (size_t)n;
+}
+
+namespace gfs {
+// Write back every dirty page of `fd` (workgroup-collective).  Pages of
+// O_GWRONCE files are OR-merged with the bytes on disk by the host
+// (libgpufs/host_loop.h:76-137).  Returns 0 or the first error.
+__device__ inline int flush_file(gfs_dev* g, int fd) {
+    gfs_file* file = &g->files[fd];
+    const bool wronce = (file->flags & (O_CREAT | O_ACCMODE)) == O_GWRONCE;
+    const uint64_t size = file->size;
+    const uint64_t npages = (size + g->page_size - 1) >> g->log_page;
+    int err = 0;
+    for (uint64_t leaf = 0; (leaf << GFS_FANOUT_LOG) < npages && !err; ++leaf) {
+        __syncthreads();
+        if (wave() == 0) {
+            const uint32_t ln = lane();
+            int e = 0;
+            uint64_t node = 0;
+            if (ln == 0) node = tree_leaf(g, file, leaf, false, &e);
+            node = __shfl(node, 0);
+            uint64_t page = (leaf << GFS_FANOUT_LOG) + ln;
+            bool dirty = false;
+            uint32_t f = GFS_NOFRAME;
+            if (node && page < npages) {
+                uint64_t v = ld_rlx(&g->nodes[node * GFS_FANOUT + ln]);
+                if (v) {
+                    f = leaf_frame(v);
+                    uint64_t old = fadd_rlx(&g->meta[f], 1ull);
+                    if (meta_gen(old) == leaf_gen(v) && meta_state(old) == GFS_F_VALID && (old & GFS_FL_DIRTY)) {
+                        dirty = true;
+                        __hip_atomic_fetch_and(&g->meta[f], ~GFS_FL_DIRTY, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                    } else {
+                        fadd_rlx(&g->meta[f], ~0ull);
+                    }
+                }
+            }
+            fence_acq_agent();
+            gfs_ctl.kind[ln] = dirty ? GFS_K_OWN : GFS_K_DONE;
+            gfs_ctl.frame[ln] = f;
+        }
+        __syncthreads();
+        for (uint32_t i = 0; i < 64; ++i) {
+            if (gfs_ctl.kind[i] != GFS_K_OWN) continue;
+            const uint32_t f = gfs_ctl.frame[i];
+            const uint64_t page = (leaf << GFS_FANOUT_LOG) + i;
+            uint64_t poff = page << g->log_page;
+            uint64_t bytes = g->fbytes[f];
+            uint64_t lim = size > poff ? size - poff : 0;
+            if (bytes > lim) bytes = lim;
+            if (tid() == 0) {
+                int s = -1;
+                uint64_t t0 = now_ticks();
+                while ((s = rpc_claim(g, true, blockIdx.x)) < 0) {
+                    if (now_ticks() - t0 > g->wait_ticks) break;
+                    backoff();
+                }
+                gfs_ctl.slot[i] = s < 0 ? 0xFFFFFFFFu : (uint32_t)s;
+            }
+            __syncthreads();
+            const uint32_t s = gfs_ctl.slot[i];
+            if (s != 0xFFFFFFFFu) {
+                wg_copy(staging_ptr(g, s), frame_ptr(g, f), bytes, false, false);
+                __threadfence_system();
+            }
+            __syncthreads();
+            if (tid() == 0) {
+                if (s == 0xFFFFFFFFu) {
+                    err = -ETIMEDOUT;
+                } else {
+                    sys_st(&g->cq[s], 0u);
+                    rpc_post(g, (int)s, wronce ? GFS_OP_DIFF : GFS_OP_WRITE, file->host_fd, 0, poff, bytes, s);
+                    count(&g->ctr.rpc_writes, 1);
+                    count(&g->ctr.bytes_written, bytes);
+                    int w = rpc_wait(g, s, 1);
+                    int64_t r = w ? (int64_t)w : sys_ld(&g->slots[s].result);
+                    if (r < 0 || (uint64_t)r != bytes) err = r < 0 ? (int)r : -EIO;
+                    rpc_release(g, (int)s);
+                }
+                fadd(&g->meta[f], ~0ull);
+                gfs_ctl.ret = err;
+            }
+            __syncthreads();
+            err = (int)gfs_ctl.ret;
+        }
+    }
+    __syncthreads();
+    return err;
+}
+} // namespace gfs
+
+// libgpufs/fs_calls.cu:107-125
+__device__ inline int gfsync(int fd) {
+    using namespace gfs;
+    gfs_dev* g = gfs_g;
+    __syncthreads();
+    if (tid() == 0) gfs_ctl.ret = (fd_valid(g, fd) && fst_state(ld_acq(&g->files[fd].st)) == GFS_FS_OPEN) ? 0 : -EBADF;
+    __syncthreads();
+    if (gfs_ctl.ret) return (int)gfs_ctl.ret;
+    int r = flush_file(g, fd);
+    __syncthreads();
+    if (tid() == 0 && r == 0) g->files[fd].dirty = 0;
+    __syncthreads();
+    return r;
+}
+
+// libgpufs/fs_calls.cu:35-105.  The last closer writes back dirty pages and
+// closes the host descriptor; cached clean pages stay valid for a reopen.
+__device__ inline int gclose(int fd) {
+    using namespace gfs;
+    gfs_dev* g = gfs_g;
+    __syncthreads();
+    if (tid() == 0) {
+        int r = 0;
+        uint32_t last = 0;
+        if (!fd_valid(g, fd)) r = -EBADF;
+        else {
+            gfs_file* e = &g->files[fd];
+            for (;;) {
+                uint64_t st = ld_acq(&e->st);
+                if (fst_state(st) != GFS_FS_OPEN || fst_opens(st) == 0) { r = -EBADF; break; }
+                uint32_t o = fst_opens(st);
+                uint64_t nw = o > 1 ? fst_make(GFS_FS_OPEN, o - 1, fst_version(st))
+                                    : fst_make(GFS_FS_CLOSING, 0, fst_version(st));
+                uint64_t exp = st;
+                if (cas(&e->st, exp, nw)) { last = o == 1; break; }
+            }
+        }
+        gfs_ctl.ret = r;
+        gfs_ctl.flag = last;
+    }
+    __syncthreads();
+    if (gfs_ctl.ret || !gfs_ctl.flag) return (int)gfs_ctl.ret;
+    gfs_file* e = &g->files[fd];
+    int r = e->dirty ? flush_file(g, fd) : 0;
+    __syncthreads();
+    if (tid() == 0) {
+        int64_t c = rpc_call_single(g, GFS_OP_CLOSE, e->host_fd, 0, 0, 0, nullptr, nullptr);
+        count(&g->ctr.rpc_closes, 1);
+        if (r == 0 && c < 0) r = (int)c;
+        e->host_fd = -1;
+        e->dirty = 0;
+        uint64_t st = ld_rlx(&e->st);
+        st_rel(&e->st, fst_make(GFS_FS_CLOSED, 0, fst_version(st)));
+        gfs_ctl.ret = r;
+    }
+    __syncthreads();
+    return (int)gfs_ctl.ret;
+}
+
+// libgpufs/fs_calls.cu:127-149.  The reference supports only size 0; any size
+// is accepted here.  Cached pages are dropped (new version) as in the reference.
+__device__ inline int gftruncate(int fd, int size) {
+    using namespace gfs;
+    gfs_dev* g = gfs_g;
+    __syncthreads();
+    if (tid() == 0) {
+        int r = 0;
+        if (!fd_valid(g, fd) || fst_state(ld_acq(&g->files[fd].st)) != GFS_FS_OPEN || size < 0) r = -EBADF;
+        else {
+            gfs_file* e = &g->files[fd];
+            int64_t c = rpc_call_single(g, GFS_OP_TRUNC, e->host_fd, 0, (uint64_t)size, 0, nullptr, nullptr);
+            if (c < 0) r = (int)c;
+            else {
+                e->size = (uint64_t)size;
+                st_rlx(&e->root, 0ull);
+                for (;;) {
+                    uint64_t st = ld_acq(&e->st);
+                    uint64_t exp = st;
+                    if (cas(&e->st, exp, fst_make(fst_state(st), fst_opens(st), fst_version(st) + 1))) break;
+                }
+            }
+        }
+        gfs_ctl.ret = r;
+    }
+    __syncthreads();
+    return (int)gfs_ctl.ret;
+}
+
+// libgpufs/fs_calls.cu:422-481.  Pins the page holding `offset` and returns a
+// pointer to `offset` inside it; [offset, offset+size) must not cross a page.
+// Files opened for writing mark the page dirty.
+__device__ inline volatile void* gmmap(void* addr, size_t size, int prot, int flags, int fd, off_t offset) {
+    using namespace gfs;
+    (void)addr; (void)prot; (void)flags;
+    gfs_dev* g = gfs_g;
+    __syncthreads();
+    if (tid() == 0) {
+        int64_t r = 0;
+        if (!fd_valid(g, fd) || fst_state(ld_acq(&g->files[fd].st)) != GFS_FS_OPEN || offset < 0) r = -EBADF;
+        else if (((uint64_t)offset & (g->page_size - 1)) + size > g->page_size) r = -EINVAL;
+        gfs_ctl.ret = r;
+        gfs_ctl.a = g->files[fd].size;
+        gfs_ctl.flag = (uint32_t)g->files[fd].flags;
+    }
+    __syncthreads();
+    if (gfs_ctl.ret) return GMAP_FAILED;
+    const uint64_t fsize = gfs_ctl.a;
+    const int oflags = (int)gfs_ctl.flag;
+    const bool writable = (oflags & O_ACCMODE) != O_RDONLY;
+    const bool wronce = (oflags & (O_CREAT | O_ACCMODE)) == O_GWRONCE;
+    const uint64_t page = (uint64_t)offset >> g->log_page;
+    const bool nohost = wronce || (page << g->log_page) >= fsize;
+    uint64_t t0 = now_ticks();
+    for (;;) {
+        __syncthreads();
+        if (wave() == 0) {
+            if (lane() == 0) gfs_ctl.err = 0;
+            resolve_batch(g, fd, page, 1, 1ull, nohost ? 1ull : 0ull, writable);
+        }
+        __syncthreads();
+        if (gfs_ctl.err) return GMAP_FAILED;
+        fill_from_host(g, 1, page, fsize);
+        if (gfs_ctl.kind[0] == GFS_K_OWN && gfs_ctl.slot[0] == 0xFFFFFFFFu) {
+            uint8_t* fp = frame_ptr(g, gfs_ctl.frame[0]);
+            for (uint64_t k = tid(); k < (g->page_size >> 4); k += nthreads()) *(v4u*)(fp + 16 * k) = v4u{0, 0, 0, 0};
+            __threadfence();
+        }
+        __syncthreads();
+        if (tid() == 0) {
+            const uint32_t k = gfs_ctl.kind[0];
+            const uint32_t f = gfs_ctl.frame[0];
+            if (k == GFS_K_ERR) {
+                fail_own(g, fd, page, 1);
+                gfs_ctl.ret = -EIO;
+            } else if (k == GFS_K_OWN) {
+                if (gfs_ctl.slot[0] != 0xFFFFFFFFu) rpc_release(g, (int)gfs_ctl.slot[0]);
+                // publish VALID but keep the pin: it is dropped by gmunmap
+                fadd(&g->meta[f], (GFS_F_VALID - GFS_F_FILLING) * (1ull << GFS_ST_SHIFT));
+                gfs_ctl.ret = 1;
+            } else if (k == GFS_K_HIT) {
+                gfs_ctl.ret = 1;
+            } else {
+                gfs_ctl.ret = 0;
+            }
+            if (gfs_ctl.ret == 1 && writable) {
+                __hip_atomic_fetch_or(&g->meta[f], GFS_FL_DIRTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                uint32_t end = (uint32_t)(((uint64_t)offset & (g->page_size - 1)) + size);
+                atomicMax(&g->fbytes[f], end);
+                g->files[fd].dirty = 1;
+                atomicMax((unsigned long long*)&g->files[fd].size, (unsigned long long)(offset + size));
+            }
+        }
+        __syncthreads();
+        if (gfs_ctl.ret < 0) return GMAP_FAILED;
+        if (gfs_ctl.ret == 1) break;
+        if (now_ticks() - t0 > g->wait_ticks) return GMAP_FAILED;
+        backoff();
+    }
+    return (volatile void*)(frame_ptr(g, gfs_ctl.frame[0]) + ((uint64_t)offset & (g->page_size - 1)));
+}
+
+// libgpufs/fs_calls.cu:385-401.  Unpins the frame holding `addr`; -1 if
+// `addr` is not inside the page pool.
+__device__ inline int gmunmap(volatile void* addr, size_t length) {
+    using namespace gfs;
+    (void)length;
+    gfs_dev* g = gfs_g;
+    const uint8_t* a = (const uint8_t*)(uintptr_t)addr;
+    uint64_t f = (uint64_t)(a - g->pool) >> g->log_page;
+    if (a < g->pool || f >= g->n_frames) return -1;
+    __threadfence();
+    __syncthreads();
+    if (tid() == 0) fadd(&g->meta[f], ~0ull);
+    __syncthreads();
+    return 0;
+}
+
+// libgpufs/fs_calls.cu:804-843.  Writes back the page holding `addr` now.
+__device__ inline int gmsync(volatile void* addr, size_t length, int flags) {
+    using namespace gfs;
+    (void)length; (void)flags;
+    gfs_dev* g = gfs_g;
+    const uint8_t* a = (const uint8_t*)(uintptr_t)addr;
+    uint64_t f = (uint64_t)(a - g->pool) >> g->log_page;
+    if (a < g->pool || f >= g->n_frames) return -1;
+    __threadfence();
+    const uint64_t key = g->fkey[f];
+    const int fd = (int)(key >> 40);
+    const uint64_t page = key & 0xFFFFFFFFFFull;
+    gfs_file* file = &g->files[fd];
+    const bool wronce = (file->flags & (O_CREAT | O_ACCMODE)) == O_GWRONCE;
+    uint64_t poff = page << g->log_page;
+    uint64_t bytes = g->fbytes[f];
+    __syncthreads();
+    if (tid() == 0) {
+        int s = -1;
+        uint64_t t0 = now_ticks();
+        while ((s = rpc_claim(g, true, blockIdx.x)) < 0) {
+            if (now_ticks() - t0 > g->wait_ticks) break;
+            backoff();
+        }
+        gfs_ctl.slot[0] = s < 0 ? 0xFFFFFFFFu : (uint32_t)s;
+        if (s >= 0) __hip_atomic_fetch_and(&g->meta[f], ~GFS_FL_DIRTY, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    const uint32_t s = gfs_ctl.slot[0];
+    if (s == 0xFFFFFFFFu) return -ETIMEDOUT;
+    wg_copy(staging_ptr(g, s), frame_ptr(g, (uint32_t)f), bytes);
+    __threadfence_system();
+    __syncthreads();
+    if (tid() == 0) {
+        sys_st(&g->cq[s], 0u);
+        rpc_post(g, (int)s, wronce ? GFS_OP_DIFF : GFS_OP_WRITE, file->host_fd, 0, poff, bytes, s);
+        count(&g->ctr.rpc_writes, 1);
+        count(&g->ctr.bytes_written, bytes);
+        int w = rpc_wait(g, s, 1);
+        int64_t r = w ? (int64_t)w : sys_ld(&g->slots[s].result);
+        rpc_release(g, (int)s);
+        gfs_ctl.ret = (r < 0 || (uint64_t)r != bytes) ? (r < 0 ? r : -EIO) : 0;
+    }
+    __syncthreads();
+    return (int)gfs_ctl.ret;
+}
+
+// Host helper: store the context's device state in this translation unit.
+// Call once per context (and again after switching contexts) before launching
+// kernels that use the API above.
+static inline int gpufs_bind(gpufs_ctx* ctx) {
+    static void* bound = nullptr;
+    static int bound_dev = -1;
+    void* p = gpufs_device_state(ctx);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -ENODEV;
+    if (p == bound && dev == bound_dev) return 0;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(gfs_g), &p, sizeof(p), 0, hipMemcpyHostToDevice) != hipSuccess) return -EIO;
+    bound = p;
+    bound_dev = dev;
+    return 0;
+}

@@ -1,0 +1,218 @@
+// host_service.h -- host side of the GPU->host RPC ring (no HIP dependency).
+//
+// Replaces the reference's host loop (libgpufs/host_loop.h:156-557: open_loop,
+// rw_task, mainLoop, async_close_loop).  Differences by design:
+//  * one submission queue instead of per-worker slot scans: the GPU appends
+//    (ticket << 16 | slot) entries; any idle worker claims the next ticket with
+//    a CAS, so requests are served in ticket order by all workers at once;
+//  * the worker pread()s straight into the slot's pinned staging buffer (the
+//    GPU copies it into HBM), so there is no host-side batching thread and no
+//    H2D memcpy on the host;
+//  * completion = result field + a release increment of the batch's completion
+//    counter, which the GPU polls;
+//  * the service runs independently of kernel completion (start/stop), which
+//    persistent and graph-launched kernels need; run_handler adds the
+//    reference's "until the stream is idle" behaviour on top.
+// All ring memory is plain host memory here; the runtime allocates it as
+// coherent pinned memory visible to the GPU.
+#pragma once
+#include <atomic>
+#include <cerrno>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <time.h>
+#include <unistd.h>
+#include "gfs_layout.h"
+
+namespace gfs_host {
+
+struct RingView {
+    gfs_rpc_slot* slots = nullptr;
+    uint64_t* sq = nullptr;
+    uint32_t* cq = nullptr;
+    uint8_t* staging = nullptr;
+    uint32_t sq_depth = 0;      // power of two
+    uint32_t ring_depth = 0;
+    uint32_t log_page = 0;
+};
+
+struct ServiceCounters {
+    std::atomic<uint64_t> requests{0}, bytes_read{0}, bytes_written{0}, errors{0};
+};
+
+static inline uint64_t ts_ns(const struct timespec& t) {
+    return (uint64_t)t.tv_sec * 1000000000ull + (uint64_t)t.tv_nsec;
+}
+
+// pread/pwrite until `len` bytes moved or EOF / error.
+static inline int64_t pread_full(int fd, uint8_t* buf, uint64_t len, uint64_t off) {
+    uint64_t done = 0;
+    while (done < len) {
+        ssize_t r = ::pread(fd, buf + done, len - done, (off_t)(off + done));
+        if (r < 0) {
+            if (errno == EINTR) continue;
+            return -(int64_t)errno;
+        }
+        if (r == 0) break;
+        done += (uint64_t)r;
+    }
+    return (int64_t)done;
+}
+static inline int64_t pwrite_full(int fd, const uint8_t* buf, uint64_t len, uint64_t off) {
+    uint64_t done = 0;
+    while (done < len) {
+        ssize_t r = ::pwrite(fd, buf + done, len - done, (off_t)(off + done));
+        if (r < 0) {
+            if (errno == EINTR) continue;
+            return -(int64_t)errno;
+        }
+        done += (uint64_t)r;
+    }
+    return (int64_t)done;
+}
+
+// O_GWRONCE write-back: new bytes are OR-ed with the bytes already on disk
+// (libgpufs/host_loop.h:76-137).  The reference ORs whole 16-byte vectors
+// against a buffer that may hold stale bytes past the read; here exactly the
+// min(len, bytes on disk) overlapping bytes are merged.
+static inline int64_t diff_merge_write(int fd, uint8_t* page, uint64_t len, uint64_t off,
+                                       std::vector<uint8_t>& tmp) {
+    tmp.resize(len);
+    int64_t r = pread_full(fd, tmp.data(), len, off);
+    if (r < 0) return r;
+    for (int64_t i = 0; i < r; ++i) page[i] |= tmp[(size_t)i];
+    return pwrite_full(fd, page, len, off);
+}
+
+// Serve one request in slot s.  Returns the result stored in the slot.
+static inline int64_t serve_slot(const RingView& rv, uint32_t s, ServiceCounters& ctr,
+                                 std::vector<uint8_t>& tmp) {
+    gfs_rpc_slot* r = &rv.slots[s];
+    const uint32_t op = r->op;
+    const int fd = r->host_fd;
+    const uint64_t off = r->offset, len = r->length;
+    uint8_t* stage = rv.staging + ((uint64_t)s << rv.log_page);
+    int64_t res = -EINVAL;
+    switch (op) {
+    case GFS_OP_READ:
+        res = pread_full(fd, stage, len, off);
+        if (res > 0) ctr.bytes_read += (uint64_t)res;
+        break;
+    case GFS_OP_WRITE:
+        res = pwrite_full(fd, stage, len, off);
+        if (res > 0) ctr.bytes_written += (uint64_t)res;
+        break;
+    case GFS_OP_DIFF:
+        res = diff_merge_write(fd, stage, len, off, tmp);
+        if (res > 0) ctr.bytes_written += (uint64_t)res;
+        break;
+    case GFS_OP_OPEN: {
+        char name[GFS_NAME_MAX + 1];
+        memcpy(name, r->name, GFS_NAME_MAX);
+        name[GFS_NAME_MAX] = 0;
+        int flags = r->flags;
+        // O_GWRONCE files are opened read-write so the OR-merge can read them
+        // (libgpufs/host_loop.h:170-173)
+        if ((flags & (O_CREAT | O_ACCMODE)) == (O_CREAT | O_WRONLY)) flags = O_RDWR | O_CREAT;
+        int hfd = ::open(name, flags | O_CLOEXEC, 0644);
+        if (hfd < 0) {
+            res = -(int64_t)errno;
+            break;
+        }
+        struct stat st;
+        if (::fstat(hfd, &st) != 0) {
+            res = -(int64_t)errno;
+            ::close(hfd);
+            break;
+        }
+        r->size = (uint64_t)st.st_size;
+        r->ctime_ns = ts_ns(st.st_ctim);
+        r->mtime_ns = ts_ns(st.st_mtim);
+        r->ino = (uint64_t)st.st_ino;
+        res = hfd;
+        break;
+    }
+    case GFS_OP_CLOSE:
+        res = ::close(fd) == 0 ? 0 : -(int64_t)errno;
+        break;
+    case GFS_OP_TRUNC:
+        res = ::ftruncate(fd, (off_t)off) == 0 ? 0 : -(int64_t)errno;
+        break;
+    default:
+        res = -EINVAL;
+    }
+    if (res < 0) ctr.errors++;
+    ctr.requests++;
+    __atomic_store_n(&r->result, res, __ATOMIC_RELAXED);
+    __atomic_fetch_add(&rv.cq[r->cq], 1u, __ATOMIC_RELEASE);
+    return res;
+}
+
+// Try to take the next published request.  Returns true and sets *slot.
+static inline bool try_take(const RingView& rv, std::atomic<uint64_t>& head, uint32_t* slot) {
+    uint64_t h = head.load(std::memory_order_relaxed);
+    uint64_t e = __atomic_load_n(&rv.sq[h & (rv.sq_depth - 1)], __ATOMIC_ACQUIRE);
+    if ((e >> 16) != (h & 0xFFFFFFFFFFFFull)) return false;
+    if (!head.compare_exchange_strong(h, h + 1, std::memory_order_acq_rel)) return false;
+    *slot = (uint32_t)(e & 0xFFFF);
+    return true;
+}
+
+// Worker loop: spin while work arrives, back off to short sleeps when idle.
+static inline void worker_loop(RingView rv, std::atomic<uint64_t>* head, std::atomic<bool>* stop,
+                               ServiceCounters* ctr) {
+    std::vector<uint8_t> tmp;
+    uint64_t idle = 0;
+    while (!stop->load(std::memory_order_relaxed)) {
+        uint32_t s;
+        if (try_take(rv, *head, &s)) {
+            serve_slot(rv, s, *ctr, tmp);
+            idle = 0;
+            continue;
+        }
+        ++idle;
+        if (idle < 20000) {
+#if defined(__x86_64__)
+            __builtin_ia32_pause();
+#endif
+        } else {
+            struct timespec ts = {0, idle < 200000 ? 2000 : 50000};
+            nanosleep(&ts, nullptr);
+        }
+    }
+}
+
+class Service {
+public:
+    void start(const RingView& rv, unsigned nthreads) {
+        if (running_) return;
+        stop_.store(false);
+        for (unsigned i = 0; i < nthreads; ++i)
+            threads_.emplace_back(worker_loop, rv, &head_, &stop_, &ctr_);
+        running_ = true;
+    }
+    void stop() {
+        if (!running_) return;
+        stop_.store(true);
+        for (auto& t : threads_) t.join();
+        threads_.clear();
+        running_ = false;
+    }
+    bool running() const { return running_; }
+    ServiceCounters& counters() { return ctr_; }
+    uint64_t head() const { return head_.load(); }
+    ~Service() { stop(); }
+
+private:
+    std::vector<std::thread> threads_;
+    std::atomic<uint64_t> head_{0};
+    std::atomic<bool> stop_{false};
+    ServiceCounters ctr_;
+    bool running_ = false;
+};
+
+} // namespace gfs_host

@@ -1,0 +1,133 @@
+// k_gread.hip -- gread/gwrite harness kernels and their C-ABI launchers.
+//
+// gpufs_launch_gread_ranges is the sequential-read workload (one gopen + one
+// gread of a disjoint byte range per workgroup, the access pattern of
+// samples/memcopy/cp.cu:73-108 scaled to large ranges); gpufs_launch_read /
+// gpufs_launch_write run a single open/read|write/close sequence in one
+// workgroup for the API tests.
+#include "gpufs.h"
+
+static int check_threads(uint32_t threads) {
+    if (threads < 64 || threads > 1024 || (threads % 64)) return -EINVAL;
+    return 0;
+}
+static hipStream_t pick_stream(gpufs_ctx* ctx, void* stream) {
+    return stream ? (hipStream_t)stream : (hipStream_t)gpufs_stream(ctx);
+}
+
+__global__ __launch_bounds__(1024) void k_gread_ranges(const char* path, uint64_t range, uint8_t* dst,
+                                                       int64_t* out) {
+    int fd = gopen(path, O_GRDONLY);
+    if (fd < 0) {
+        if (gfs::tid() == 0) out[blockIdx.x] = fd;
+        return;
+    }
+    size_t r = gread(fd, (size_t)blockIdx.x * range, range, dst + (size_t)blockIdx.x * range);
+    int c = gclose(fd);
+    if (gfs::tid() == 0) out[blockIdx.x] = (int64_t)r < 0 ? (int64_t)r : (c < 0 ? (int64_t)c : (int64_t)r);
+}
+
+extern "C" int gpufs_launch_gread_ranges(gpufs_ctx* ctx, const char* path_dev, uint32_t n_ranges,
+                                         uint64_t range_bytes, void* dst_dev, int64_t* bytes_out_dev,
+                                         uint32_t threads, void* stream) {
+    if (!ctx || !path_dev || !dst_dev || !bytes_out_dev || n_ranges == 0) return -EINVAL;
+    int v = check_threads(threads);
+    if (v) return v;
+    if ((v = gpufs_bind(ctx))) return v;
+    hipLaunchKernelGGL(k_gread_ranges, dim3(n_ranges), dim3(threads), 0, pick_stream(ctx, stream), path_dev,
+                       range_bytes, (uint8_t*)dst_dev, bytes_out_dev);
+    return hipGetLastError() == hipSuccess ? 0 : -EIO;
+}
+
+__global__ __launch_bounds__(1024) void k_read_one(const char* path, int flags, uint64_t off, uint64_t len,
+                                                   uint8_t* dst, int64_t* out) {
+    int fd = gopen(path, flags);
+    if (fd < 0) {
+        if (gfs::tid() == 0) out[0] = fd;
+        return;
+    }
+    size_t r = gread(fd, off, len, dst);
+    size_t sz = fstat(fd);
+    int c = gclose(fd);
+    if (gfs::tid() == 0) {
+        out[0] = (int64_t)r;
+        out[1] = (int64_t)sz;
+        out[2] = c;
+    }
+}
+
+extern "C" int gpufs_launch_read(gpufs_ctx* ctx, const char* path_dev, int32_t open_flags, uint64_t offset,
+                                 uint64_t length, void* dst_dev, int64_t* result_dev, uint32_t threads,
+                                 void* stream) {
+    if (!ctx || !path_dev || !result_dev) return -EINVAL;
+    int v = check_threads(threads);
+    if (v) return v;
+    if ((v = gpufs_bind(ctx))) return v;
+    hipLaunchKernelGGL(k_read_one, dim3(1), dim3(threads), 0, pick_stream(ctx, stream), path_dev, open_flags,
+                       offset, length, (uint8_t*)dst_dev, result_dev);
+    return hipGetLastError() == hipSuccess ? 0 : -EIO;
+}
+
+__global__ __launch_bounds__(1024) void k_write_one(const char* path, int flags, uint64_t off, uint64_t len,
+                                                    const uint8_t* src, int do_fsync, int64_t* out) {
+    int fd = gopen(path, flags);
+    if (fd < 0) {
+        if (gfs::tid() == 0) out[0] = fd;
+        return;
+    }
+    size_t r = gwrite(fd, off, len, (uchar*)src);
+    int s = do_fsync ? gfsync(fd) : 0;
+    size_t sz = fstat(fd);
+    int c = gclose(fd);
+    if (gfs::tid() == 0) {
+        out[0] = (int64_t)r;
+        out[1] = (int64_t)sz;
+        out[2] = c;
+        out[3] = s;
+    }
+}
+
+extern "C" int gpufs_launch_write(gpufs_ctx* ctx, const char* path_dev, int32_t open_flags, uint64_t offset,
+                                  uint64_t length, const void* src_dev, int32_t do_fsync, int64_t* result_dev,
+                                  uint32_t threads, void* stream) {
+    if (!ctx || !path_dev || !result_dev) return -EINVAL;
+    int v = check_threads(threads);
+    if (v) return v;
+    if ((v = gpufs_bind(ctx))) return v;
+    hipLaunchKernelGGL(k_write_one, dim3(1), dim3(threads), 0, pick_stream(ctx, stream), path_dev, open_flags,
+                       offset, length, (const uint8_t*)src_dev, do_fsync, result_dev);
+    return hipGetLastError() == hipSuccess ? 0 : -EIO;
+}
+
+// FNV-1a-64 of each of n_ranges consecutive ranges of a device buffer (one
+// lane per range; the hash is byte-serial).  Used to check multi-GiB gread
+// outputs against the host without copying them back.
+__global__ void k_fnv_ranges(const uint8_t* p, uint32_t n, uint64_t range, uint64_t* out) {
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const uint8_t* b = p + (uint64_t)r * range;
+    uint64_t h = 0xcbf29ce484222325ull;
+    uint64_t i = 0;
+    const uint64_t words = ((uintptr_t)b & 7) ? 0 : range / 8;
+    for (; i < words; ++i) {
+        uint64_t w = ((const uint64_t*)b)[i];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            h ^= (w >> (8 * k)) & 0xFF;
+            h *= 0x100000001b3ull;
+        }
+    }
+    for (i *= 8; i < range; ++i) {
+        h ^= b[i];
+        h *= 0x100000001b3ull;
+    }
+    out[r] = h;
+}
+
+extern "C" int gpufs_fnv1a64_ranges(const void* data_dev, uint32_t n_ranges, uint64_t range_bytes,
+                                    uint64_t* out_dev, void* stream) {
+    if (!data_dev || !out_dev || n_ranges == 0) return -EINVAL;
+    hipLaunchKernelGGL(k_fnv_ranges, dim3((n_ranges + 63) / 64), dim3(64), 0, (hipStream_t)stream,
+                       (const uint8_t*)data_dev, n_ranges, range_bytes, out_dev);
+    return hipGetLastError() == hipSuccess ? 0 : -EIO;
+}

@@ -1,0 +1,145 @@
+/*
+ * gpufs_amd.h -- C ABI of the MI355X-native GPUfs file I/O path.
+ *
+ * Host side of the drop-in boundary.  Device code (gopen/gread/gwrite/gmmap/
+ * gmunmap/gfsync/gclose/...) is the header-only HIP API in
+ * gpufs_amd/csrc/gpufs.h; it cannot cross a C ABI, so the entry points below
+ * are the host init / host-loop calls of the reference plus the launchers of
+ * the workload kernels that run on top of the device API.
+ *
+ * Conventions: every function returns 0 (or a non-negative count) on success
+ * and a negative GPUFS_E* / -errno code on failure; nothing here calls exit().
+ * Pointers named *_dev are device (HBM) addresses, e.g. a torch CUDA tensor's
+ * data_ptr(); pointers named *_host are host addresses.  Streams are
+ * hipStream_t passed as void*; NULL means the context's own stream.
+ */
+#ifndef GPUFS_AMD_H
+#define GPUFS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes shared with the device API.  -1/-2 keep the values of the
+ * reference's E_FSTABLE_FULL / E_IPC_OPEN_ERROR (libgpufs/fs_constants.h:50-51);
+ * everything else is a negated errno. */
+#define GPUFS_OK               0
+#define GPUFS_E_FSTABLE_FULL  (-1)
+#define GPUFS_E_IPC_OPEN      (-2)
+
+typedef struct gpufs_ctx gpufs_ctx;
+
+/* Runtime sizing of everything the reference fixes with #defines in
+ * libgpufs/fs_constants.h:25-77 (FS_BLOCKSIZE, PPOOL_SIZE, MAX_NUM_FILES,
+ * RW_IPC_SIZE, RW_HOST_WORKERS). */
+typedef struct gpufs_config {
+    uint64_t cache_bytes;   /* page-cache pool in HBM; frames = cache_bytes / page_size */
+    uint32_t page_size;     /* power of two, 4 KiB .. 4 MiB */
+    uint32_t max_files;     /* file-table capacity (distinct file names per context) */
+    uint32_t ring_depth;    /* GPU->host RPC slots; each owns page_size bytes of pinned staging */
+    uint32_t host_threads;  /* host service threads doing open/pread/pwrite */
+    uint32_t tree_nodes;    /* radix-tree node pool (512 B each); 0 = sized automatically */
+    int32_t  device;        /* HIP device ordinal */
+    uint32_t flags;         /* reserved, must be 0 */
+    uint32_t reserved;
+} gpufs_config;
+
+/* Counters exported by gpufs_stats_get (the reference's MALLOC_STATS /
+ * PRINT_TIMES counters, libgpufs/fs_debug.cu.h:74-149, host_loop.h:617-655). */
+typedef struct gpufs_stats {
+    uint64_t page_hits;      /* page lookups served from a VALID frame */
+    uint64_t page_misses;    /* page lookups that filled a frame */
+    uint64_t page_waits;     /* lookups that found a frame being filled by another workgroup */
+    uint64_t evictions;      /* VALID frames reclaimed by the clock sweep */
+    uint64_t alloc_fail;     /* frame allocations that found nothing evictable */
+    uint64_t rpc_reads;      /* READ requests posted to the host */
+    uint64_t rpc_writes;     /* WRITE / DIFF requests posted to the host */
+    uint64_t rpc_opens;      /* OPEN requests */
+    uint64_t rpc_closes;     /* CLOSE requests */
+    uint64_t bytes_filled;   /* bytes copied from pinned staging into frames */
+    uint64_t bytes_written;  /* bytes copied from frames into staging for write-back */
+    uint64_t nodes_used;     /* radix-tree nodes allocated */
+    uint64_t files_used;     /* file-table entries in use */
+    uint64_t host_requests;  /* requests served by the host service */
+    uint64_t host_bytes_read;
+    uint64_t host_bytes_written;
+    uint64_t frames;         /* frames in the pool */
+    uint64_t frames_valid;   /* frames VALID at the time of the call */
+} gpufs_stats;
+
+/* ---- lifecycle: replaces initializer(GPUGlobals_ptr*) (libgpufs/fs_initializer.cu:116)
+ * and `delete GPUGlobals` (libgpufs/fs_initializer.cu.h:180-198). */
+int   gpufs_config_default(gpufs_config* cfg);
+int   gpufs_init(const gpufs_config* cfg, gpufs_ctx** out);
+int   gpufs_destroy(gpufs_ctx* ctx);
+/* The stream kernels that call the device API are launched on
+ * (GPUGlobals::streamMgr->kernelStream, libgpufs/fs_initializer.cu.h:56-75). */
+void* gpufs_stream(gpufs_ctx* ctx);
+/* Device address of the context's device state; gpufs.h's gpufs_bind() stores it
+ * in the translation unit that launches device-API kernels. */
+void* gpufs_device_state(gpufs_ctx* ctx);
+int   gpufs_get_config(gpufs_ctx* ctx, gpufs_config* out);
+
+/* ---- host service: replaces run_gpufs_handler(volatile GPUGlobals*, int)
+ * (libgpufs/host_loop.h:559-656).  gpufs_run_handler has the reference's
+ * semantics: it serves RPCs until the context stream is idle, then returns.
+ * gpufs_host_start/stop run the same service free-standing, independent of
+ * kernel completion, for persistent, graph-captured and serving kernels. */
+int   gpufs_run_handler(gpufs_ctx* ctx);
+int   gpufs_host_start(gpufs_ctx* ctx);
+int   gpufs_host_stop(gpufs_ctx* ctx);
+
+/* ---- cache control and introspection (no reference counterpart; used by the
+ * cold-read measurements and the invariant tests). */
+int   gpufs_cache_drop(gpufs_ctx* ctx);   /* empty the page cache; no kernel may be running */
+int   gpufs_stats_get(gpufs_ctx* ctx, gpufs_stats* out);
+int   gpufs_stats_reset(gpufs_ctx* ctx);
+/* Checks page-cache invariants on the host: no two live frames own the same
+ * (file, page) key, no frame is pinned, every live leaf points at a live frame
+ * of the right key.  Returns 0, or a negative code naming the first violation. */
+int   gpufs_check_invariants(gpufs_ctx* ctx);
+const char* gpufs_strerror(int code);
+
+/* ---- workload launchers (device API callers), one per benchmark shape.
+ * Each launches on the context stream and returns without waiting; call
+ * gpufs_run_handler (or keep the host service started) and synchronise. */
+
+/* Sequential gread (samples/memcopy/cp.cu:73-108 pattern): workgroup w greads
+ * [w*range_bytes, (w+1)*range_bytes) of `path_dev` (a NUL-terminated device
+ * string) into dst_dev + w*range_bytes.  bytes_out_dev[w] = gread's return. */
+int gpufs_launch_gread_ranges(gpufs_ctx* ctx, const char* path_dev, uint32_t n_ranges,
+                              uint64_t range_bytes, void* dst_dev, int64_t* bytes_out_dev,
+                              uint32_t threads, void* stream);
+
+/* grep_text-style scan (samples/grep_text/grep_text_gpu.cu:193-387, scan
+ * semantics :79,139-160, stitching :298-318): workgroup f gopens file
+ * names_dev + f*name_stride, greads it in chunk_bytes windows with a 32-byte
+ * look-ahead and adds whole-word matches of the n_words dictionary words
+ * (dict_dev: n_words records of 32 bytes, NUL-padded) into
+ * counts_dev[f*n_words + q] (uint32, must be zeroed by the caller).
+ * status_dev[f] = 0 or a negative error code. */
+int gpufs_launch_grep(gpufs_ctx* ctx, const char* names_dev, uint32_t name_stride,
+                      uint32_t n_files, const char* dict_dev, uint32_t n_words,
+                      uint32_t chunk_bytes, uint32_t* counts_dev, int32_t* status_dev,
+                      uint32_t threads, void* stream);
+
+/* FNV-1a-64 of n_ranges consecutive ranges of a device buffer (the per-range
+ * checksum of the sequential-read workload; no reference counterpart). */
+int gpufs_fnv1a64_ranges(const void* data_dev, uint32_t n_ranges, uint64_t range_bytes,
+                         uint64_t* out_dev, void* stream);
+
+/* Generic single-call launchers used by the API tests. */
+int gpufs_launch_read(gpufs_ctx* ctx, const char* path_dev, int32_t open_flags,
+                      uint64_t offset, uint64_t length, void* dst_dev,
+                      int64_t* result_dev, uint32_t threads, void* stream);
+int gpufs_launch_write(gpufs_ctx* ctx, const char* path_dev, int32_t open_flags,
+                       uint64_t offset, uint64_t length, const void* src_dev,
+                       int32_t do_fsync, int64_t* result_dev, uint32_t threads, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPUFS_AMD_H */

@@ -1,0 +1,100 @@
+"""GPU parity of the grep_text-style scan (gopen + gread into LDS + whole-word
+counts) against the CPU oracle (literal restatement of match_string and the
+chunk loop) and against the generator's tally."""
+import os
+
+import numpy as np
+import pytest
+
+from gpufs_amd import datagen as dg
+from tests.gpu_helpers import grep, make_ctx, say
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_grep_reference_fixture(tmp_path, orc):
+    say("test_grep_reference_fixture")
+    import json
+    g = os.path.join(HERE, "golden")
+    text = open(os.path.join(g, "pg100_small.txt"), "rb").read()
+    d = np.fromfile(os.path.join(g, "dict_subset.bin"), dtype=np.uint8).reshape(-1, 32)
+    exp = json.load(open(os.path.join(g, "pg100_counts.json")))
+    p = tmp_path / "pg100.txt"
+    p.write_bytes(text)
+    with make_ctx(page_size=4096, cache_bytes=4 << 20) as ctx:
+        for chunk in (16384, 64):
+            counts, status = grep(ctx, [str(p)], d, chunk)
+            assert status[0] == 0
+            assert (counts[0] == np.array(exp["counts"][str(chunk)], dtype=np.uint32)).all(), chunk
+        assert ctx.check_invariants() == 0
+
+
+def test_grep_synthetic_matches_tally_and_oracle(tmp_path, orc):
+    say("test_grep_synthetic_matches_tally_and_oracle")
+    voc = dg.vocab(5000, seed=31)
+    qi = dg.query(5000, 1024, seed=32)
+    q = voc[qi]
+    paths = [str(tmp_path / f"t{i}.txt") for i in range(8)]
+    sizes = [256 << 10] * 6 + [1000, (256 << 10) + 77]
+    tally = dg.text_files(paths, sizes, 33, voc, with_tally=True, threads=4)
+    cpu, st = orc.scan_files(paths, q, 65536, threads=4)
+    assert (st == 0).all()
+    with make_ctx(page_size=65536, cache_bytes=32 << 20) as ctx:
+        counts, status = grep(ctx, paths, q, 65536)
+        assert (status == 0).all(), status
+        assert (counts == cpu).all()
+        for i in range(6):
+            assert (counts[i] == tally[i][qi]).all()
+        # warm pass gives the same counts
+        counts2, _ = grep(ctx, paths, q, 65536)
+        assert (counts2 == counts).all()
+        assert ctx.check_invariants() == 0
+
+
+@pytest.mark.parametrize("chunk", [8, 64, 100, 4096])
+def test_grep_adversarial_small_chunks(tmp_path, orc, chunk):
+    say(f"test_grep_adversarial_small_chunks chunk={chunk}")
+    rng = np.random.default_rng(chunk)
+    alphabet = np.frombuffer(b"aab_A ' \n\0\xff-1", dtype=np.uint8)
+    paths, datas = [], []
+    for i in range(6):
+        n = int(rng.integers(0, 20000))
+        t = rng.choice(alphabet, size=n) if i % 2 else rng.choice(np.frombuffer(b"aab ", dtype=np.uint8), size=n)
+        if i == 4:
+            t = np.frombuffer(b"x" * 40 + b"yy z" + b"a" * 70 + b" aab", dtype=np.uint8)
+        p = str(tmp_path / f"a{i}.bin")
+        t.tofile(p)
+        paths.append(p)
+        datas.append(t)
+    words = [b"", b"a", b"aa", b"aab", b"b", b"A", b"_", b"a_b", b"yy", b"z", b"x" * 32, b"a" * 32, b"aab", b"cross-bun"]
+    for _ in range(40):
+        words.append(bytes(rng.choice(np.frombuffer(b"ab_A", dtype=np.uint8), size=int(rng.integers(1, 8)))))
+    d = np.zeros((len(words), 32), dtype=np.uint8)
+    for i, w in enumerate(words):
+        d[i, :len(w)] = np.frombuffer(w, dtype=np.uint8)
+    with make_ctx(page_size=4096, cache_bytes=8 << 20) as ctx:
+        counts, status = grep(ctx, paths, d, chunk, threads=256)
+        assert (status == 0).all(), status
+        for i, t in enumerate(datas):
+            assert (counts[i] == orc.scan_naive(t, d, chunk)).all(), i
+        assert ctx.check_invariants() == 0
+
+
+def test_grep_many_workgroups_one_file(tmp_path, orc):
+    """64 workgroups open and read the same file at once: concurrent opens,
+    shared pages and fills waited on by other workgroups."""
+    say("test_grep_many_workgroups_one_file")
+    voc = dg.vocab(3000, seed=41)
+    q = voc[dg.query(3000, 512, seed=42)]
+    p = str(tmp_path / "same.txt")
+    dg.text_files([p], [3 << 20], 43, voc)
+    cpu, _ = orc.scan_files([p], q, 65536)
+    with make_ctx(page_size=65536, cache_bytes=16 << 20, ring_depth=64) as ctx:
+        counts, status = grep(ctx, [p] * 64, q, 65536)
+        assert (status == 0).all(), status
+        assert (counts == cpu[0][None, :]).all()
+        s = ctx.stats()
+        assert s["page_misses"] == 48
+        assert s["rpc_opens"] >= 1
+        assert ctx.check_invariants() == 0

@@ -1,0 +1,125 @@
+"""GPU parity tests of gopen/gread/gclose through the C ABI: bytes returned by
+gread equal the file's bytes (bit-exact), at every page size, offset, length
+and cache state, with the page-cache invariants checked after each run."""
+import os
+
+import numpy as np
+import pytest
+
+from gpufs_amd import datagen as dg
+from tests.gpu_helpers import gread_ranges, make_ctx, read_file, say
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand_file(path, size, seed):
+    dg.random_file(str(path), size, seed, threads=4)
+    return np.fromfile(str(path), dtype=np.uint8).tobytes()
+
+
+@pytest.mark.parametrize("page", [4096, 65536])
+def test_read_whole_files(tmp_path, page):
+    say(f"test_read_whole_files page={page}")
+    sizes = [1, 100, 4095, 4096, 4097, 3 * 65536 + 5, 1 << 20]
+    with make_ctx(page_size=page, cache_bytes=16 << 20) as ctx:
+        for i, size in enumerate(sizes):
+            p = tmp_path / f"w{i}.bin"
+            data = _rand_file(p, size, 100 + i)
+            got, fsize, crc, out = read_file(ctx, str(p), 0, size)
+            assert (got, fsize, crc) == (size, size, 0), (size, got, fsize, crc)
+            assert out == data, size
+        assert ctx.check_invariants() == 0
+
+
+def test_read_offsets_eof_and_misaligned_dst(tmp_path):
+    say("test_read_offsets_eof_and_misaligned_dst")
+    size = 5 * 65536 + 1234
+    p = tmp_path / "o.bin"
+    data = _rand_file(p, size, 7)
+    rng = np.random.default_rng(1)
+    cases = [(0, 10), (1, 65535), (65535, 2), (65536 * 2 - 3, 70000), (size - 5, 100), (size, 10),
+             (size + 100, 10), (0, size + 1000)]
+    for _ in range(12):
+        off = int(rng.integers(0, size))
+        cases.append((off, int(rng.integers(1, 200000))))
+    with make_ctx(page_size=65536, cache_bytes=4 << 20) as ctx:
+        for k, (off, ln) in enumerate(cases):
+            pad = k % 7   # destination misaligned by 0..6 bytes
+            got, fsize, crc, out = read_file(ctx, str(p), off, ln, dst_pad=pad)
+            exp = data[off:off + ln] if off < size else b""
+            assert got == len(exp), (off, ln, got)
+            assert out == exp, (off, ln)
+        assert ctx.check_invariants() == 0
+
+
+def test_cold_then_warm_counts_hits(tmp_path):
+    say("test_cold_then_warm_counts_hits")
+    p = tmp_path / "cw.bin"
+    data = _rand_file(p, 1 << 20, 3)
+    with make_ctx(page_size=65536, cache_bytes=8 << 20) as ctx:
+        got, _, _, out = read_file(ctx, str(p), 0, 1 << 20)
+        s1 = ctx.stats()
+        assert out == data and s1["page_misses"] == 16 and s1["page_hits"] == 0
+        got, _, _, out = read_file(ctx, str(p), 0, 1 << 20)
+        s2 = ctx.stats()
+        assert out == data and s2["page_misses"] == 16 and s2["page_hits"] == 16
+        assert s2["rpc_opens"] == 2 and s2["rpc_closes"] == 2
+        ctx.cache_drop()
+        got, _, _, out = read_file(ctx, str(p), 0, 1 << 20)
+        assert out == data and ctx.stats()["page_misses"] == 32
+        assert ctx.check_invariants() == 0
+
+
+def test_gread_ranges_parity_cold_and_warm(tmp_path, orc):
+    say("test_gread_ranges_parity_cold_and_warm")
+    n, rb = 48, 1 << 20
+    p = tmp_path / "ranges.bin"
+    data = _rand_file(p, n * rb, 11)
+    fnv, _ = orc.pread_ranges(str(p), n, rb, threads=4)
+    with make_ctx(page_size=256 << 10, cache_bytes=96 << 20, ring_depth=256) as ctx:
+        for rnd in ("cold", "warm"):
+            out, dst = gread_ranges(ctx, str(p), n, rb)
+            assert (out == rb).all(), (rnd, out)
+            host = dst.cpu().numpy()
+            assert host.tobytes() == data, rnd
+            for r in range(n):
+                assert orc.fnv1a64(host[r * rb:(r + 1) * rb]) == int(fnv[r])
+        s = ctx.stats()
+        assert s["page_misses"] == n * 4 and s["page_hits"] == n * 4
+        assert ctx.check_invariants() == 0
+
+
+def test_eviction_with_small_cache(tmp_path):
+    say("test_eviction_with_small_cache")
+    n, rb = 16, 1 << 20
+    p = tmp_path / "ev.bin"
+    data = _rand_file(p, n * rb, 12)
+    with make_ctx(page_size=65536, cache_bytes=2 << 20, ring_depth=64) as ctx:
+        for _ in range(2):
+            out, dst = gread_ranges(ctx, str(p), n, rb, threads=256)
+            assert (out == rb).all(), out
+            assert dst.cpu().numpy().tobytes() == data
+        s = ctx.stats()
+        assert s["evictions"] > 0
+        assert ctx.check_invariants() == 0
+
+
+def test_open_missing_file_returns_ipc_error(tmp_path):
+    say("test_open_missing_file_returns_ipc_error")
+    with make_ctx() as ctx:
+        got, _, _, _ = read_file(ctx, str(tmp_path / "nope.bin"), 0, 10)
+        assert got == -2
+        assert ctx.check_invariants() == 0
+
+
+def test_reopen_sees_changed_file(tmp_path):
+    say("test_reopen_sees_changed_file")
+    p = tmp_path / "chg.bin"
+    a = _rand_file(p, 300000, 21)
+    with make_ctx(page_size=65536) as ctx:
+        assert read_file(ctx, str(p), 0, 300000)[3] == a
+        assert read_file(ctx, str(p), 0, 300000)[3] == a
+        b = _rand_file(p, 200000, 22)
+        got, fsize, _, out = read_file(ctx, str(p), 0, 300000)
+        assert (got, fsize) == (200000, 200000) and out == b
+        assert ctx.check_invariants() == 0

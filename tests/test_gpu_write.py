@@ -1,0 +1,64 @@
+"""GPU parity of the write path (gwrite / gfsync / gclose write-back, O_GWRONCE
+OR-merge): the bytes on disk after the kernel equal the expected bytes."""
+import os
+
+import numpy as np
+import pytest
+
+from gpufs_amd import O_GRDWR, O_GWRONCE, O_GWRONLY, O_GCREAT
+from tests.gpu_helpers import make_ctx, read_file, say, write_file
+
+pytestmark = pytest.mark.gpu
+
+
+def test_write_new_file_then_read_back(tmp_path):
+    say("test_write_new_file_then_read_back")
+    p = tmp_path / "new.bin"
+    data = np.random.default_rng(1).integers(0, 256, size=200000, dtype=np.uint8).tobytes()
+    with make_ctx(page_size=65536) as ctx:
+        r = write_file(ctx, str(p), 0, data, O_GWRONLY | O_GCREAT)
+        assert r[0] == len(data) and r[2] == 0, r
+        assert p.read_bytes() == data
+        got, fsize, _, out = read_file(ctx, str(p), 0, len(data))
+        assert got == len(data) and out == data
+        assert ctx.check_invariants() == 0
+
+
+def test_partial_page_read_modify_write(tmp_path):
+    say("test_partial_page_read_modify_write")
+    p = tmp_path / "rmw.bin"
+    base = np.random.default_rng(2).integers(0, 256, size=300000, dtype=np.uint8)
+    base.tofile(str(p))
+    patch = np.random.default_rng(3).integers(0, 256, size=5000, dtype=np.uint8).tobytes()
+    off = 65536 - 1000
+    with make_ctx(page_size=65536) as ctx:
+        r = write_file(ctx, str(p), off, patch, O_GRDWR, fsync=True)
+        assert r[0] == len(patch) and r[3] == 0 and r[2] == 0, r
+        exp = bytearray(base.tobytes())
+        exp[off:off + len(patch)] = patch
+        assert p.read_bytes() == bytes(exp)
+        assert ctx.check_invariants() == 0
+
+
+def test_write_extends_file(tmp_path):
+    say("test_write_extends_file")
+    p = tmp_path / "ext.bin"
+    p.write_bytes(b"abc")
+    with make_ctx(page_size=4096) as ctx:
+        r = write_file(ctx, str(p), 10000, b"XYZ", O_GRDWR)
+        assert r[0] == 3 and r[1] == 10003, r
+        exp = b"abc" + b"\0" * (10000 - 3) + b"XYZ"
+        assert p.read_bytes() == exp
+
+
+def test_wronce_or_merges_with_disk(tmp_path):
+    say("test_wronce_or_merges_with_disk")
+    p = tmp_path / "once.bin"
+    p.write_bytes(bytes([0x01, 0x10, 0x00, 0xF0]) * 1000)
+    new = bytes([0x02, 0x01, 0x80, 0x00]) * 500
+    with make_ctx(page_size=4096) as ctx:
+        r = write_file(ctx, str(p), 0, new, O_GWRONCE)
+        assert r[0] == len(new) and r[2] == 0, r
+    got = p.read_bytes()
+    assert got[:2000] == bytes([0x03, 0x11, 0x80, 0xF0]) * 500
+    assert got[2000:] == bytes([0x01, 0x10, 0x00, 0xF0]) * 500
