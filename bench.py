@@ -53,6 +53,12 @@ def parse():
     ap.add_argument("--cold-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grep", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--batched-files", type=int, default=10000)
+    ap.add_argument("--serve-files", type=int, default=100000)
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-rates", default="1000,10000,50000")
     ap.add_argument("--workdir", default=os.environ.get("GPUFS_BENCH_DIR", "/tmp/gpufs_bench"))
     return ap.parse_args()
 
@@ -199,6 +205,10 @@ def main():
 
     ctx.host_stop()
     ctx.close()
+    del dst
+    torch.cuda.empty_cache()
+    batched_res = bench_batched(args, torch, L, _lib, dg) if not args.no_batched else None
+    serve_res = bench_serve(args, torch, L, _lib, dg) if not args.no_serve else None
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -226,6 +236,8 @@ def main():
         "bit_exact": {"cold_fnv": cold_ok, "warm_fnv": warm_ok},
         "cpu_baseline": cpu,
         "grep": grep_res,
+        "batched": batched_res,
+        "served": serve_res,
     }
     print(json.dumps(line), flush=True)
 
@@ -286,6 +298,173 @@ def bench_grep(args, ctx, torch, L, _lib, dg):
         res["cpu_baseline"] = {"value": round(total / dt / 1e9, 3), "unit": "GB/s", "cores": ncores, "kind": "port"}
     log(f"grep: cold {res['cold_GBps']} GB/s, warm {res['warm_GBps']} GB/s")
     return res
+
+
+def _query_words(dg):
+    voc = dg.vocab(50000, seed=dg.SEED_VOCAB)
+    return voc, np.ascontiguousarray(voc[dg.query(50000, 1024, seed=dg.SEED_QUERY)])
+
+
+def _text_pool(dg, voc, sub, n, seed, workdir):
+    """n text files with log-normal sizes (median 64 KiB, sigma 1.5, clipped to
+    4 KiB .. 64 MiB), generated once per work directory."""
+    sizes = dg.lognormal_sizes(n, seed=seed + 1, median=65536.0, sigma=1.5, lo=4096, hi=64 << 20)
+    d = os.path.join(workdir, sub)
+    os.makedirs(d, exist_ok=True)
+    paths = [os.path.join(d, f"{i:06d}.txt") for i in range(n)]
+    stamp = os.path.join(d, "DONE")
+    if not os.path.exists(stamp) or open(stamp).read() != f"{n} {seed} {int(sizes.sum())}":
+        t = time.time()
+        dg.text_files(paths, sizes, seed, voc, threads=16)
+        open(stamp, "w").write(f"{n} {seed} {int(sizes.sum())}")
+        log(f"generated {n} files, {sizes.sum() / 2**30:.2f} GiB in {time.time() - t:.1f}s")
+    return paths, sizes
+
+
+def _names_dev(torch, paths):
+    a = np.zeros((len(paths), 256), dtype=np.uint8)
+    for i, p in enumerate(paths):
+        b = os.fsencode(p)
+        a[i, :len(b)] = np.frombuffer(b, dtype=np.uint8)
+    return torch.from_numpy(a).cuda()
+
+
+def bench_batched(args, torch, L, _lib, dg):
+    """Uneven files: one persistent kernel from a hipGraph vs one launch per file."""
+    from gpufs_amd import autotune
+    voc, q = _query_words(dg)
+    n = args.batched_files
+    paths, sizes = _text_pool(dg, voc, "batched", n, dg.SEED_C3, args.workdir)
+    subprocess.run(["cat", *paths[: min(n, 20000)]], stdout=subprocess.DEVNULL, check=False)
+    names_d = _names_dev(torch, paths)
+    dict_d = torch.from_numpy(q.reshape(-1)).cuda()
+    cache = 8 << 30
+    torch.cuda.synchronize()
+    sample = min(n, 2000)
+    best, table = autotune.tune_batched(names_d, sample, dict_d, 1024, 65536, cache, host_threads=args.host_threads,
+                                        log=log)
+    log(f"autotune picked {best}")
+    ctx = _lib.Context(cache_bytes=cache, page_size=best["page_size"], ring_depth=best["ring_depth"],
+                       host_threads=args.host_threads, max_files=2 * n)
+    counts = torch.zeros(n * 1024, dtype=torch.int32, device="cuda")
+    status = torch.zeros(n, dtype=torch.int32, device="cuda")
+    queue = torch.zeros(1, dtype=torch.int32, device="cuda")
+    times = torch.zeros(2 * n, dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()
+    ctx.host_start()
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    g = ctypes.c_void_p()
+    _lib.check("graph", L.gpufs_grep_batched_graph(
+        ctx.h, ctypes.c_void_p(names_d.data_ptr()), 256, n, ctypes.c_void_p(dict_d.data_ptr()), 1024, 65536,
+        ctypes.c_void_p(counts.data_ptr()), ctypes.c_void_p(status.data_ptr()), ctypes.c_void_p(queue.data_ptr()),
+        best["workgroups"], best["threads"], ctypes.c_void_p(times.data_ptr()), ctypes.byref(g)))
+
+    def replay():
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        _lib.check("graph launch", L.gpufs_graph_launch(g, ctx.stream))
+        e1.record(stream)
+        e1.synchronize()
+        assert (status.cpu().numpy() == 0).all()
+        return e0.elapsed_time(e1) / 1e3
+
+    ctx.cache_drop()
+    t_cold = replay()
+    tk = times.cpu().numpy().reshape(n, 2)
+    lat_cold = (tk[:, 1] - tk[:, 0]) / 1e8
+    c_b = counts.cpu().numpy().view(np.uint32).reshape(n, 1024).copy()
+    t_warm = replay()
+    c_w = counts.cpu().numpy().view(np.uint32).reshape(n, 1024)
+    assert (c_w == c_b).all()
+    L.gpufs_graph_destroy(g)
+    # plain form: one launch per file on the same stream, cold
+    ctx.cache_drop()
+    one = torch.zeros(1024, dtype=torch.int32, device="cuda")
+    st1 = torch.zeros(1, dtype=torch.int32, device="cuda")
+    n_plain = min(n, 2000)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for i in range(n_plain):
+        _lib.check("plain", L.gpufs_launch_grep(ctx.h, ctypes.c_void_p(names_d[i].data_ptr()), 256, 1,
+                                                ctypes.c_void_p(dict_d.data_ptr()), 1024, 65536,
+                                                ctypes.c_void_p(one.data_ptr()), ctypes.c_void_p(st1.data_ptr()),
+                                                1024, None))
+    ev = torch.cuda.Event()
+    ev.record(stream)
+    ev.synchronize()
+    t_plain = time.perf_counter() - t
+    ctx.host_stop()
+    ctx.close()
+    res = {"files": n, "bytes": int(sizes.sum()), "autotuned": best, "autotune_table": table,
+           "batched_cold_files_per_s": round(n / t_cold, 1), "batched_cold_GBps": round(sizes.sum() / t_cold / 1e9, 3),
+           "batched_warm_files_per_s": round(n / t_warm, 1),
+           "file_latency_cold_p50_ms": round(float(np.percentile(lat_cold, 50)) * 1e3, 4),
+           "file_latency_cold_p99_ms": round(float(np.percentile(lat_cold, 99)) * 1e3, 4),
+           "plain_cold_files_per_s": round(n_plain / t_plain, 1), "plain_files_measured": n_plain}
+    res["speedup_vs_plain"] = round(res["batched_cold_files_per_s"] / res["plain_cold_files_per_s"], 2)
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        ncores = min(16, os.cpu_count() or 1)
+        t = time.perf_counter()
+        cpu, st = orc.scan_files(paths, q, 65536, threads=ncores)
+        dt = time.perf_counter() - t
+        res["bit_exact"] = bool((cpu == c_b).all() and (st == 0).all())
+        assert res["bit_exact"], "batched counts differ from the CPU path"
+        res["cpu_baseline"] = {"files_per_s": round(n / dt, 1), "cores": ncores, "kind": "port"}
+    log(f"batched: {res['batched_cold_files_per_s']} files/s cold, plain {res['plain_cold_files_per_s']} files/s, "
+        f"x{res['speedup_vs_plain']}")
+    return res
+
+
+def bench_serve(args, torch, L, _lib, dg):
+    """Served reads: Poisson arrivals, Zipf(1.1) popularity, resident cache."""
+    from gpufs_amd.serve import Server
+    voc, q = _query_words(dg)
+    n = args.serve_files
+    paths, sizes = _text_pool(dg, voc, "served", n, dg.SEED_C4, args.workdir)
+    ctx = _lib.Context(cache_bytes=32 << 30, page_size=64 << 10, ring_depth=1024, host_threads=args.host_threads,
+                       max_files=2 * n)
+    srv = Server(ctx, paths, sizes, q, workgroups=256, slots=1024)
+    out = {"files": n, "bytes": int(sizes.sum()), "cache_bytes": 32 << 30, "page_bytes": 64 << 10, "rates": []}
+    all_ids, all_counts = [], []
+    try:
+        for k, rate in enumerate(float(r) for r in args.serve_rates.split(",")):
+            nreq = max(100, int(rate * args.serve_seconds))
+            ids = dg.zipf_sample(n, nreq, seed=dg.SEED_C4 + 10 + k, s=1.1)
+            arr = np.cumsum(dg.poisson_gaps(rate, nreq, seed=dg.SEED_C4 + 20 + k))
+            s0 = ctx.stats()
+            c, st, lat, el = srv.run(ids, arr)
+            s1 = ctx.stats()
+            assert (st == 0).all()
+            hits = s1["page_hits"] - s0["page_hits"]
+            miss = s1["page_misses"] - s0["page_misses"]
+            r = {"rate_req_s": rate, "requests": nreq, "files_per_s": round(nreq / el, 1),
+                 "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 4),
+                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 4),
+                 "hit_ratio": round(hits / max(1, hits + miss), 4)}
+            out["rates"].append(r)
+            log(f"served @ {rate:.0f}/s: {r}")
+            all_ids.append(ids)
+            all_counts.append(c)
+    finally:
+        srv.stop()
+        ctx.close()
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        ids = np.concatenate(all_ids)
+        c = np.concatenate(all_counts)
+        uniq = np.unique(ids)
+        ncores = min(16, os.cpu_count() or 1)
+        t = time.perf_counter()
+        cpu, cst = orc.scan_files([paths[i] for i in uniq], q, 65536, threads=ncores)
+        dt = time.perf_counter() - t
+        row = {int(f): i for i, f in enumerate(uniq)}
+        ok = all((c[j] == cpu[row[int(f)]]).all() for j, f in enumerate(ids))
+        out["bit_exact"] = bool(ok and (cst == 0).all())
+        assert out["bit_exact"], "served counts differ from the CPU path"
+        out["cpu_baseline"] = {"files_per_s": round(len(uniq) / dt, 1), "cores": ncores, "kind": "port",
+                               "sample": f"{len(uniq)} distinct requested files, pread + scan, closed loop"}
+    return out
 
 
 if __name__ == "__main__":

@@ -42,6 +42,11 @@ class Stats(ctypes.Structure):
         return {n: int(getattr(self, n)) for n in STATS_FIELDS}
 
 
+class ServeConfig(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in
+                ("workgroups", "threads", "slots", "chunk_bytes", "n_words", "frame_budget")]
+
+
 # name -> (restype, argtypes) of every exported entry point
 P, I, U32, U64, I32, I64 = (ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64,
                             ctypes.c_int32, ctypes.c_int64)
@@ -62,6 +67,14 @@ SIGNATURES = {
     "gpufs_strerror": (ctypes.c_char_p, [I]),
     "gpufs_launch_gread_ranges": (I, [P, P, U32, U64, P, P, U32, P]),
     "gpufs_launch_grep": (I, [P, P, U32, U32, P, U32, U32, P, P, U32, P]),
+    "gpufs_launch_grep_batched": (I, [P, P, U32, U32, P, U32, U32, P, P, P, U32, U32, P, P]),
+    "gpufs_grep_batched_graph": (I, [P, P, U32, U32, P, U32, U32, P, P, P, U32, U32, P, ctypes.POINTER(P)]),
+    "gpufs_graph_launch": (I, [P, P]),
+    "gpufs_graph_destroy": (I, [P]),
+    "gpufs_serve_start": (I, [P, ctypes.POINTER(ServeConfig), P, U32, U32, P, P, ctypes.POINTER(P)]),
+    "gpufs_serve_run": (I, [P, P, P, U32, P, P, P, ctypes.POINTER(ctypes.c_double)]),
+    "gpufs_serve_stop": (I, [P]),
+    "gpufs_launch_gemv_mmap": (I, [P, P, U64, U64, P, P, U32, P, P, U32, P]),
     "gpufs_fnv1a64_ranges": (I, [P, U32, U64, P, P]),
     "gpufs_launch_read": (I, [P, P, I32, U64, U64, P, P, U32, P]),
     "gpufs_launch_write": (I, [P, P, I32, U64, U64, P, I32, P, U32, P]),

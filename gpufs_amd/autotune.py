@@ -1,0 +1,82 @@
+"""Autotuner for the batched form: page size, workgroup count / size and RPC
+ring depth are chosen by measuring the real workload on a sample of its files.
+
+Search: an exhaustive sweep of a small grid (page size x threads per
+workgroup x workgroups x ring depth), each point run cold (page cache
+emptied, host page cache warm) on the same file sample; the point with the
+highest files/s wins.  The grid is kept small on purpose: every point builds
+a fresh context (the page size is fixed per context).
+"""
+import ctypes
+import itertools
+import time
+
+import numpy as np
+
+from . import _lib
+
+DEFAULT_SPACE = {
+    "page_size": [64 << 10, 256 << 10, 1 << 20],
+    "threads": [512, 1024],
+    "workgroups_per_cu": [1],
+    "ring_depth": [256, 1024],
+}
+
+
+def _device_cus():
+    import torch
+    return torch.cuda.get_device_properties(0).multi_processor_count
+
+
+def batched_trial(cfg, names_d, n_files, dict_d, n_words, chunk, cache_bytes, host_threads, torch):
+    """One cold run of the batched scan with configuration `cfg`; returns files/s."""
+    L = _lib.lib()
+    ctx = _lib.Context(cache_bytes=cache_bytes, page_size=cfg["page_size"], ring_depth=cfg["ring_depth"],
+                       host_threads=host_threads, max_files=max(1024, 2 * n_files))
+    try:
+        counts = torch.zeros(n_files * n_words, dtype=torch.int32, device="cuda")
+        status = torch.zeros(n_files, dtype=torch.int32, device="cuda")
+        queue = torch.zeros(1, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        ctx.host_start()
+        stream = torch.cuda.ExternalStream(ctx.stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        rc = L.gpufs_launch_grep_batched(ctx.h, ctypes.c_void_p(names_d.data_ptr()), 256, n_files,
+                                         ctypes.c_void_p(dict_d.data_ptr()), n_words, chunk,
+                                         ctypes.c_void_p(counts.data_ptr()), ctypes.c_void_p(status.data_ptr()),
+                                         ctypes.c_void_p(queue.data_ptr()), cfg["workgroups"], cfg["threads"], None,
+                                         None)
+        _lib.check("gpufs_launch_grep_batched", rc)
+        e1.record(stream)
+        e1.synchronize()
+        dt = e0.elapsed_time(e1) / 1e3
+        ok = bool((status.cpu().numpy() == 0).all())
+        ctx.host_stop()
+    finally:
+        ctx.close()
+    return (n_files / dt if ok else 0.0), dt
+
+
+def tune_batched(names_d, n_files, dict_d, n_words, chunk, cache_bytes, host_threads=16, space=None, log=None):
+    """Sweep `space` (see DEFAULT_SPACE) on the given files.  Returns
+    (best_cfg, table) where table lists every point with its files/s."""
+    import torch
+    space = dict(DEFAULT_SPACE, **(space or {}))
+    cus = _device_cus()
+    table = []
+    best = None
+    for page, threads, wpc, ring in itertools.product(space["page_size"], space["threads"],
+                                                      space["workgroups_per_cu"], space["ring_depth"]):
+        if page * ring > (1 << 30):   # pinned staging kept at or below 1 GiB
+            continue
+        cfg = {"page_size": page, "threads": threads, "workgroups": int(wpc * cus), "ring_depth": ring}
+        t = time.time()
+        fps, dt = batched_trial(cfg, names_d, n_files, dict_d, n_words, chunk, cache_bytes, host_threads, torch)
+        row = dict(cfg, files_per_s=round(fps, 1), kernel_s=round(dt, 5))
+        table.append(row)
+        if log:
+            log(f"autotune {row} ({time.time() - t:.1f}s)")
+        if best is None or fps > best[1]:
+            best = (cfg, fps)
+    return best[0], table

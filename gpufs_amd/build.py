@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("GPUFS_OFFLOAD_ARCH", "gfx950")
-HIP_SOURCES = ["runtime.hip", "k_gread.hip", "k_grep.hip"]
+HIP_SOURCES = ["runtime.hip", "k_gread.hip", "k_grep.hip", "k_gemv.hip", "k_serve.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",

@@ -1,0 +1,108 @@
+// k_gemv.hip -- y = A x with A gmmap()ed page by page through the page cache
+// (the access pattern of samples/bigmatrix_mmap/bigmatrix_mmap.cu:66-168:
+// each workgroup owns a block of rows, maps the page holding the next piece of
+// its row, consumes it, unmaps it).  Differences by design: x lives in device
+// memory, the reduction order is fixed (per-thread partials in a fixed column
+// assignment, then a fixed butterfly and a fixed cross-wave sum) so y is
+// reproducible run to run, and every mapped 8-byte word is added into a
+// per-workgroup 64-bit checksum so the mapped bytes can be verified exactly.
+#include "gpufs.h"
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__global__ __launch_bounds__(1024) void k_gemv_mmap(const char* path, uint64_t rows, uint64_t cols, const float* x,
+                                                    float* y, uint32_t rows_per_wg, int32_t* status,
+                                                    unsigned long long* wsum) {
+    __shared__ float part[16];
+    __shared__ unsigned long long upart[16];
+    __shared__ int st;
+    const uint32_t t = gfs::tid(), T = gfs::nthreads(), w = t >> 6, nw = (T + 63) >> 6;
+    int fd = gopen(path, O_GRDONLY);
+    if (fd < 0) {
+        if (t == 0) status[blockIdx.x] = fd;
+        return;
+    }
+    if (t == 0) st = (fstat(fd) < rows * cols * 4) ? -EINVAL : 0;
+    __syncthreads();
+    const uint64_t PS = gfs_g->page_size;
+    const uint64_t rb = cols * 4;
+    unsigned long long csum = 0;
+    const uint64_t r0 = (uint64_t)blockIdx.x * rows_per_wg;
+    for (uint64_t r = r0; r < r0 + rows_per_wg && r < rows && st == 0; ++r) {
+        float acc = 0.f;
+        const uint64_t row_end = (r + 1) * rb;
+        for (uint64_t off = r * rb; off < row_end;) {
+            const uint64_t pstart = off & ~(PS - 1);
+            const uint64_t len = (pstart + PS < row_end ? pstart + PS : row_end) - off;
+            volatile void* m = gmmap(nullptr, len, 0, O_GRDONLY, fd, (off_t)off);
+            if (m == GMAP_FAILED) {
+                if (t == 0) st = -EIO;
+                break;
+            }
+            const float* p = (const float*)(uintptr_t)m;
+            const float* xx = x + (off - r * rb) / 4;
+            const uint64_t n4 = len / 16;
+            for (uint64_t i = t; i < n4; i += T) {
+                float4 a = ((const float4*)p)[i];
+                float4 b = ((const float4*)xx)[i];
+                acc = fmaf(a.x, b.x, acc);
+                acc = fmaf(a.y, b.y, acc);
+                acc = fmaf(a.z, b.z, acc);
+                acc = fmaf(a.w, b.w, acc);
+                const unsigned long long* u = (const unsigned long long*)&a;
+                csum += u[0] + u[1];
+            }
+            for (uint64_t i = n4 * 4 + t; i < len / 4; i += T) {
+                acc = fmaf(p[i], xx[i], acc);
+                csum += (unsigned long long)((const uint32_t*)p)[i];
+            }
+            gmunmap(m, len);
+            off += len;
+        }
+        float s = wave_sum(acc);
+        if ((t & 63) == 0) part[w] = s;
+        __syncthreads();
+        if (t == 0) {
+            float tot = 0.f;
+            for (uint32_t k = 0; k < nw; ++k) tot += part[k];
+            y[r] = tot;
+        }
+        __syncthreads();
+    }
+    unsigned long long cs = wave_sum_u64(csum);
+    if ((t & 63) == 0) upart[w] = cs;
+    __syncthreads();
+    int c = gclose(fd);
+    if (t == 0) {
+        unsigned long long tot = 0;
+        for (uint32_t k = 0; k < nw; ++k) tot += upart[k];
+        wsum[blockIdx.x] = tot;
+        status[blockIdx.x] = st ? st : c;
+    }
+}
+
+extern "C" int gpufs_launch_gemv_mmap(gpufs_ctx* ctx, const char* path_dev, uint64_t rows, uint64_t cols,
+                                      const float* x_dev, float* y_dev, uint32_t rows_per_wg,
+                                      int32_t* status_dev, uint64_t* checksum_dev, uint32_t threads,
+                                      void* stream) {
+    if (!ctx || !path_dev || !x_dev || !y_dev || !status_dev || !checksum_dev) return -EINVAL;
+    if (rows == 0 || cols == 0 || rows_per_wg == 0 || (cols % 4)) return -EINVAL;
+    if (threads < 64 || threads > 1024 || threads % 64) return -EINVAL;
+    int v = gpufs_bind(ctx);
+    if (v) return v;
+    const uint64_t nwg = (rows + rows_per_wg - 1) / rows_per_wg;
+    if (nwg > 0x7FFFFFFF) return -EINVAL;
+    hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)gpufs_stream(ctx);
+    hipLaunchKernelGGL(k_gemv_mmap, dim3((uint32_t)nwg), dim3(threads), 0, s, path_dev, rows, cols, x_dev, y_dev,
+                       rows_per_wg, status_dev, (unsigned long long*)checksum_dev);
+    return hipGetLastError() == hipSuccess ? 0 : -EIO;
+}

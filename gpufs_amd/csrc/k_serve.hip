@@ -1,0 +1,248 @@
+// k_serve.hip -- serving form: a resident request loop on the GPU fed by an
+// open-loop host driver with continuous admission.
+//
+// Device: `workgroups` persistent workgroups each take the next request ticket
+// from a device counter, wait for the host to publish that ticket in the
+// request ring (coherent pinned host memory), run open -> read -> scan -> close
+// on the requested file through the page cache (grep_one_file, shared with
+// the plain and batched forms), write the per-word counts straight into the
+// request's result row in pinned host memory and publish a completion word.
+// Page cache and file table stay resident across requests (and across runs
+// of the same server).
+//
+// Host (gpufs_serve_run): releases requests at their arrival times, admits
+// them FIFO while a request slot is free and the frames they may need fit the
+// frame budget (admission by free cache frames), collects completions,
+// releases slot and frames, and timestamps each request (arrival -> result).
+// Requests arriving while others run are admitted as soon as slots and
+// frames free up, so their work overlaps on the device.
+#include "grep_scan.h"
+
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <new>
+#include <vector>
+
+#define SERVE_STOP 0xFFFFFFFFu
+
+struct serve_req {
+    uint64_t seq;       // ticket + 1 once published
+    uint32_t file_id;   // SERVE_STOP ends the workgroup
+    uint32_t slot;
+};
+
+__global__ __launch_bounds__(1024) void k_serve(const char* names, uint32_t stride, uint32_t nfiles,
+                                                const char* dict, uint32_t nw, uint32_t chunk,
+                                                serve_req* ring, uint32_t ring_mask, uint32_t* results,
+                                                int32_t* res_status, uint64_t* done,
+                                                unsigned long long* ticket, uint64_t idle_ticks) {
+    __shared__ __attribute__((aligned(16))) grep_lds S;
+    __shared__ uint32_t fid, slot;
+    __shared__ uint64_t myseq;
+    grep_build_dict(S, dict, nw);
+    const uint32_t t = gfs::tid();
+    for (;;) {
+        if (t == 0) {
+            unsigned long long k = atomicAdd(ticket, 1ull);
+            serve_req* e = &ring[k & ring_mask];
+            uint64_t t0 = gfs::now_ticks();
+            uint32_t spins = 0;
+            fid = SERVE_STOP;
+            while (gfs::sys_ld_acq(&e->seq) != k + 1) {
+                if (++spins > 64) __builtin_amdgcn_s_sleep(127);
+                else __builtin_amdgcn_s_sleep(8);
+                if ((spins & 63) == 0 && gfs::now_ticks() - t0 > idle_ticks) break;
+            }
+            if (gfs::sys_ld(&e->seq) == k + 1) {
+                fid = gfs::sys_ld(&e->file_id);
+                slot = gfs::sys_ld(&e->slot);
+            }
+            myseq = k + 1;
+        }
+        __syncthreads();
+        const uint32_t f = fid, s = slot;
+        const uint64_t sq = myseq;
+        __syncthreads();
+        if (f == SERVE_STOP) break;
+        int st = f < nfiles ? grep_one_file(S, names + (size_t)f * stride, nw, chunk, results + (size_t)s * nw)
+                            : -EINVAL;
+        __threadfence_system();
+        __syncthreads();
+        if (t == 0) {
+            gfs::sys_st(&res_status[s], st);
+            gfs::fence_rel_sys();
+            gfs::sys_st_rel(&done[s], sq);
+        }
+    }
+}
+
+struct gpufs_server {
+    gpufs_ctx* ctx;
+    hipStream_t stream;
+    uint32_t workgroups, threads, slots, nw, chunk, nfiles, stride;
+    uint64_t frame_budget, page_size;
+    std::vector<uint64_t> sizes;
+    serve_req* ring;     // host coherent
+    uint32_t ring_mask;
+    uint32_t* results;   // host coherent: slots x nw
+    int32_t* rstatus;    // host coherent: slots
+    uint64_t* done;      // host coherent: slots
+    unsigned long long* ticket;   // device
+    uint64_t next_ticket;
+    bool running;
+};
+
+static uint64_t serve_now_ns() {
+    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static void serve_publish(gpufs_server* S, uint32_t file_id, uint32_t slot) {
+    uint64_t k = S->next_ticket++;
+    serve_req* e = &S->ring[k & S->ring_mask];
+    e->file_id = file_id;
+    e->slot = slot;
+    __atomic_store_n(&e->seq, k + 1, __ATOMIC_RELEASE);
+}
+
+extern "C" int gpufs_serve_start(gpufs_ctx* ctx, const gpufs_serve_config* cfg, const char* names_dev,
+                                 uint32_t name_stride, uint32_t n_files, const uint64_t* file_sizes,
+                                 const char* dict_dev, gpufs_server** out) {
+    if (!ctx || !cfg || !names_dev || !file_sizes || !dict_dev || !out || n_files == 0) return -EINVAL;
+    if (cfg->workgroups == 0 || cfg->slots == 0 || cfg->slots > 65536) return -EINVAL;
+    int v = grep_args_ok(n_files, cfg->n_words, cfg->chunk_bytes, cfg->threads);
+    if (v) return v;
+    if ((v = gpufs_bind(ctx))) return v;
+    gpufs_config cc;
+    gpufs_get_config(ctx, &cc);
+    gpufs_server* S = new (std::nothrow) gpufs_server();
+    if (!S) return -ENOMEM;
+    S->ctx = ctx;
+    S->stream = (hipStream_t)gpufs_stream(ctx);
+    S->workgroups = cfg->workgroups;
+    S->threads = cfg->threads;
+    S->slots = cfg->slots;
+    S->nw = cfg->n_words;
+    S->chunk = cfg->chunk_bytes;
+    S->nfiles = n_files;
+    S->stride = name_stride;
+    S->page_size = cc.page_size;
+    const uint64_t frames = cc.cache_bytes / cc.page_size;
+    S->frame_budget = cfg->frame_budget ? cfg->frame_budget : frames - frames / 10;
+    S->sizes.assign(file_sizes, file_sizes + n_files);
+    uint32_t rs = 1;
+    while (rs < 4 * (cfg->slots + cfg->workgroups)) rs <<= 1;
+    S->ring_mask = rs - 1;
+    const unsigned hf = hipHostMallocMapped | hipHostMallocCoherent;
+    bool ok = hipHostMalloc((void**)&S->ring, sizeof(serve_req) * rs, hf) == hipSuccess &&
+              hipHostMalloc((void**)&S->results, sizeof(uint32_t) * (size_t)cfg->slots * cfg->n_words, hf) == hipSuccess &&
+              hipHostMalloc((void**)&S->rstatus, sizeof(int32_t) * cfg->slots, hf) == hipSuccess &&
+              hipHostMalloc((void**)&S->done, sizeof(uint64_t) * cfg->slots, hf) == hipSuccess &&
+              hipMalloc((void**)&S->ticket, sizeof(unsigned long long)) == hipSuccess;
+    if (!ok) {
+        (void)hipHostFree(S->ring); (void)hipHostFree(S->results); (void)hipHostFree(S->rstatus);
+        (void)hipHostFree(S->done); (void)hipFree(S->ticket);
+        delete S;
+        return -ENOMEM;
+    }
+    memset(S->ring, 0, sizeof(serve_req) * rs);
+    memset(S->done, 0, sizeof(uint64_t) * cfg->slots);
+    if (hipMemsetAsync(S->ticket, 0, sizeof(unsigned long long), S->stream) != hipSuccess) return -EIO;
+    S->next_ticket = 0;
+    gpufs_host_start(ctx);
+    // idle workgroups give up after twice the RPC wait limit without a request
+    gfs_dev hd;
+    if (hipMemcpy(&hd, gpufs_device_state(ctx), sizeof(hd), hipMemcpyDeviceToHost) != hipSuccess) return -EIO;
+    const uint64_t idle = hd.wait_ticks * 2;
+    hipLaunchKernelGGL(k_serve, dim3(S->workgroups), dim3(S->threads), 0, S->stream, names_dev, name_stride,
+                       n_files, dict_dev, cfg->n_words, cfg->chunk_bytes, S->ring, S->ring_mask, S->results,
+                       S->rstatus, S->done, S->ticket, idle);
+    if (hipGetLastError() != hipSuccess) return -EIO;
+    S->running = true;
+    *out = S;
+    return 0;
+}
+
+// Open-loop run: request i arrives arrival_s[i] seconds after the start
+// (non-decreasing).  Fills counts_out (n x n_words), status_out, latency_s_out
+// (arrival -> completion observed by the host); *elapsed_s_out = first
+// arrival to last completion.
+extern "C" int gpufs_serve_run(gpufs_server* S, const uint32_t* file_ids, const double* arrival_s,
+                               uint32_t n, uint32_t* counts_out, int32_t* status_out, double* latency_s_out,
+                               double* elapsed_s_out) {
+    if (!S || !S->running || !file_ids || !arrival_s || !counts_out || !status_out || !latency_s_out) return -EINVAL;
+    std::vector<int64_t> slot_req(S->slots, -1);
+    std::vector<uint64_t> slot_frames(S->slots, 0), slot_seq(S->slots, 0);
+    std::vector<uint32_t> free_slots;
+    for (uint32_t s = S->slots; s-- > 0;) free_slots.push_back(s);
+    std::deque<uint32_t> pending;
+    uint64_t reserved = 0;
+    uint32_t next_arrival = 0, completed = 0, inflight = 0;
+    const uint64_t t0 = serve_now_ns();
+    auto frames_for = [&](uint32_t i) -> uint64_t {
+        uint32_t f = file_ids[i];
+        uint64_t sz = f < S->nfiles ? S->sizes[f] : 0;
+        uint64_t p = (sz + S->page_size - 1) / S->page_size;
+        return p < 1 ? 1 : p;
+    };
+    uint64_t last_done = t0;
+    uint64_t guard_t = t0;
+    while (completed < n) {
+        const uint64_t now = serve_now_ns();
+        while (next_arrival < n && (double)(now - t0) * 1e-9 >= arrival_s[next_arrival]) pending.push_back(next_arrival++);
+        // completions
+        for (uint32_t s = 0; s < S->slots; ++s) {
+            if (slot_req[s] < 0) continue;
+            if (__atomic_load_n(&S->done[s], __ATOMIC_ACQUIRE) != slot_seq[s]) continue;
+            const uint32_t i = (uint32_t)slot_req[s];
+            const uint64_t tc = serve_now_ns();
+            memcpy(counts_out + (size_t)i * S->nw, S->results + (size_t)s * S->nw, sizeof(uint32_t) * S->nw);
+            status_out[i] = S->rstatus[s];
+            latency_s_out[i] = (double)(tc - t0) * 1e-9 - arrival_s[i];
+            last_done = tc;
+            reserved -= slot_frames[s];
+            slot_req[s] = -1;
+            free_slots.push_back(s);
+            ++completed;
+            --inflight;
+            guard_t = tc;
+        }
+        // admission: FIFO, a free slot and the frame budget
+        while (!pending.empty() && !free_slots.empty()) {
+            const uint32_t i = pending.front();
+            const uint64_t need = frames_for(i);
+            if (inflight > 0 && reserved + need > S->frame_budget) break;
+            pending.pop_front();
+            const uint32_t s = free_slots.back();
+            free_slots.pop_back();
+            slot_req[s] = i;
+            slot_frames[s] = need;
+            reserved += need;
+            slot_seq[s] = S->next_ticket + 1;
+            serve_publish(S, file_ids[i], s);
+            ++inflight;
+        }
+        if (inflight > 0 && serve_now_ns() - guard_t > 120ull * 1000000000ull) return -ETIMEDOUT;
+        if (inflight == 0) guard_t = serve_now_ns();
+    }
+    if (elapsed_s_out) *elapsed_s_out = (double)(last_done - t0) * 1e-9 - (n ? arrival_s[0] : 0.0);
+    return 0;
+}
+
+extern "C" int gpufs_serve_stop(gpufs_server* S) {
+    if (!S) return -EINVAL;
+    int rc = 0;
+    if (S->running) {
+        for (uint32_t w = 0; w < S->workgroups; ++w) serve_publish(S, SERVE_STOP, 0);
+        if (hipStreamSynchronize(S->stream) != hipSuccess) rc = -EIO;
+        S->running = false;
+    }
+    (void)hipHostFree(S->ring);
+    (void)hipHostFree(S->results);
+    (void)hipHostFree(S->rstatus);
+    (void)hipHostFree(S->done);
+    (void)hipFree(S->ticket);
+    delete S;
+    return rc;
+}

@@ -126,6 +126,68 @@ int gpufs_launch_grep(gpufs_ctx* ctx, const char* names_dev, uint32_t name_strid
                       uint32_t chunk_bytes, uint32_t* counts_dev, int32_t* status_dev,
                       uint32_t threads, void* stream);
 
+/* Batched form of the grep scan over many files: `workgroups` persistent
+ * workgroups build the dictionary table once, then pull file indices from the
+ * device counter *queue_dev (reset to 0 by the launch itself) until n_files are
+ * done.  Same per-file outputs and status as gpufs_launch_grep.  When
+ * file_times_dev is not NULL it receives each file's start and end wall-clock
+ * ticks (2 per file, 100 MHz).  No reference counterpart (the reference
+ * launches one grid over a fixed file list). */
+int gpufs_launch_grep_batched(gpufs_ctx* ctx, const char* names_dev, uint32_t name_stride,
+                              uint32_t n_files, const char* dict_dev, uint32_t n_words,
+                              uint32_t chunk_bytes, uint32_t* counts_dev, int32_t* status_dev,
+                              uint32_t* queue_dev, uint32_t workgroups, uint32_t threads,
+                              uint64_t* file_times_dev, void* stream);
+/* The same two launches (queue reset, persistent kernel) captured on the
+ * context stream into an executable hipGraph; replay with gpufs_graph_launch.
+ * The host service must be running (gpufs_host_start) while a replay runs. */
+int gpufs_grep_batched_graph(gpufs_ctx* ctx, const char* names_dev, uint32_t name_stride,
+                             uint32_t n_files, const char* dict_dev, uint32_t n_words,
+                             uint32_t chunk_bytes, uint32_t* counts_dev, int32_t* status_dev,
+                             uint32_t* queue_dev, uint32_t workgroups, uint32_t threads,
+                             uint64_t* file_times_dev, void** graph_exec_out);
+int gpufs_graph_launch(void* graph_exec, void* stream);
+int gpufs_graph_destroy(void* graph_exec);
+
+/* ---- serving form: a resident request loop (no reference counterpart).
+ * gpufs_serve_start launches `workgroups` persistent workgroups on the context
+ * stream and starts the host service; each request (file id) is an
+ * open -> gread -> whole-word scan -> close of that file, with the grep
+ * semantics above.  gpufs_serve_run drives an open-loop arrival schedule:
+ * request i becomes eligible arrival_s[i] seconds after the start and is
+ * admitted (FIFO) when a request slot is free and its pages fit the frame
+ * budget; results land in counts_out[i*n_words ...], latency_s_out[i] is
+ * arrival -> completion seen by the host.  Page cache and file table stay
+ * resident across requests and runs.  gpufs_serve_stop ends the workgroups. */
+typedef struct gpufs_server gpufs_server;
+typedef struct gpufs_serve_config {
+    uint32_t workgroups;    /* persistent server workgroups */
+    uint32_t threads;       /* threads per workgroup (multiple of 64, <= 1024) */
+    uint32_t slots;         /* request slots: most requests in flight at once */
+    uint32_t chunk_bytes;   /* scan window */
+    uint32_t n_words;       /* dictionary words (32-byte records) */
+    uint32_t frame_budget;  /* frames in-flight requests may need; 0 = 90% of the pool */
+} gpufs_serve_config;
+int gpufs_serve_start(gpufs_ctx* ctx, const gpufs_serve_config* cfg, const char* names_dev,
+                      uint32_t name_stride, uint32_t n_files, const uint64_t* file_sizes_host,
+                      const char* dict_dev, gpufs_server** out);
+int gpufs_serve_run(gpufs_server* srv, const uint32_t* file_ids_host, const double* arrival_s_host,
+                    uint32_t n_requests, uint32_t* counts_out_host, int32_t* status_out_host,
+                    double* latency_s_out_host, double* elapsed_s_out);
+int gpufs_serve_stop(gpufs_server* srv);
+
+/* y = A x with A (rows x cols fp32, row-major) gmmap()ed page by page from
+ * `path_dev` (samples/bigmatrix_mmap/bigmatrix_mmap.cu:66-168 pattern): workgroup
+ * w owns rows [w*rows_per_wg, (w+1)*rows_per_wg).  x_dev has cols floats in
+ * device memory.  The reduction order is fixed, so y is reproducible.
+ * checksum_dev[w] = sum mod 2^64 of every mapped 8-byte word of the
+ * workgroup's rows (order-independent check of the mapped bytes);
+ * status_dev[w] = 0 or a negative error code. */
+int gpufs_launch_gemv_mmap(gpufs_ctx* ctx, const char* path_dev, uint64_t rows, uint64_t cols,
+                           const float* x_dev, float* y_dev, uint32_t rows_per_wg,
+                           int32_t* status_dev, uint64_t* checksum_dev, uint32_t threads,
+                           void* stream);
+
 /* FNV-1a-64 of n_ranges consecutive ranges of a device buffer (the per-range
  * checksum of the sequential-read workload; no reference counterpart). */
 int gpufs_fnv1a64_ranges(const void* data_dev, uint32_t n_ranges, uint64_t range_bytes,

@@ -124,3 +124,40 @@ def grep(ctx, paths, dict_words, chunk, threads=1024):
     _lib.check("gpufs_launch_grep", rc)
     finish(ctx)
     return counts.cpu().numpy().view(np.uint32).reshape(len(paths), nw), status.cpu().numpy()
+
+
+def gemv_mmap(ctx, path, rows, cols, x, rows_per_wg, threads=1024):
+    torch = torch_cuda()
+    name = dev_names([path])
+    xd = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).cuda()
+    y = torch.zeros(rows, dtype=torch.float32, device="cuda")
+    nwg = (rows + rows_per_wg - 1) // rows_per_wg
+    status = dev_zeros(nwg, torch.int32)
+    csum = dev_zeros(nwg, torch.int64)
+    torch.cuda.synchronize()
+    rc = _lib.lib().gpufs_launch_gemv_mmap(ctx.h, ctypes.c_void_p(name.data_ptr()), rows, cols,
+                                           ctypes.c_void_p(xd.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                           rows_per_wg, ctypes.c_void_p(status.data_ptr()),
+                                           ctypes.c_void_p(csum.data_ptr()), threads, None)
+    _lib.check("gpufs_launch_gemv_mmap", rc)
+    finish(ctx)
+    return y.cpu().numpy(), status.cpu().numpy(), csum.cpu().numpy().view(np.uint64)
+
+
+def grep_batched(ctx, names_t, nfiles, dict_t, nw, chunk, counts, status, queue, workgroups, threads=1024,
+                 graph=False):
+    """names_t/dict_t/counts/status/queue: device tensors.  Returns a graph
+    handle when graph=True (caller replays and destroys it)."""
+    L = _lib.lib()
+    args = (ctx.h, ctypes.c_void_p(names_t.data_ptr()), STRIDE, nfiles, ctypes.c_void_p(dict_t.data_ptr()), nw,
+            chunk, ctypes.c_void_p(counts.data_ptr()), ctypes.c_void_p(status.data_ptr()),
+            ctypes.c_void_p(queue.data_ptr()), workgroups, threads)
+    if graph:
+        h = ctypes.c_void_p()
+        _lib.check("gpufs_grep_batched_graph", L.gpufs_grep_batched_graph(*args, None, ctypes.byref(h)))
+        return h
+    _lib.check("gpufs_launch_grep_batched", L.gpufs_launch_grep_batched(*args, None, None))
+    return None
+
+
+from gpufs_amd.serve import Server  # noqa: E402,F401

@@ -1,0 +1,44 @@
+"""Serving form: a resident request loop with continuous admission.  Every
+request's counts equal the CPU path for its file, and are the same when the
+request runs alone and when it runs in a Poisson mix with others in flight."""
+import numpy as np
+import pytest
+
+from gpufs_amd import datagen as dg
+from tests.gpu_helpers import Server, make_ctx, say
+
+pytestmark = pytest.mark.gpu
+
+
+def test_serve_alone_equals_mix_equals_cpu(tmp_path, orc):
+    say("test_serve_alone_equals_mix_equals_cpu")
+    n = 200
+    voc = dg.vocab(4000, seed=71)
+    q = np.ascontiguousarray(voc[dg.query(4000, 1024, seed=72)])
+    sizes = dg.lognormal_sizes(n, seed=73, median=16384.0, sigma=1.5, lo=1, hi=1 << 20)
+    paths = [str(tmp_path / f"s{i:04d}.txt") for i in range(n)]
+    dg.text_files(paths, sizes, 74, voc, threads=8)
+    cpu, cst = orc.scan_files(paths, q, 65536, threads=8)
+    assert (cst == 0).all()
+    ids = dg.zipf_sample(n, 600, seed=75, s=1.1)
+    arr = np.cumsum(dg.poisson_gaps(20000.0, 600, seed=76))
+    with make_ctx(page_size=64 << 10, cache_bytes=64 << 20, ring_depth=256) as ctx:
+        srv = Server(ctx, paths, sizes, q, workgroups=8, slots=16)
+        try:
+            alone = {}
+            for f in sorted(set(ids[:40].tolist())):
+                c, st, lat, _ = srv.run([f], [0.0])
+                assert st[0] == 0 and (c[0] == cpu[f]).all(), f
+                alone[f] = c[0].copy()
+            c, st, lat, el = srv.run(ids, arr)
+            assert (st == 0).all(), st[:10]
+            for i, f in enumerate(ids):
+                assert (c[i] == cpu[f]).all(), (i, f)
+                if f in alone:
+                    assert (c[i] == alone[f]).all()
+            assert (lat > 0).all() and el > 0
+            s = ctx.stats()
+            assert s["page_hits"] > 0   # popular files are served from the resident cache
+        finally:
+            srv.stop()
+        assert ctx.check_invariants() == 0
