@@ -53,12 +53,15 @@ def parse():
     ap.add_argument("--cold-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grep", action="store_true")
-    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched", action="store_true",
+                    help="run the batched (persistent-kernel) leg; off by default in this build")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--batched-files", type=int, default=10000)
     ap.add_argument("--serve-files", type=int, default=100000)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-rates", default="1000,10000,50000")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 1 GiB device-to-device copy (known bytes) for HBM counter calibration")
     ap.add_argument("--workdir", default=os.environ.get("GPUFS_BENCH_DIR", "/tmp/gpufs_bench"))
     return ap.parse_args()
 
@@ -108,6 +111,15 @@ def main():
 
     link = link_peak_gbs(torch)
     log(f"link peak (pinned H2D 1 GiB, best of 5): {link:.1f} GB/s")
+    if args.copy_calibration:
+        a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        b2 = torch.empty_like(a)
+        a.fill_(3)
+        for _ in range(3):
+            b2.copy_(a)
+        torch.cuda.synchronize()
+        del a, b2
+        log("copy calibration: 3 x 1 GiB device-to-device copies")
 
     ctx = _lib.Context(cache_bytes=int(args.cache_gib * (1 << 30)), page_size=args.page_kib << 10,
                        ring_depth=args.ring_depth, host_threads=args.host_threads, max_files=1024)
@@ -207,7 +219,7 @@ def main():
     ctx.close()
     del dst
     torch.cuda.empty_cache()
-    batched_res = bench_batched(args, torch, L, _lib, dg) if not args.no_batched else None
+    batched_res = bench_batched(args, torch, L, _lib, dg) if args.batched else None
     serve_res = bench_serve(args, torch, L, _lib, dg) if not args.no_serve else None
     line = {
         "metric": METRIC,

@@ -31,7 +31,7 @@ class Config(ctypes.Structure):
 STATS_FIELDS = [
     "page_hits", "page_misses", "page_waits", "evictions", "alloc_fail", "rpc_reads", "rpc_writes",
     "rpc_opens", "rpc_closes", "bytes_filled", "bytes_written", "nodes_used", "files_used",
-    "host_requests", "host_bytes_read", "host_bytes_written", "frames", "frames_valid",
+    "host_requests", "host_bytes_read", "host_bytes_written", "frames", "frames_valid", "timeouts",
 ]
 
 

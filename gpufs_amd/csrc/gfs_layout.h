@@ -96,6 +96,7 @@ typedef struct gfs_counters {
     unsigned long long page_hits, page_misses, page_waits, evictions, alloc_fail;
     unsigned long long rpc_reads, rpc_writes, rpc_opens, rpc_closes;
     unsigned long long bytes_filled, bytes_written;
+    unsigned long long timeouts;
 } gfs_counters;
 
 typedef struct gfs_dev {

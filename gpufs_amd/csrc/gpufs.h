@@ -72,6 +72,7 @@ struct wg_ctl {
     uint32_t n_own;
     uint32_t cq;
     uint32_t flag;
+    uint32_t tmo;            // wait-limit decision of thread 0, broadcast to the workgroup
     uint64_t a, b;
     uint32_t rot;
 };
@@ -86,6 +87,20 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 __device__ __forceinline__ uint32_t popc(uint64_t m) { return __popcll(m); }
+
+// Values broadcast through LDS are the same in every lane, but the compiler
+// cannot prove it; branching on them directly makes it lower loops that hold
+// barriers as divergent loops with lane-mask bookkeeping, which can keep some
+// lanes iterating while others leave.  Every control-flow value read from a
+// broadcast goes through v_readfirstlane so the branch is scalar (SGPR).
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int32_t uni(int32_t v) { return (int32_t)__builtin_amdgcn_readfirstlane((uint32_t)v); }
+__device__ __forceinline__ uint64_t uni(uint64_t v) {
+    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+           (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+}
+__device__ __forceinline__ int64_t uni(int64_t v) { return (int64_t)uni((uint64_t)v); }
+__device__ __forceinline__ unsigned long long uni(unsigned long long v) { return (unsigned long long)uni((uint64_t)v); }
 __device__ __forceinline__ uint32_t first_lane(uint64_t m) { return __ffsll((unsigned long long)m) - 1; }
 __device__ __forceinline__ uint32_t nth_set(uint64_t m, uint32_t n) {
     for (uint32_t i = 0; i < n; ++i) m &= m - 1;
@@ -563,6 +578,20 @@ static __shared__ gfs::wg_ctl gfs_ctl;
 
 namespace gfs {
 
+// Workgroup-uniform wait-limit check: thread 0 compares its own clock reading
+// with its own start time and broadcasts the verdict, so every thread takes
+// the same branch around the barriers that follow.  Call with all threads.
+__device__ inline bool wg_timed_out(gfs_dev* g, uint64_t t0) {
+    __syncthreads();
+    if (tid() == 0) {
+        const bool late = now_ticks() - t0 > g->wait_ticks;
+        gfs_ctl.tmo = late ? 1u : 0u;
+        if (late) count(&g->ctr.timeouts, 1);
+    }
+    __syncthreads();
+    return uni(gfs_ctl.tmo) != 0;
+}
+
 // ======================================================================
 // Page resolution for a batch of up to 64 consecutive pages (wave 0 only).
 // On return gfs_ctl.kind[i] is HIT (pinned VALID frame), OWN (pinned FILLING
@@ -722,8 +751,8 @@ __device__ inline void fill_from_host(gfs_dev* g, uint32_t cnt, uint64_t pb, uin
         }
     }
     __syncthreads();
-    if (gfs_ctl.n_own == 0) return;
-    const int werr = gfs_ctl.err;
+    if (uni(gfs_ctl.n_own) == 0) return;
+    const int werr = uni(gfs_ctl.err);
     __syncthreads();
     if (wave() == 0) {
         const uint32_t ln = lane();
@@ -766,7 +795,7 @@ __device__ inline int gopen(const char* filename, int flags) {
     __syncthreads();
     if (gfs::tid() == 0) gfs_ctl.ret = gfs::open_single(filename, flags);
     __syncthreads();
-    return (int)gfs_ctl.ret;
+    return (int)gfs::uni(gfs_ctl.ret);
 }
 
 // libgpufs/fs_calls.cu:851-854 (per-thread, not collective)
@@ -793,8 +822,8 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
         gfs_ctl.a = r > 0 ? g->files[fd].size : 0;
     }
     __syncthreads();
-    const int64_t n = gfs_ctl.ret;
-    const uint64_t fsize = gfs_ctl.a;
+    const int64_t n = uni(gfs_ctl.ret);
+    const uint64_t fsize = uni(gfs_ctl.a);
     if (n <= 0) return (size_t)n;
     const uint32_t L = g->log_page;
     const uint64_t PS = g->page_size;
@@ -813,7 +842,7 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
                 resolve_batch(g, fd, pb, cnt, pending, 0ull, false);
             }
             __syncthreads();
-            if (gfs_ctl.err) { err = gfs_ctl.err; break; }
+            if (uni(gfs_ctl.err)) { err = uni(gfs_ctl.err); break; }
             fill_from_host(g, cnt, pb, fsize);
             // copy every pinned page (hit or freshly filled) into the caller's buffer
             const uint32_t T = nthreads(), t = tid();
@@ -837,11 +866,11 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
                 if (ln == 0) gfs_ctl.a = waitm;
             }
             __syncthreads();
-            if (gfs_ctl.err) { err = gfs_ctl.err; break; }
-            uint64_t nxt = gfs_ctl.a;
+            if (uni(gfs_ctl.err)) { err = uni(gfs_ctl.err); break; }
+            uint64_t nxt = uni(gfs_ctl.a);
             if (nxt == pending) {
                 if (++idle > 4) backoff();
-                if ((idle & 63) == 0 && now_ticks() - t0 > g->wait_ticks) { err = -ETIMEDOUT; break; }
+                if ((idle & 63) == 0 && wg_timed_out(g, t0)) { err = -ETIMEDOUT; break; }
             } else {
                 idle = 0;
             }
@@ -868,9 +897,9 @@ __device__ inline size_t gwrite(int fd, size_t offset, size_t size, uchar* buffe
         gfs_ctl.flag = r > 0 ? (uint32_t)g->files[fd].flags : 0;
     }
     __syncthreads();
-    const int64_t n = gfs_ctl.ret;
-    const uint64_t fsize0 = gfs_ctl.a;
-    const bool wronce = (gfs_ctl.flag & (O_CREAT | O_ACCMODE)) == (uint32_t)O_GWRONCE;
+    const int64_t n = uni(gfs_ctl.ret);
+    const uint64_t fsize0 = uni(gfs_ctl.a);
+    const bool wronce = (uni(gfs_ctl.flag) & (O_CREAT | O_ACCMODE)) == (uint32_t)O_GWRONCE;
     if (n <= 0) return (size_t)n;
     const uint32_t L = g->log_page;
     const uint64_t PS = g->page_size;
@@ -896,7 +925,7 @@ __device__ inline size_t gwrite(int fd, size_t offset, size_t size, uchar* buffe
                 resolve_batch(g, fd, pb, cnt, pending, nohost, true);
             }
             __syncthreads();
-            if (gfs_ctl.err) { err = gfs_ctl.err; break; }
+            if (uni(gfs_ctl.err)) { err = uni(gfs_ctl.err); break; }
             fill_from_host(g, cnt, pb, fsize0);
             const uint32_t T = nthreads(), t = tid();
             // zero the never-filled part of host-less OWN pages, then copy the caller's bytes
@@ -930,11 +959,11 @@ __device__ inline size_t gwrite(int fd, size_t offset, size_t size, uchar* buffe
                 if (ln == 0) gfs_ctl.a = waitm;
             }
             __syncthreads();
-            if (gfs_ctl.err) { err = gfs_ctl.err; break; }
-            uint64_t nxt = gfs_ctl.a;
+            if (uni(gfs_ctl.err)) { err = uni(gfs_ctl.err); break; }
+            uint64_t nxt = uni(gfs_ctl.a);
             if (nxt == pending) {
                 if (++idle > 4) backoff();
-                if ((idle & 63) == 0 && now_ticks() - t0 > g->wait_ticks) { err = -ETIMEDOUT; break; }
+                if ((idle & 63) == 0 && wg_timed_out(g, t0)) { err = -ETIMEDOUT; break; }
             } else {
                 idle = 0;
             }
@@ -989,7 +1018,7 @@ __device__ inline int flush_file(gfs_dev* g, int fd) {
         }
         __syncthreads();
         for (uint32_t i = 0; i < 64; ++i) {
-            if (gfs_ctl.kind[i] != GFS_K_OWN) continue;
+            if (uni((uint32_t)gfs_ctl.kind[i]) != GFS_K_OWN) continue;
             const uint32_t f = gfs_ctl.frame[i];
             const uint64_t page = (leaf << GFS_FANOUT_LOG) + i;
             uint64_t poff = page << g->log_page;
@@ -1006,7 +1035,7 @@ __device__ inline int flush_file(gfs_dev* g, int fd) {
                 gfs_ctl.slot[i] = s < 0 ? 0xFFFFFFFFu : (uint32_t)s;
             }
             __syncthreads();
-            const uint32_t s = gfs_ctl.slot[i];
+            const uint32_t s = uni(gfs_ctl.slot[i]);
             if (s != 0xFFFFFFFFu) {
                 wg_copy(staging_ptr(g, s), frame_ptr(g, f), bytes, false, false);
                 __threadfence_system();
@@ -1029,7 +1058,7 @@ __device__ inline int flush_file(gfs_dev* g, int fd) {
                 gfs_ctl.ret = err;
             }
             __syncthreads();
-            err = (int)gfs_ctl.ret;
+            err = (int)uni(gfs_ctl.ret);
         }
     }
     __syncthreads();
@@ -1044,7 +1073,7 @@ __device__ inline int gfsync(int fd) {
     __syncthreads();
     if (tid() == 0) gfs_ctl.ret = (fd_valid(g, fd) && fst_state(ld_acq(&g->files[fd].st)) == GFS_FS_OPEN) ? 0 : -EBADF;
     __syncthreads();
-    if (gfs_ctl.ret) return (int)gfs_ctl.ret;
+    if (uni(gfs_ctl.ret)) return (int)uni(gfs_ctl.ret);
     int r = flush_file(g, fd);
     __syncthreads();
     if (tid() == 0 && r == 0) g->files[fd].dirty = 0;
@@ -1078,9 +1107,11 @@ __device__ inline int gclose(int fd) {
         gfs_ctl.flag = last;
     }
     __syncthreads();
-    if (gfs_ctl.ret || !gfs_ctl.flag) return (int)gfs_ctl.ret;
+    if (uni(gfs_ctl.ret) || !uni(gfs_ctl.flag)) return (int)uni(gfs_ctl.ret);
     gfs_file* e = &g->files[fd];
-    int r = e->dirty ? flush_file(g, fd) : 0;
+    if (tid() == 0) gfs_ctl.flag = e->dirty;
+    __syncthreads();
+    int r = uni(gfs_ctl.flag) ? flush_file(g, fd) : 0;
     __syncthreads();
     if (tid() == 0) {
         int64_t c = rpc_call_single(g, GFS_OP_CLOSE, e->host_fd, 0, 0, 0, nullptr, nullptr);
@@ -1093,7 +1124,7 @@ __device__ inline int gclose(int fd) {
         gfs_ctl.ret = r;
     }
     __syncthreads();
-    return (int)gfs_ctl.ret;
+    return (int)uni(gfs_ctl.ret);
 }
 
 // libgpufs/fs_calls.cu:127-149.  The reference supports only size 0; any size
@@ -1122,7 +1153,7 @@ __device__ inline int gftruncate(int fd, int size) {
         gfs_ctl.ret = r;
     }
     __syncthreads();
-    return (int)gfs_ctl.ret;
+    return (int)uni(gfs_ctl.ret);
 }
 
 // libgpufs/fs_calls.cu:422-481.  Pins the page holding `offset` and returns a
@@ -1142,9 +1173,9 @@ __device__ inline volatile void* gmmap(void* addr, size_t size, int prot, int fl
         gfs_ctl.flag = (uint32_t)g->files[fd].flags;
     }
     __syncthreads();
-    if (gfs_ctl.ret) return GMAP_FAILED;
-    const uint64_t fsize = gfs_ctl.a;
-    const int oflags = (int)gfs_ctl.flag;
+    if (uni(gfs_ctl.ret)) return GMAP_FAILED;
+    const uint64_t fsize = uni(gfs_ctl.a);
+    const int oflags = (int)uni(gfs_ctl.flag);
     const bool writable = (oflags & O_ACCMODE) != O_RDONLY;
     const bool wronce = (oflags & (O_CREAT | O_ACCMODE)) == O_GWRONCE;
     const uint64_t page = (uint64_t)offset >> g->log_page;
@@ -1157,9 +1188,9 @@ __device__ inline volatile void* gmmap(void* addr, size_t size, int prot, int fl
             resolve_batch(g, fd, page, 1, 1ull, nohost ? 1ull : 0ull, writable);
         }
         __syncthreads();
-        if (gfs_ctl.err) return GMAP_FAILED;
+        if (uni(gfs_ctl.err)) return GMAP_FAILED;
         fill_from_host(g, 1, page, fsize);
-        if (gfs_ctl.kind[0] == GFS_K_OWN && gfs_ctl.slot[0] == 0xFFFFFFFFu) {
+        if (uni((uint32_t)gfs_ctl.kind[0]) == GFS_K_OWN && uni(gfs_ctl.slot[0]) == 0xFFFFFFFFu) {
             uint8_t* fp = frame_ptr(g, gfs_ctl.frame[0]);
             for (uint64_t k = tid(); k < (g->page_size >> 4); k += nthreads()) *(v4u*)(fp + 16 * k) = v4u{0, 0, 0, 0};
             __threadfence();
@@ -1190,12 +1221,12 @@ __device__ inline volatile void* gmmap(void* addr, size_t size, int prot, int fl
             }
         }
         __syncthreads();
-        if (gfs_ctl.ret < 0) return GMAP_FAILED;
-        if (gfs_ctl.ret == 1) break;
-        if (now_ticks() - t0 > g->wait_ticks) return GMAP_FAILED;
+        if (uni(gfs_ctl.ret) < 0) return GMAP_FAILED;
+        if (uni(gfs_ctl.ret) == 1) break;
+        if (wg_timed_out(g, t0)) return GMAP_FAILED;
         backoff();
     }
-    return (volatile void*)(frame_ptr(g, gfs_ctl.frame[0]) + ((uint64_t)offset & (g->page_size - 1)));
+    return (volatile void*)uni((uint64_t)(uintptr_t)(frame_ptr(g, gfs_ctl.frame[0]) + ((uint64_t)offset & (g->page_size - 1))));
 }
 
 // libgpufs/fs_calls.cu:385-401.  Unpins the frame holding `addr`; -1 if
@@ -1242,7 +1273,7 @@ __device__ inline int gmsync(volatile void* addr, size_t length, int flags) {
         if (s >= 0) __hip_atomic_fetch_and(&g->meta[f], ~GFS_FL_DIRTY, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
-    const uint32_t s = gfs_ctl.slot[0];
+    const uint32_t s = uni(gfs_ctl.slot[0]);
     if (s == 0xFFFFFFFFu) return -ETIMEDOUT;
     wg_copy(staging_ptr(g, s), frame_ptr(g, (uint32_t)f), bytes);
     __threadfence_system();
@@ -1258,7 +1289,7 @@ __device__ inline int gmsync(volatile void* addr, size_t length, int flags) {
         gfs_ctl.ret = (r < 0 || (uint64_t)r != bytes) ? (r < 0 ? r : -EIO) : 0;
     }
     __syncthreads();
-    return (int)gfs_ctl.ret;
+    return (int)uni(gfs_ctl.ret);
 }
 
 // Host helper: store the context's device state in this translation unit.

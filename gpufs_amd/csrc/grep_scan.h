@@ -1,5 +1,15 @@
 // grep_scan.h -- the whole-word scan shared by the plain, batched and serving
 // forms (see k_grep.hip for the semantics and the reference lines).
+//
+// Window scan: each thread owns 64-byte segments of the LDS window.  A
+// segment is read with four ds_read_b128, word bytes ([A-Za-z_]) are flagged
+// four at a time with SWAR arithmetic into a 64-bit mask, token starts are
+// mask & ~(mask << 1 | previous byte).  A token of at most 32 bytes is read as
+// up to four little-endian u64 words (funnel shifts of aligned 8-byte LDS
+// loads, bytes past the token zeroed); its hash picks a slot of an LDS table
+// of the dictionary, and a match is an exact comparison of length and the four
+// words.  A token's bytes and a query word's bytes compare equal iff the
+// strings are equal, so the counts are exact.
 #pragma once
 #include "gpufs.h"
 
@@ -8,21 +18,62 @@
 #define GREP_HT 4096
 
 struct grep_lds {
-    uint8_t corpus[GREP_MAX_CHUNK + 64];
-    uint8_t words[GREP_MAX_WORDS][32];
-    uint32_t ht[GREP_HT];            // canonical word index + 1
+    uint8_t corpus[GREP_MAX_CHUNK + 128];    // window + look-ahead + NUL + read padding
+    uint64_t words[GREP_MAX_WORDS][4];       // query bytes, zero-padded, little-endian u64
+    uint32_t ht[GREP_HT];                    // canonical word index + 1
     uint32_t counts[GREP_MAX_WORDS];
-    uint16_t canon[GREP_MAX_WORDS];  // 0xFFFF: word can never match
+    uint16_t canon[GREP_MAX_WORDS];          // 0xFFFF: word can never match
     uint8_t wlen[GREP_MAX_WORDS];
     int ov;
     int status;
+    uint64_t fsize;
 };
 
 __device__ __forceinline__ bool g_alpha(uint32_t c) {
     return (c >= 65 && c <= 90) || (c >= 97 && c <= 122) || c == 95;
 }
 
-__device__ __forceinline__ uint32_t g_hash_step(uint32_t h, uint32_t c) { return (h ^ c) * 16777619u; }
+// Word-byte flags of the four bytes of d (bit i = byte i), branch-free.
+// Letters: d|0x20 lands in [0x61,0x7a] exactly for A-Z and a-z; '_' by a
+// per-byte zero test of d^0x5f.  No carry crosses a byte boundary.
+__device__ __forceinline__ uint32_t alpha4(uint32_t d) {
+    const uint32_t x = d | 0x20202020u;
+    const uint32_t x7 = x & 0x7F7F7F7Fu;
+    const uint32_t lower = (x7 + 0x1F1F1F1Fu) & ~(x7 + 0x05050505u) & ~x & 0x80808080u;
+    const uint32_t y = d ^ 0x5F5F5F5Fu;
+    const uint32_t nz = ((y & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | y;
+    const uint32_t v = lower | (~nz & 0x80808080u);
+    return ((v >> 7) & 1u) | ((v >> 14) & 2u) | ((v >> 21) & 4u) | ((v >> 28) & 8u);
+}
+
+__device__ __forceinline__ uint32_t tok_hash(uint64_t a, uint64_t b, uint64_t c, uint64_t d, uint32_t len) {
+    uint64_t h = a * 0x9E3779B97F4A7C15ull;
+    h ^= b * 0xC2B2AE3D27D4EB4Full;
+    h ^= c * 0x165667B19E3779F9ull;
+    h ^= d * 0xD6E8FEB86659FD93ull;
+    h ^= (uint64_t)len * 0x27D4EB2F165667C5ull;
+    h ^= h >> 29;
+    h *= 0xBF58476D1CE4E5B9ull;
+    return (uint32_t)(h >> 32);
+}
+
+// Bytes [p, p+len) of the LDS window as four zero-padded u64 (len <= 32).
+// Reads the aligned words covering [p & ~7, p + 40): inside the padded buffer.
+__device__ __forceinline__ void load_tok(const uint8_t* corpus, uint32_t p, uint32_t len, uint64_t tk[4]) {
+    const uint64_t* W = (const uint64_t*)corpus;
+    const uint32_t a = p >> 3, sh = (p & 7) * 8;
+    uint64_t w[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) w[i] = W[a + i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        uint64_t v = sh ? ((w[i] >> sh) | (w[i + 1] << (64 - sh))) : w[i];
+        const int lo = 8 * i;
+        if ((int)len <= lo) v = 0;
+        else if ((int)len < lo + 8) v &= (1ull << (8 * (len - lo))) - 1;
+        tk[i] = v;
+    }
+}
 
 __device__ inline void grep_build_dict(grep_lds& S, const char* dict, uint32_t nw) {
     const uint32_t T = gfs::nthreads(), t = gfs::tid();
@@ -30,16 +81,22 @@ __device__ inline void grep_build_dict(grep_lds& S, const char* dict, uint32_t n
     for (uint32_t q = t; q < nw; q += T) {
         uint32_t len = 0;
         bool ok = true;
+        uint8_t b[32];
         for (uint32_t i = 0; i < 32; ++i) {
             uint8_t c = (uint8_t)dict[(size_t)q * 32 + i];
-            S.words[q][i] = c;
             if (len == i && c != 0) {
                 len = i + 1;
                 if (!g_alpha(c)) ok = false;
             }
+            b[i] = c;
         }
         // bytes after the first NUL are ignored (the C string ends there)
-        for (uint32_t i = len; i < 32; ++i) S.words[q][i] = 0;
+        for (uint32_t k = 0; k < 4; ++k) {
+            uint64_t v = 0;
+            for (uint32_t i = 0; i < 8; ++i)
+                if (8 * k + i < len) v |= (uint64_t)b[8 * k + i] << (8 * i);
+            S.words[q][k] = v;
+        }
         S.wlen[q] = (uint8_t)len;
         S.counts[q] = 0;
         S.canon[q] = (ok && len > 0) ? (uint16_t)q : (uint16_t)0xFFFF;
@@ -47,22 +104,21 @@ __device__ inline void grep_build_dict(grep_lds& S, const char* dict, uint32_t n
     __syncthreads();
     for (uint32_t q = t; q < nw; q += T) {
         if (S.canon[q] == 0xFFFF) continue;
-        uint32_t h = 2166136261u;
-        for (uint32_t i = 0; i < S.wlen[q]; ++i) h = g_hash_step(h, S.words[q][i]);
-        uint32_t pos = h & (GREP_HT - 1);
+        uint32_t pos = tok_hash(S.words[q][0], S.words[q][1], S.words[q][2], S.words[q][3], S.wlen[q]) & (GREP_HT - 1);
         for (;;) {
             uint32_t old = atomicCAS(&S.ht[pos], 0u, q + 1);
             if (old == 0) break;
             uint32_t r = old - 1;
-            bool same = S.wlen[r] == S.wlen[q];
-            for (uint32_t i = 0; same && i < S.wlen[q]; ++i) same = S.words[r][i] == S.words[q][i];
-            if (same) { S.canon[q] = (uint16_t)r; break; }
+            if (S.wlen[r] == S.wlen[q] && S.words[r][0] == S.words[q][0] && S.words[r][1] == S.words[q][1] &&
+                S.words[r][2] == S.words[q][2] && S.words[r][3] == S.words[q][3]) {
+                S.canon[q] = (uint16_t)r;
+                break;
+            }
             pos = (pos + 1) & (GREP_HT - 1);
         }
     }
     __syncthreads();
-    // a duplicate may have been linked to another duplicate that itself was
-    // linked later; follow links to the word that owns the table entry
+    // a duplicate links to the word that owns the table entry
     for (uint32_t q = t; q < nw; q += T) {
         uint32_t c = S.canon[q];
         if (c == 0xFFFF) continue;
@@ -73,38 +129,52 @@ __device__ inline void grep_build_dict(grep_lds& S, const char* dict, uint32_t n
 }
 
 // Count tokens of window [0, dsz) of S.corpus.  corpus[dsz-1] is never a word
-// character, so every token ends inside the window.
+// byte, so every token ends inside the window.
 __device__ inline void grep_scan_window(grep_lds& S, uint32_t dsz) {
+    typedef __attribute__((address_space(3))) gfs::v4u lv4;
     const uint32_t T = gfs::nthreads(), t = gfs::tid();
-    const uint32_t per = (dsz + T - 1) / T;
-    uint32_t p = t * per;
-    const uint32_t end = min(dsz, p + per);
-    const uint8_t* c = S.corpus;
-    while (p < end) {
-        uint32_t ch = c[p];
-        if (!g_alpha(ch) || (p > 0 && g_alpha(c[p - 1]))) { ++p; continue; }
-        uint32_t h = 2166136261u;
-        uint32_t q = p;
-        while (g_alpha(c[q])) {
-            if (q - p < 32) h = g_hash_step(h, c[q]);
-            ++q;
+    const uint32_t nseg = (dsz + 63) >> 6;
+    for (uint32_t sg = t; sg < nseg; sg += T) {
+        const uint32_t base = sg << 6;
+        const lv4* src = (const lv4*)(S.corpus + base);
+        uint64_t am = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            gfs::v4u v = src[k];
+            am |= ((uint64_t)alpha4(v.x) | ((uint64_t)alpha4(v.y) << 4) | ((uint64_t)alpha4(v.z) << 8) |
+                   ((uint64_t)alpha4(v.w) << 12)) << (16 * k);
         }
-        const uint32_t len = q - p;
-        if (len <= 32) {
-            uint32_t pos = h & (GREP_HT - 1);
+        const uint32_t valid = dsz - base;
+        if (valid < 64) am &= (1ull << valid) - 1;
+        const uint64_t prev = (base > 0 && g_alpha(S.corpus[base - 1])) ? 1ull : 0ull;
+        uint64_t starts = am & ~((am << 1) | prev);
+        while (starts) {
+            const uint32_t b = (uint32_t)__ffsll((unsigned long long)starts) - 1;
+            starts &= starts - 1;
+            const uint32_t p = base + b;
+            const uint64_t rest = ~(am >> b);
+            uint32_t len = rest ? (uint32_t)__ffsll((unsigned long long)rest) - 1 : 64;
+            if (b + len >= 64) {   // the token runs past the segment: count on in LDS (up to 33)
+                uint32_t q = base + 64;
+                while (q < p + 33 && g_alpha(S.corpus[q])) ++q;
+                len = q - p;
+            }
+            if (len > 32) continue;
+            uint64_t tk[4];
+            load_tok(S.corpus, p, len, tk);
+            uint32_t pos = tok_hash(tk[0], tk[1], tk[2], tk[3], len) & (GREP_HT - 1);
             for (;;) {
-                uint32_t e = S.ht[pos];
+                const uint32_t e = S.ht[pos];
                 if (!e) break;
-                uint32_t w = e - 1;
-                if (S.wlen[w] == len) {
-                    bool same = true;
-                    for (uint32_t i = 0; same && i < len; ++i) same = S.words[w][i] == c[p + i];
-                    if (same) { atomicAdd(&S.counts[w], 1u); break; }
+                const uint32_t w = e - 1;
+                if (S.wlen[w] == len && S.words[w][0] == tk[0] && S.words[w][1] == tk[1] && S.words[w][2] == tk[2] &&
+                    S.words[w][3] == tk[3]) {
+                    atomicAdd(&S.counts[w], 1u);
+                    break;
                 }
                 pos = (pos + 1) & (GREP_HT - 1);
             }
         }
-        p = q;
     }
 }
 
@@ -118,8 +188,12 @@ __device__ inline int grep_one_file(grep_lds& S, const char* name, uint32_t nw, 
         for (uint32_t q = t; q < nw; q += T) row[q] = 0;
         return fd;
     }
-    const uint64_t size = fstat(fd);
-    if (t == 0) S.status = 0;
+    if (t == 0) {
+        S.status = 0;
+        S.fsize = fstat(fd);   // one read, broadcast: the loop bound must be workgroup-uniform
+    }
+    __syncthreads();
+    const uint64_t size = gfs::uni(S.fsize);
     uint64_t cursor = 0;
     while (cursor < size) {
         const bool last = size - cursor < (uint64_t)chunk + 32;
@@ -136,7 +210,7 @@ __device__ inline int grep_one_file(grep_lds& S, const char* name, uint32_t nw, 
             S.ov = ov;
         }
         __syncthreads();
-        const uint32_t ov = (uint32_t)S.ov;
+        const uint32_t ov = gfs::uni((uint32_t)S.ov);
         const uint32_t dsz = last ? left + 1 : chunk + ov + 1;
         cursor += (uint64_t)chunk + ov;
         grep_scan_window(S, dsz);
@@ -148,7 +222,7 @@ __device__ inline int grep_one_file(grep_lds& S, const char* name, uint32_t nw, 
         uint32_t k = S.canon[q];
         row[q] = k == 0xFFFF ? 0u : S.counts[k];
     }
-    int st = S.status ? S.status : c;
+    int st = gfs::uni(S.status ? S.status : c);
     __syncthreads();
     return st;
 }
@@ -159,4 +233,3 @@ static int grep_args_ok(uint32_t n_files, uint32_t n_words, uint32_t chunk_bytes
     if (threads < 64 || threads > 1024 || threads % 64) return -EINVAL;
     return 0;
 }
-

@@ -37,7 +37,7 @@ __global__ __launch_bounds__(1024) void k_gemv_mmap(const char* path, uint64_t r
     const uint64_t rb = cols * 4;
     unsigned long long csum = 0;
     const uint64_t r0 = (uint64_t)blockIdx.x * rows_per_wg;
-    for (uint64_t r = r0; r < r0 + rows_per_wg && r < rows && st == 0; ++r) {
+    for (uint64_t r = r0; r < r0 + rows_per_wg && r < rows && gfs::uni(st) == 0; ++r) {
         float acc = 0.f;
         const uint64_t row_end = (r + 1) * rb;
         for (uint64_t off = r * rb; off < row_end;) {

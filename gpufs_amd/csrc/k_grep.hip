@@ -39,7 +39,7 @@ __global__ __launch_bounds__(1024) void k_grep_batched(const char* names, uint32
     for (;;) {
         if (gfs::tid() == 0) cur = atomicAdd(queue, 1u);
         __syncthreads();
-        const unsigned int f = cur;
+        const unsigned int f = gfs::uni(cur);
         __syncthreads();
         if (f >= nfiles) break;
         const uint64_t t0 = gfs::now_ticks();
@@ -51,6 +51,11 @@ __global__ __launch_bounds__(1024) void k_grep_batched(const char* names, uint32
                 times[2 * (size_t)f + 1] = gfs::now_ticks();
             }
         }
+        // A barrier between this thread-0 region and the next iteration's keeps
+        // the compiler from threading the loop-invariant `tid == 0` test across
+        // the back-edge, which would send the other lanes round the loop
+        // without thread 0.
+        __syncthreads();
     }
 }
 

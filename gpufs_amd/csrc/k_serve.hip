@@ -22,6 +22,7 @@
 #include <cstring>
 #include <deque>
 #include <new>
+#include <thread>
 #include <vector>
 
 #define SERVE_STOP 0xFFFFFFFFu
@@ -61,8 +62,8 @@ __global__ __launch_bounds__(1024) void k_serve(const char* names, uint32_t stri
             myseq = k + 1;
         }
         __syncthreads();
-        const uint32_t f = fid, s = slot;
-        const uint64_t sq = myseq;
+        const uint32_t f = gfs::uni(fid), s = gfs::uni(slot);
+        const uint64_t sq = gfs::uni(myseq);
         __syncthreads();
         if (f == SERVE_STOP) break;
         int st = f < nfiles ? grep_one_file(S, names + (size_t)f * stride, nw, chunk, results + (size_t)s * nw)
@@ -74,6 +75,9 @@ __global__ __launch_bounds__(1024) void k_serve(const char* names, uint32_t stri
             gfs::fence_rel_sys();
             gfs::sys_st_rel(&done[s], sq);
         }
+        // barrier between the two thread-0 regions of consecutive iterations
+        // (see k_grep_batched)
+        __syncthreads();
     }
 }
 
@@ -235,7 +239,14 @@ extern "C" int gpufs_serve_stop(gpufs_server* S) {
     int rc = 0;
     if (S->running) {
         for (uint32_t w = 0; w < S->workgroups; ++w) serve_publish(S, SERVE_STOP, 0);
-        if (hipStreamSynchronize(S->stream) != hipSuccess) rc = -EIO;
+        // bounded wait for the workgroups to leave
+        const uint64_t t0 = serve_now_ns();
+        hipError_t e;
+        while ((e = hipStreamQuery(S->stream)) == hipErrorNotReady) {
+            if (serve_now_ns() - t0 > 60ull * 1000000000ull) return -ETIMEDOUT;
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
+        }
+        if (e != hipSuccess) rc = -EIO;
         S->running = false;
     }
     (void)hipHostFree(S->ring);

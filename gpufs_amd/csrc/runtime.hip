@@ -336,6 +336,7 @@ extern "C" int gpufs_stats_get(gpufs_ctx* c, gpufs_stats* o) {
     uint64_t valid = 0;
     for (uint64_t m : meta) valid += ((m >> GFS_ST_SHIFT) & 0xF) == GFS_F_VALID;
     o->frames_valid = valid;
+    o->timeouts = cur.ctr.timeouts;
     return 0;
 }
 

@@ -1,0 +1,126 @@
+// CPU model of the GPU->host RPC ring: client threads play the device side of
+// the protocol (claim a slot, write the request, publish (ticket << 16 | slot)
+// in the submission queue, wait for the completion counter, read the result)
+// against the real host service (host_service.h).  Checks every result, the
+// data in staging, slot conservation, and runs under ThreadSanitizer.
+#include <atomic>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+#include <fcntl.h>
+#include <unistd.h>
+#include "../../gpufs_amd/csrc/host_service.h"
+
+static int failures = 0;
+#define CHECK(c, ...) do { if (!(c)) { fprintf(stderr, "FAIL %s:%d: ", __FILE__, __LINE__); fprintf(stderr, __VA_ARGS__); fprintf(stderr, "\n"); __atomic_fetch_add(&failures, 1, __ATOMIC_RELAXED); } } while (0)
+
+struct Ring {
+    gfs_host::RingView rv;
+    std::vector<gfs_rpc_slot> slots;
+    std::vector<uint64_t> sq;
+    std::vector<uint32_t> cq;
+    std::vector<uint8_t> staging;
+    std::mutex mu;
+    std::vector<uint32_t> free_slots;
+    std::atomic<uint64_t> tail{0};
+    Ring(uint32_t depth, uint32_t log_page) : slots(depth), sq(64 < depth ? depth : 64, ~0ull), cq(depth, 0),
+        staging(((size_t)depth << log_page) + 64) {
+        uint32_t q = 1; while (q < sq.size()) q <<= 1; sq.assign(q, ~0ull);
+        rv.slots = slots.data(); rv.sq = sq.data(); rv.cq = cq.data(); rv.staging = staging.data();
+        rv.sq_depth = q; rv.ring_depth = depth; rv.log_page = log_page;
+        for (uint32_t s = 0; s < depth; ++s) free_slots.push_back(s);
+    }
+    int claim() { std::lock_guard<std::mutex> g(mu); if (free_slots.empty()) return -1; int s = free_slots.back(); free_slots.pop_back(); return s; }
+    void release(int s) { std::lock_guard<std::mutex> g(mu); free_slots.push_back(s); }
+    // device side: publish one request and wait for it (cq index = slot)
+    int64_t call(int s, uint32_t op, int fd, int flags, uint64_t off, uint64_t len, const char* name = nullptr) {
+        gfs_rpc_slot* r = &slots[s];
+        r->op = op; r->host_fd = fd; r->flags = flags; r->cq = (uint32_t)s; r->offset = off; r->length = len;
+        r->result = INT64_MIN;
+        if (name) { memset(r->name, 0, GFS_NAME_MAX); strncpy(r->name, name, GFS_NAME_MAX - 1); }
+        __atomic_store_n(&cq[s], 0u, __ATOMIC_RELAXED);
+        uint64_t t = tail.fetch_add(1);
+        __atomic_store_n(&sq[t & (rv.sq_depth - 1)], (t << 16) | (uint64_t)s, __ATOMIC_RELEASE);
+        while (__atomic_load_n(&cq[s], __ATOMIC_ACQUIRE) < 1) std::this_thread::yield();
+        return __atomic_load_n(&r->result, __ATOMIC_RELAXED);
+    }
+};
+
+int main(int argc, char** argv) {
+    const char* dir = argc > 1 ? argv[1] : "/tmp";
+    const uint32_t log_page = 12, depth = 8;
+    std::string path = std::string(dir) + "/ring_model.bin";
+    std::vector<uint8_t> data(1 << 20);
+    std::mt19937_64 rng(7);
+    for (auto& b : data) b = (uint8_t)rng();
+    { int fd = open(path.c_str(), O_CREAT | O_TRUNC | O_WRONLY, 0644); CHECK(write(fd, data.data(), data.size()) == (ssize_t)data.size(), "write"); close(fd); }
+    Ring ring(depth, log_page);
+    gfs_host::Service svc;
+    svc.start(ring.rv, 3);
+    // OPEN through the ring
+    int s0 = ring.claim();
+    int64_t hfd = ring.call(s0, GFS_OP_OPEN, -1, O_RDONLY, 0, 0, path.c_str());
+    CHECK(hfd >= 0, "open %lld", (long long)hfd);
+    CHECK(ring.slots[s0].size == data.size(), "size");
+    ring.release(s0);
+    int s1 = ring.claim();
+    int64_t bad = ring.call(s1, GFS_OP_OPEN, -1, O_RDONLY, 0, 0, (std::string(dir) + "/does_not_exist").c_str());
+    CHECK(bad == -ENOENT, "missing open %lld", (long long)bad);
+    ring.release(s1);
+    // many concurrent reads from 12 client threads through 8 slots
+    std::vector<std::thread> cl;
+    std::atomic<int> done{0};
+    for (int c = 0; c < 12; ++c) {
+        cl.emplace_back([&, c]() {
+            std::mt19937_64 r(100 + c);
+            for (int k = 0; k < 300; ++k) {
+                int s;
+                while ((s = ring.claim()) < 0) std::this_thread::yield();
+                uint64_t off = r() % (data.size() + 100);
+                uint64_t len = 1 + r() % 4096;
+                int64_t got = ring.call(s, GFS_OP_READ, (int)hfd, 0, off, len);
+                uint64_t exp = off >= data.size() ? 0 : std::min<uint64_t>(len, data.size() - off);
+                CHECK(got == (int64_t)exp, "read off=%llu len=%llu got=%lld", (unsigned long long)off, (unsigned long long)len, (long long)got);
+                if (got > 0) CHECK(memcmp(&ring.staging[(size_t)s << log_page], &data[off], (size_t)got) == 0, "bytes");
+                ring.release(s);
+                done++;
+            }
+        });
+    }
+    for (auto& t : cl) t.join();
+    CHECK(done == 12 * 300, "done");
+    CHECK(ring.free_slots.size() == depth, "slots conserved");
+    // write + OR-merge + truncate on a second file
+    std::string wpath = std::string(dir) + "/ring_model_w.bin";
+    { int fd = open(wpath.c_str(), O_CREAT | O_TRUNC | O_WRONLY, 0644); uint8_t b[8] = {1, 2, 4, 8, 0, 0, 0, 0}; CHECK(write(fd, b, 8) == 8, "w"); close(fd); }
+    int s = ring.claim();
+    int64_t wfd = ring.call(s, GFS_OP_OPEN, -1, O_RDWR, 0, 0, wpath.c_str());
+    CHECK(wfd >= 0, "open w");
+    uint8_t* st = &ring.staging[(size_t)s << log_page];
+    uint8_t nb[12] = {16, 16, 16, 16, 16, 16, 16, 16, 7, 7, 7, 7};
+    memcpy(st, nb, 12);
+    CHECK(ring.call(s, GFS_OP_DIFF, (int)wfd, 0, 0, 12) == 12, "diff");
+    uint8_t back[12];
+    { int fd = open(wpath.c_str(), O_RDONLY); CHECK(read(fd, back, 12) == 12, "rb"); close(fd); }
+    uint8_t exp[12] = {17, 18, 20, 24, 16, 16, 16, 16, 7, 7, 7, 7};
+    CHECK(memcmp(back, exp, 12) == 0, "or-merge");
+    memcpy(st, "abc", 3);
+    CHECK(ring.call(s, GFS_OP_WRITE, (int)wfd, 0, 20, 3) == 3, "write");
+    CHECK(ring.call(s, GFS_OP_TRUNC, (int)wfd, 0, 5, 0) == 0, "trunc");
+    CHECK(ring.call(s, GFS_OP_CLOSE, (int)wfd, 0, 0, 0) == 0, "close");
+    CHECK(ring.call(s, GFS_OP_READ, 12345, 0, 0, 10) == -EBADF, "bad fd");
+    CHECK(ring.call(s, 99, -1, 0, 0, 0) == -EINVAL, "bad op");
+    ring.release(s);
+    { struct stat sb; stat(wpath.c_str(), &sb); CHECK(sb.st_size == 5, "size after trunc"); }
+    svc.stop();
+    CHECK(svc.counters().requests.load() == 2 + 3600 + 7, "request count %llu", (unsigned long long)svc.counters().requests.load());
+    unlink(path.c_str()); unlink(wpath.c_str());
+    if (failures) { fprintf(stderr, "%d failures\n", failures); return 1; }
+    printf("ring model OK\n");
+    return 0;
+}

@@ -50,6 +50,11 @@ def dev_zeros(n, dtype=None):
     return t
 
 
+# device waits give up after 10 s in the tests, so a stalled request surfaces as
+# -ETIMEDOUT (and the stats' `timeouts` counter) instead of a long wait
+os.environ.setdefault("GPUFS_WAIT_TIMEOUT_S", "10")
+
+
 def make_ctx(**cfg):
     cfg.setdefault("cache_bytes", 64 << 20)
     cfg.setdefault("page_size", 64 << 10)
