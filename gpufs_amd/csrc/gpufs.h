@@ -95,9 +95,12 @@ __device__ __forceinline__ uint32_t popc(uint64_t m) { return __popcll(m); }
 // broadcast goes through v_readfirstlane so the branch is scalar (SGPR).
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ int32_t uni(int32_t v) { return (int32_t)__builtin_amdgcn_readfirstlane((uint32_t)v); }
+// __builtin_amdgcn_readfirstlane returns int: widen through uint32_t, never
+// straight from int (that would sign-extend bit 31 into the high word).
 __device__ __forceinline__ uint64_t uni(uint64_t v) {
-    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
-           (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+    return ((uint64_t)hi << 32) | (uint64_t)lo;
 }
 __device__ __forceinline__ int64_t uni(int64_t v) { return (int64_t)uni((uint64_t)v); }
 __device__ __forceinline__ unsigned long long uni(unsigned long long v) { return (unsigned long long)uni((uint64_t)v); }
