@@ -52,6 +52,9 @@
 #define GFS_FS_CLOSED  4u
 #define GFS_FS_DEAD    5u   // entry lost a name-index race; never used
 
+// ---- tuning switches
+#define GFS_TUNE_PAGE_SYNC 1u   // gread: workgroup barrier after each page copied to the caller
+
 // ---- RPC opcodes
 #define GFS_OP_READ  1u
 #define GFS_OP_WRITE 2u
@@ -132,6 +135,8 @@ typedef struct gfs_dev {
     unsigned long long files_next;
     unsigned long long sq_tail;
     uint64_t wait_ticks;             // bounded waits: wall-clock ticks before -ETIMEDOUT
+    uint32_t tune_flags;             // GFS_TUNE_* (GPUFS_TUNE environment variable at init)
+    uint32_t pad1;
     gfs_counters ctr;
 } gfs_dev;
 

@@ -204,8 +204,10 @@ __device__ __forceinline__ void store16(uint8_t* p, v4u v, bool dst_lds, bool nt
     *(gv4u*)p = v;
 }
 
-__device__ inline void copy_seg(uint8_t* dst, const uint8_t* src, uint64_t len, uint32_t j,
-                                uint32_t T, bool nt_load, bool nt_store) {
+// The non-temporal choices are template constants: as runtime flags the
+// compiler merges the two load forms and the nontemporal hint is lost.
+template <bool nt_load, bool nt_store>
+__device__ inline void copy_seg(uint8_t* dst, const uint8_t* src, uint64_t len, uint32_t j, uint32_t T) {
     if (len == 0) return;
     const bool dl = is_lds(dst);
     const bool sl = is_lds(src);
@@ -221,6 +223,30 @@ __device__ inline void copy_seg(uint8_t* dst, const uint8_t* src, uint64_t len, 
     for (uint64_t k = j; k < tail; k += T) D[units * 16 + k] = S[units * 16 + k];
     if (units == 0) return;
     if (((s ^ d) & 15) == 0) {
+        if (!sl && !dl) {
+            // global -> global: eight 16-byte loads in flight per lane
+            const gv4u* S4 = (const gv4u*)S;
+            gv4u* D4 = (gv4u*)D;
+            const uint64_t st = T;
+            uint64_t u = j;
+            for (; u + 7 * st < units; u += 8 * st) {
+                v4u r[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    r[k] = nt_load ? __builtin_nontemporal_load(S4 + u + k * st) : S4[u + k * st];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (nt_store) __builtin_nontemporal_store(r[k], D4 + u + k * st);
+                    else D4[u + k * st] = r[k];
+                }
+            }
+            for (; u < units; u += st) {
+                v4u r0 = nt_load ? __builtin_nontemporal_load(S4 + u) : S4[u];
+                if (nt_store) __builtin_nontemporal_store(r0, D4 + u);
+                else D4[u] = r0;
+            }
+            return;
+        }
         uint64_t u = j;
         const uint64_t st = T;
         for (; u + 3 * st < units; u += 4 * st) {
@@ -256,9 +282,8 @@ __device__ inline void copy_seg(uint8_t* dst, const uint8_t* src, uint64_t len, 
 }
 
 // Whole-workgroup copy of one buffer.
-__device__ inline void wg_copy(uint8_t* dst, const uint8_t* src, uint64_t len, bool nt_load = false,
-                               bool nt_store = false) {
-    copy_seg(dst, src, len, tid(), nthreads(), nt_load, nt_store);
+__device__ inline void wg_copy(uint8_t* dst, const uint8_t* src, uint64_t len) {
+    copy_seg<false, false>(dst, src, len, tid(), nthreads());
 }
 
 // ======================================================================
@@ -779,7 +804,7 @@ __device__ inline void fill_from_host(gfs_dev* g, uint32_t cnt, uint64_t pb, uin
         if (gfs_ctl.kind[i] != GFS_K_OWN || gfs_ctl.slot[i] == 0xFFFFFFFFu) continue;
         uint32_t f = gfs_ctl.frame[i];
         uint64_t bytes = g->fbytes[f];
-        copy_seg(frame_ptr(g, f), staging_ptr(g, gfs_ctl.slot[i]), bytes, (t + T - rot) % T, T, true, false);
+        copy_seg<true, false>(frame_ptr(g, f), staging_ptr(g, gfs_ctl.slot[i]), bytes, (t + T - rot) % T, T);
         rot = (rot + (uint32_t)(bytes >> 4)) % T;
         if (t == 0) count(&g->ctr.bytes_filled, bytes);
     }
@@ -830,6 +855,7 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
     if (n <= 0) return (size_t)n;
     const uint32_t L = g->log_page;
     const uint64_t PS = g->page_size;
+    const bool page_sync = (uni(g->tune_flags) & GFS_TUNE_PAGE_SYNC) != 0;
     const uint64_t p_first = offset >> L, p_last = (offset + n - 1) >> L;
     const uint32_t bmax = 64;
     int err = 0;
@@ -851,14 +877,16 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
             const uint32_t T = nthreads(), t = tid();
             uint32_t rot = 0;
             for (uint32_t i = 0; i < cnt; ++i) {
-                uint32_t k = gfs_ctl.kind[i];
+                uint32_t k = uni((uint32_t)gfs_ctl.kind[i]);
                 if (k != GFS_K_HIT && k != GFS_K_OWN) continue;
                 uint64_t poff = (pb + i) << L;
                 uint64_t lo = poff > offset ? poff : offset;
                 uint64_t hi = poff + PS < offset + n ? poff + PS : offset + n;
-                copy_seg(buffer + (lo - offset), frame_ptr(g, gfs_ctl.frame[i]) + (lo - poff), hi - lo,
-                         (t + T - rot) % T, T, true, true);
+                copy_seg<true, true>(buffer + (lo - offset), frame_ptr(g, gfs_ctl.frame[i]) + (lo - poff), hi - lo,
+                                     (t + T - rot) % T, T);
                 rot = (rot + (uint32_t)((hi - lo) >> 4)) % T;
+                // keep the workgroup's waves on the same page (streams the pool in order)
+                if (page_sync) __syncthreads();
             }
             __syncthreads();
             if (wave() == 0) {
@@ -944,7 +972,7 @@ __device__ inline size_t gwrite(int fd, size_t offset, size_t size, uchar* buffe
                 uint64_t poff = (pb + i) << L;
                 uint64_t lo = poff > offset ? poff : offset;
                 uint64_t hi = poff + PS < offset + n ? poff + PS : offset + n;
-                copy_seg(frame_ptr(g, gfs_ctl.frame[i]) + (lo - poff), buffer + (lo - offset), hi - lo, t, T, false, false);
+                copy_seg<false, false>(frame_ptr(g, gfs_ctl.frame[i]) + (lo - poff), buffer + (lo - offset), hi - lo, t, T);
                 if (t == 0) atomicMax(&g->fbytes[gfs_ctl.frame[i]], (uint32_t)(hi - poff));
             }
             __threadfence();
@@ -1040,7 +1068,7 @@ __device__ inline int flush_file(gfs_dev* g, int fd) {
             __syncthreads();
             const uint32_t s = uni(gfs_ctl.slot[i]);
             if (s != 0xFFFFFFFFu) {
-                wg_copy(staging_ptr(g, s), frame_ptr(g, f), bytes, false, false);
+                wg_copy(staging_ptr(g, s), frame_ptr(g, f), bytes);
                 __threadfence_system();
             }
             __syncthreads();

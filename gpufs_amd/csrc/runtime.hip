@@ -207,6 +207,8 @@ extern "C" int gpufs_init(const gpufs_config* cfg_in, gpufs_ctx** out) {
     double tsec = tmo ? atof(tmo) : 30.0;
     if (tsec <= 0) tsec = 30.0;
     h.wait_ticks = (uint64_t)(tsec * 1000.0 * wall_khz);
+    const char* tune = getenv("GPUFS_TUNE");
+    h.tune_flags = tune ? (uint32_t)strtoul(tune, nullptr, 0) : 0u;
     memset(&h.ctr, 0, sizeof(h.ctr));
     TRY_ALLOC(hipMalloc(&c->dev, sizeof(gfs_dev)));
     TRY_ALLOC(hipMemcpy(c->dev, &h, sizeof(gfs_dev), hipMemcpyHostToDevice));
