@@ -38,6 +38,7 @@
 #include <errno.h>
 #include <stdint.h>
 #include "gfs_layout.h"
+#include "fstate.h"
 #include "../../include/gpufs_amd.h"
 
 typedef unsigned char uchar;
@@ -153,6 +154,10 @@ __device__ __forceinline__ void fence_acq_sys() { __builtin_amdgcn_fence(__ATOMI
 __device__ __forceinline__ void fence_rel_sys() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, ""); }
 __device__ __forceinline__ void backoff() { __builtin_amdgcn_s_sleep(16); }
 __device__ __forceinline__ uint64_t now_ticks() { return wall_clock64(); }
+// Phase hook for timing studies: expands to nothing unless a build defines it.
+#ifndef GFS_TRACE
+#define GFS_TRACE(id) ((void)0)
+#endif
 
 __device__ __forceinline__ uint32_t meta_state(uint64_t m) { return (uint32_t)((m >> GFS_ST_SHIFT) & 0xF); }
 __device__ __forceinline__ uint32_t meta_gen(uint64_t m) { return (uint32_t)(m >> GFS_GEN_SHIFT); }
@@ -161,12 +166,6 @@ __device__ __forceinline__ uint64_t leaf_pack(uint32_t f, uint32_t gen) { return
 __device__ __forceinline__ uint32_t leaf_frame(uint64_t v) { return (uint32_t)(v & 0xFFFFFFFFu) - 1u; }
 __device__ __forceinline__ uint32_t leaf_gen(uint64_t v) { return (uint32_t)(v >> 32); }
 
-__device__ __forceinline__ uint32_t fst_state(uint64_t s) { return (uint32_t)(s & 0xFF); }
-__device__ __forceinline__ uint32_t fst_opens(uint64_t s) { return (uint32_t)((s >> 8) & 0xFFFFFF); }
-__device__ __forceinline__ uint32_t fst_version(uint64_t s) { return (uint32_t)(s >> 32); }
-__device__ __forceinline__ uint64_t fst_make(uint32_t state, uint32_t opens, uint32_t version) {
-    return ((uint64_t)version << 32) | ((uint64_t)(opens & 0xFFFFFF) << 8) | state;
-}
 
 __device__ __forceinline__ void count(unsigned long long* c, unsigned long long v) {
     fadd_rlx(c, v);
@@ -489,35 +488,104 @@ __device__ inline int64_t rpc_call_single(gfs_dev* g, uint32_t op, int32_t host_
 __device__ __forceinline__ bool fd_valid(gfs_dev* g, int fd) {
     return fd >= 0 && (unsigned long long)fd < ld_rlx(&g->files_next) && (uint32_t)fd < g->max_files;
 }
-__device__ inline uint32_t name_hash(const char* s, int* len) {
-    uint32_t h = 2166136261u;
-    int i = 0;
-    for (; i < GFS_NAME_MAX && s[i]; ++i) { h ^= (uint8_t)s[i]; h *= 16777619u; }
-    *len = i;
-    return h;
+// A path as GFS_NAME_MAX/8 little-endian words, zero from the terminator on.
+struct gname {
+    uint64_t w[GFS_NAME_MAX / 8];
+    int len;          // strlen, or GFS_NAME_MAX when no terminator in the first GFS_NAME_MAX bytes
+    uint32_t hash;
+};
+__device__ __forceinline__ uint64_t zero_bytes(uint64_t v) {
+    // high bit of each zero byte; the lowest flagged byte is always a zero byte
+    return (v - 0x0101010101010101ull) & ~v & 0x8080808080808080ull;
 }
-__device__ inline bool name_eq(const char* a, const char* b) {
-    for (int i = 0; i < GFS_NAME_MAX; ++i) {
-        if (a[i] != b[i]) return false;
-        if (!a[i]) return true;
+// Reads the NUL-terminated path `s` with aligned 8-byte loads issued together
+// instead of one dependent load per byte.  First round: the words inside the
+// 4 KiB page of `s`.  Second round, only when the first found no terminator at
+// or after `s`: the remaining words, in the next page, which the string then
+// reaches.  So no load touches a page that holds no byte of the string.
+__device__ inline void name_load(const char* s, gname& n) {
+    constexpr int NW = GFS_NAME_MAX / 8 + 1;   // covers GFS_NAME_MAX bytes at any alignment
+    const uintptr_t a = (uintptr_t)s;
+    const uint64_t* W = (const uint64_t*)(a & ~(uintptr_t)7);
+    const uint32_t sh = (uint32_t)(a & 7) * 8;
+    const uintptr_t page_end = (a | 4095) + 1;
+    uint64_t raw[NW];
+    bool term = false;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) raw[k] = ((uintptr_t)(W + k) < page_end) ? W[k] : 0ull;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        if ((uintptr_t)(W + k) < page_end) {
+            uint64_t v = raw[k];
+            if (k == 0 && sh) v |= (1ull << sh) - 1;   // bytes before s are not part of the string
+            term |= zero_bytes(v) != 0;
+        }
     }
-    return true;
+    if (!term) {
+#pragma unroll
+        for (int k = 0; k < NW; ++k)
+            if ((uintptr_t)(W + k) >= page_end) raw[k] = W[k];
+    }
+    int len = GFS_NAME_MAX;
+#pragma unroll
+    for (int i = GFS_NAME_MAX / 8 - 1; i >= 0; --i) {
+        const uint64_t v = sh ? ((raw[i] >> sh) | (raw[i + 1] << (64 - sh))) : raw[i];
+        n.w[i] = v;
+        const uint64_t z = zero_bytes(v);
+        if (z) len = 8 * i + (int)((__ffsll((unsigned long long)z) - 1) >> 3);
+    }
+    uint64_t h = 0xCBF29CE484222325ull ^ (uint64_t)len;
+#pragma unroll
+    for (int i = 0; i < GFS_NAME_MAX / 8; ++i) {
+        const int lo = 8 * i;
+        if (len <= lo) n.w[i] = 0;
+        else if (len < lo + 8) n.w[i] &= (1ull << (8 * (len - lo))) - 1;
+        h = (h ^ n.w[i]) * 0x100000001B3ull;
+    }
+    n.len = len;
+    n.hash = (uint32_t)(h ^ (h >> 32));
 }
-// Host open of entry e (new or reopened).  Fills host_fd/size/stat fields.
+// Entry names are stored zero-padded, so equality is sixteen word compares.
+__device__ inline bool name_eq(const gfs_file* e, const gname& n) {
+    const uint64_t* w = (const uint64_t*)e->name;
+    uint64_t d = 0;
+#pragma unroll
+    for (int i = 0; i < GFS_NAME_MAX / 8; ++i) d |= w[i] ^ n.w[i];
+    return d == 0;
+}
+
+// fstate.h policy: agent-scope atomics, the context's wait limit.
+struct fst_dev {
+    uint64_t t0, limit;
+    __device__ uint64_t fadd(uint64_t* w, uint64_t v) { return gfs::fadd(w, v); }
+    __device__ uint64_t load(uint64_t* w) { return ld_acq(w); }
+    __device__ bool cas(uint64_t* w, uint64_t& exp, uint64_t v) { return gfs::cas(w, exp, v); }
+    __device__ bool expired() { return now_ticks() - t0 > limit; }
+    __device__ void pause() { backoff(); }
+};
+
+// Host open of entry e (new or reopened), whose state is OPENING.  Fills the
+// host fields and moves the state to OPEN, or to CLOSED dropping this
+// caller's open on failure.
 __device__ inline int open_host(gfs_dev* g, gfs_file* e, int flags, bool reopen) {
     int s = -1;
     int64_t r = rpc_call_single(g, GFS_OP_OPEN, -1, flags, 0, 0, e->name, &s);
     count(&g->ctr.rpc_opens, 1);
-    if (s < 0) return (int)r;
-    gfs_rpc_slot* sl = &g->slots[s];
-    uint64_t size = sys_ld(&sl->size), ct = sys_ld(&sl->ctime_ns), mt = sys_ld(&sl->mtime_ns), ino = sys_ld(&sl->ino);
-    rpc_release(g, s);
-    if (r < 0) return (int)r;
-    uint64_t st = ld_rlx(&e->st);
-    uint32_t version = fst_version(st);
+    uint64_t size = 0, ct = 0, mt = 0, ino = 0;
+    if (s >= 0) {
+        gfs_rpc_slot* sl = &g->slots[s];
+        size = sys_ld(&sl->size), ct = sys_ld(&sl->ctime_ns), mt = sys_ld(&sl->mtime_ns), ino = sys_ld(&sl->ino);
+        rpc_release(g, s);
+    }
+    fst_dev P{0, 0};
+    if (s < 0 || r < 0) {
+        fst_open_done(&e->st, P, false, false);
+        return (int)r;
+    }
+    bool changed = false;
     if (reopen && (e->ctime_ns != ct || e->mtime_ns != mt || e->size != size || e->ino != ino)) {
         // the file changed on the host: drop every cached page of the old version
-        ++version;
+        changed = true;
         st_rlx(&e->root, 0ull);
     }
     e->host_fd = (int32_t)r;
@@ -527,17 +595,18 @@ __device__ inline int open_host(gfs_dev* g, gfs_file* e, int flags, bool reopen)
     e->mtime_ns = mt;
     e->ino = ino;
     e->dirty = 0;
-    st_rel(&e->st, fst_make(GFS_FS_OPEN, 1, version));
+    fst_open_done(&e->st, P, true, changed);
     return 0;
 }
 
 // Single-thread open (reference: single_thread_open, libgpufs/fs_calls.cu:152-225).
 __device__ inline int open_single(const char* name, int flags) {
     gfs_dev* g = gfs_g;
-    int len = 0;
-    uint32_t h = name_hash(name, &len);
-    if (len == 0) return -ENOENT;
-    if (len >= GFS_NAME_MAX) return -ENAMETOOLONG;
+    gname nm;
+    name_load(name, nm);
+    if (nm.len == 0) return -ENOENT;
+    if (nm.len >= GFS_NAME_MAX) return -ENAMETOOLONG;
+    const uint32_t h = nm.hash;
     const uint32_t mask = g->name_cap - 1;
     uint32_t pos = h & mask;
     int idx = -1;
@@ -550,54 +619,31 @@ __device__ inline int open_single(const char* name, int flags) {
                 if (m >= g->max_files) return E_FSTABLE_FULL;
                 mine = (int)m;
                 gfs_file* e = &g->files[mine];
-                for (int i = 0; i < GFS_NAME_MAX; ++i) e->name[i] = i <= len ? name[i] : 0;
-                e->name[len] = 0;
+                uint64_t* en = (uint64_t*)e->name;
+                for (int i = 0; i < GFS_NAME_MAX / 8; ++i) en[i] = nm.w[i];
                 e->name_hash = h;
                 e->root = 0;
                 e->host_fd = -1;
-                st_rlx(&e->st, fst_make(GFS_FS_OPENING, 1, 0));
+                st_rlx(&e->st, fst_make(GFS_FS_OPENING, 1, 0));   // not yet published
             }
             uint32_t exp = 0;
             if (cas(&g->name_index[pos], exp, (uint32_t)mine + 1u)) {
                 int r = open_host(g, &g->files[mine], flags, false);
-                if (r < 0) {
-                    st_rel(&g->files[mine].st, fst_make(GFS_FS_CLOSED, 0, 0));
-                    return E_IPC_OPEN_ERROR;
-                }
-                return mine;
+                return r < 0 ? E_IPC_OPEN_ERROR : mine;
             }
             v = exp;
         }
         gfs_file* e = &g->files[v - 1];
-        if (e->name_hash == h && name_eq(e->name, name)) { idx = (int)v - 1; break; }
+        if (e->name_hash == h && name_eq(e, nm)) { idx = (int)v - 1; break; }
     }
     if (mine >= 0 && idx != mine) st_rel(&g->files[mine].st, fst_make(GFS_FS_DEAD, 0, 0));
     if (idx < 0) return E_FSTABLE_FULL;
     gfs_file* e = &g->files[idx];
-    uint64_t t0 = now_ticks();
-    for (;;) {
-        uint64_t st = ld_acq(&e->st);
-        uint32_t s = fst_state(st);
-        if (s == GFS_FS_OPEN) {
-            uint64_t exp = st;
-            if (cas(&e->st, exp, fst_make(GFS_FS_OPEN, fst_opens(st) + 1, fst_version(st)))) return idx;
-            continue;
-        }
-        if (s == GFS_FS_CLOSED) {
-            uint64_t exp = st;
-            if (cas(&e->st, exp, fst_make(GFS_FS_OPENING, 1, fst_version(st)))) {
-                int r = open_host(g, e, flags, true);
-                if (r < 0) {
-                    st_rel(&e->st, fst_make(GFS_FS_CLOSED, 0, fst_version(st)));
-                    return E_IPC_OPEN_ERROR;
-                }
-                return idx;
-            }
-            continue;
-        }
-        if (now_ticks() - t0 > g->wait_ticks) return -ETIMEDOUT;
-        backoff();
-    }
+    fst_dev P{now_ticks(), g->wait_ticks};
+    const int r = fst_open(&e->st, P);
+    if (r == FST_JOINED) return idx;
+    if (r < 0) return r;
+    return open_host(g, e, flags, true) < 0 ? E_IPC_OPEN_ERROR : idx;
 }
 
 } // namespace gfs
@@ -872,7 +918,9 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
             }
             __syncthreads();
             if (uni(gfs_ctl.err)) { err = uni(gfs_ctl.err); break; }
+            GFS_TRACE(1);
             fill_from_host(g, cnt, pb, fsize);
+            GFS_TRACE(2);
             // copy every pinned page (hit or freshly filled) into the caller's buffer
             const uint32_t T = nthreads(), t = tid();
             uint32_t rot = 0;
@@ -889,6 +937,7 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
                 if (page_sync) __syncthreads();
             }
             __syncthreads();
+            GFS_TRACE(3);
             if (wave() == 0) {
                 const uint32_t ln = lane();
                 if (gfs_ctl.err) fail_own(g, fd, pb, cnt);
@@ -897,6 +946,7 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
                 if (ln == 0) gfs_ctl.a = waitm;
             }
             __syncthreads();
+            GFS_TRACE(4);
             if (uni(gfs_ctl.err)) { err = uni(gfs_ctl.err); break; }
             uint64_t nxt = uni(gfs_ctl.a);
             if (nxt == pending) {
@@ -1123,16 +1173,10 @@ __device__ inline int gclose(int fd) {
         uint32_t last = 0;
         if (!fd_valid(g, fd)) r = -EBADF;
         else {
-            gfs_file* e = &g->files[fd];
-            for (;;) {
-                uint64_t st = ld_acq(&e->st);
-                if (fst_state(st) != GFS_FS_OPEN || fst_opens(st) == 0) { r = -EBADF; break; }
-                uint32_t o = fst_opens(st);
-                uint64_t nw = o > 1 ? fst_make(GFS_FS_OPEN, o - 1, fst_version(st))
-                                    : fst_make(GFS_FS_CLOSING, 0, fst_version(st));
-                uint64_t exp = st;
-                if (cas(&e->st, exp, nw)) { last = o == 1; break; }
-            }
+            fst_dev P{0, 0};
+            const int c = fst_close(&g->files[fd].st, P);
+            if (c < 0) r = c;
+            else last = (uint32_t)c;
         }
         gfs_ctl.ret = r;
         gfs_ctl.flag = last;
@@ -1150,8 +1194,8 @@ __device__ inline int gclose(int fd) {
         if (r == 0 && c < 0) r = (int)c;
         e->host_fd = -1;
         e->dirty = 0;
-        uint64_t st = ld_rlx(&e->st);
-        st_rel(&e->st, fst_make(GFS_FS_CLOSED, 0, fst_version(st)));
+        fst_dev P{0, 0};
+        fst_close_done(&e->st, P);
         gfs_ctl.ret = r;
     }
     __syncthreads();
