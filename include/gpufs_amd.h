@@ -69,6 +69,7 @@ typedef struct gpufs_stats {
     uint64_t frames;         /* frames in the pool */
     uint64_t frames_valid;   /* frames VALID at the time of the call */
     uint64_t timeouts;       /* device waits that hit the wait limit (GPUFS_WAIT_TIMEOUT_S) */
+    uint64_t host_busy_ns;   /* time the host workers spent serving requests, summed over workers */
 } gpufs_stats;
 
 /* ---- lifecycle: replaces initializer(GPUGlobals_ptr*) (libgpufs/fs_initializer.cu:116)
