@@ -78,12 +78,13 @@ typedef struct gfs_file {
 } gfs_file;
 
 // One RPC slot; lives in coherent pinned host memory.  The GPU writes the
-// request fields, the host writes `result` and the open results.
+// request fields, the host writes `result` and the open results, then sets the
+// request's bit in its completion mask.
 typedef struct gfs_rpc_slot {
     uint32_t op;
     int32_t  host_fd;
     int32_t  flags;
-    uint32_t cq;        // completion counter the host increments when done
+    uint32_t cq;        // completion word index [0,16) | request tag [16,22): the host sets bit `tag` of cq[index]
     uint64_t offset;
     uint64_t length;
     int64_t  result;    // bytes moved, host fd for OPEN, or -errno
@@ -126,7 +127,7 @@ typedef struct gfs_dev {
     // RPC ring (coherent pinned host memory, device-visible pointers)
     gfs_rpc_slot* slots;
     uint64_t* sq;           // submission queue: (ticket << 16) | slot
-    uint32_t* cq;           // completion counters, one per slot
+    uint64_t* cq;           // completion masks, one per slot (a batch of <= 64 requests shares its first slot's)
     uint8_t*  staging;      // ring_depth << log_page bytes (+ padding)
     // allocation cursors (device atomics)
     unsigned long long fresh_next;   // frames handed out from the never-used range

@@ -70,7 +70,8 @@ struct wg_ctl {
     uint8_t  kind[64];
     int64_t  ret;
     int32_t  err;
-    uint32_t n_own;
+    uint64_t own;            // batch pages with a host read posted (bit = page index in the batch)
+    uint64_t rel;            // of those, pages whose ring slot was already returned
     uint32_t cq;
     uint32_t flag;
     uint32_t tmo;            // wait-limit decision of thread 0, broadcast to the workgroup
@@ -429,7 +430,8 @@ __device__ inline int rpc_claim(gfs_dev* g, bool need, uint32_t hint) {
 __device__ __forceinline__ void rpc_release(gfs_dev* g, int s) {
     __hip_atomic_fetch_or(&g->slot_free[s >> 6], 1ull << (s & 63), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
-// Fill the request fields of slot s and publish it to the host.
+// Fill the request fields of slot s and publish it to the host.  `cq` is the
+// completion word index | tag << 16: when done the host sets bit `tag` of it.
 __device__ inline void rpc_post(gfs_dev* g, int s, uint32_t op, int32_t host_fd, int32_t flags,
                                 uint64_t off, uint64_t len, uint32_t cq) {
     gfs_rpc_slot* r = &g->slots[s];
@@ -444,11 +446,12 @@ __device__ inline void rpc_post(gfs_dev* g, int s, uint32_t op, int32_t host_fd,
     unsigned long long t = fadd_rlx(&g->sq_tail, 1ull);
     sys_st_rel(&g->sq[t & (g->sq_depth - 1)], (uint64_t)((t << 16) | (uint64_t)s));
 }
-// Wait until completion counter `cq` reaches n.  Returns 0 or -ETIMEDOUT.
-__device__ inline int rpc_wait(gfs_dev* g, uint32_t cq, uint32_t n) {
+// Wait until every bit of `want` is set in completion mask `cq`.  Returns 0
+// or -ETIMEDOUT.
+__device__ inline int rpc_wait(gfs_dev* g, uint32_t cq, uint64_t want) {
     uint64_t t0 = now_ticks();
     uint32_t spins = 0;
-    while (sys_ld_acq(&g->cq[cq]) < n) {
+    while ((sys_ld_acq(&g->cq[cq]) & want) != want) {
         if ((++spins & 255) == 0 && now_ticks() - t0 > g->wait_ticks) return -ETIMEDOUT;
         backoff();
     }
@@ -475,7 +478,7 @@ __device__ inline int64_t rpc_call_single(gfs_dev* g, uint32_t op, int32_t host_
     }
     sys_st(&g->cq[s], 0u);
     rpc_post(g, s, op, host_fd, flags, off, len, (uint32_t)s);
-    int w = rpc_wait(g, (uint32_t)s, 1);
+    int w = rpc_wait(g, (uint32_t)s, 1ull);
     int64_t res = w ? (int64_t)w : sys_ld(&g->slots[s].result);
     if (slot_out) *slot_out = s;   // caller reads extra results, then releases
     else rpc_release(g, s);
@@ -766,11 +769,11 @@ __device__ inline void resolve_batch(gfs_dev* g, int fd, uint64_t pb, uint32_t c
             uint64_t poff = page << g->log_page;
             uint64_t plen = g->page_size;
             if (poff + plen > file->size) plen = file->size > poff ? file->size - poff : 0;
-            rpc_post(g, s, GFS_OP_READ, file->host_fd, 0, poff, plen, cq);
+            rpc_post(g, s, GFS_OP_READ, file->host_fd, 0, poff, plen, cq | (ln << 16));
         }
         wave_count(&g->ctr.rpc_reads, post);
     }
-    if (ln == 0) { gfs_ctl.n_own = popc(postm); gfs_ctl.cq = cq; }
+    if (ln == 0) { gfs_ctl.own = postm; gfs_ctl.rel = 0; gfs_ctl.cq = cq; }
 }
 
 // Drop the pins taken by resolve_batch; OWN frames are published VALID first
@@ -786,7 +789,7 @@ __device__ inline void finish_batch(gfs_dev* g, uint32_t cnt) {
         // FILLING -> VALID and drop the owner's pin in one release add
         fadd(&g->meta[f], (GFS_F_VALID - GFS_F_FILLING) * (1ull << GFS_ST_SHIFT) - 1ull);
     }
-    if (gfs_ctl.slot[ln] != 0xFFFFFFFFu) rpc_release(g, (int)gfs_ctl.slot[ln]);
+    if (gfs_ctl.slot[ln] != 0xFFFFFFFFu && !((gfs_ctl.rel >> ln) & 1ull)) rpc_release(g, (int)gfs_ctl.slot[ln]);
 }
 
 // Undo OWN frames whose fill failed: clear the leaf and free the frame.
@@ -803,7 +806,7 @@ __device__ inline void fail_own(gfs_dev* g, int fd, uint64_t pb, uint32_t cnt) {
         cas(sp, exp, 0ull);
     }
     frame_release(g, f);
-    if (gfs_ctl.slot[ln] != 0xFFFFFFFFu) rpc_release(g, (int)gfs_ctl.slot[ln]);
+    if (gfs_ctl.slot[ln] != 0xFFFFFFFFu && !((gfs_ctl.rel >> ln) & 1ull)) rpc_release(g, (int)gfs_ctl.slot[ln]);
     gfs_ctl.slot[ln] = 0xFFFFFFFFu;
 }
 
@@ -814,46 +817,78 @@ __device__ __forceinline__ uint8_t* staging_ptr(gfs_dev* g, uint32_t s) {
     return g->staging + ((uint64_t)s << g->log_page);
 }
 
-// Wait for the batch's host reads and copy staging -> frames (all threads).
-// Sets kind ERR for failed pages; returns the first error or 0.
+// Host reads of the batch's OWN pages, streamed (all threads): thread 0 waits
+// for any read not yet taken, then the workgroup copies every page that has
+// arrived from staging into its frame and returns those slots to the ring,
+// while the host works on the rest.  The slot whose completion word the batch
+// uses is kept to the end (finish_batch returns it).  Pages whose read failed
+// or came up short are marked ERR and gfs_ctl.err is set.
 __device__ inline void fill_from_host(gfs_dev* g, uint32_t cnt, uint64_t pb, uint64_t fsize) {
-    if (wave() == 0 && lane() == 0) {
-        gfs_ctl.err = 0;
-        if (gfs_ctl.n_own) {
-            int w = rpc_wait(g, gfs_ctl.cq, gfs_ctl.n_own);
-            if (w) gfs_ctl.err = w;
+    (void)cnt;
+    const uint64_t own = uni(gfs_ctl.own);
+    if (own == 0) return;
+    const uint32_t cq = uni(gfs_ctl.cq);
+    const uint32_t T = nthreads(), t = tid();
+    uint64_t done = 0, filled = 0;
+    uint32_t rot = 0;
+    while (done != own) {
+        if (t == 0) {
+            const uint64_t t0 = now_ticks();
+            uint32_t spins = 0;
+            uint64_t m;
+            gfs_ctl.tmo = 0u;
+            while (((m = sys_ld_acq(&g->cq[cq])) & own & ~done) == 0) {
+                if ((++spins & 255) == 0 && now_ticks() - t0 > g->wait_ticks) {
+                    gfs_ctl.tmo = 1u;
+                    count(&g->ctr.timeouts, 1);
+                    m = own;   // everything not yet taken fails
+                    break;
+                }
+                backoff();
+            }
+            gfs_ctl.a = m & own & ~done;
         }
-    }
-    __syncthreads();
-    if (uni(gfs_ctl.n_own) == 0) return;
-    const int werr = uni(gfs_ctl.err);
-    __syncthreads();
-    if (wave() == 0) {
-        const uint32_t ln = lane();
-        if (ln < cnt && gfs_ctl.kind[ln] == GFS_K_OWN && gfs_ctl.slot[ln] != 0xFFFFFFFFu) {
-            int64_t r = werr ? (int64_t)werr : sys_ld(&g->slots[gfs_ctl.slot[ln]].result);
-            uint64_t poff = (pb + ln) << g->log_page;
-            uint64_t want = fsize > poff ? fsize - poff : 0;
-            if (want > g->page_size) want = g->page_size;
-            if (r < 0 || (uint64_t)r != want) {
-                gfs_ctl.kind[ln] = GFS_K_ERR;
-                atomicCAS(&gfs_ctl.err, 0, r < 0 ? (int)r : -EIO);
-            } else {
-                g->fbytes[gfs_ctl.frame[ln]] = (uint32_t)r;
+        __syncthreads();
+        const uint64_t got = uni(gfs_ctl.a);
+        const bool tmo = uni(gfs_ctl.tmo) != 0u;
+        if (wave() == 0) {
+            const uint32_t ln = lane();
+            if ((got >> ln) & 1ull) {
+                int64_t r = tmo ? (int64_t)-ETIMEDOUT : sys_ld(&g->slots[gfs_ctl.slot[ln]].result);
+                const uint64_t poff = (pb + ln) << g->log_page;
+                uint64_t want = fsize > poff ? fsize - poff : 0;
+                if (want > g->page_size) want = g->page_size;
+                if (r < 0 || (uint64_t)r != want) {
+                    gfs_ctl.kind[ln] = GFS_K_ERR;
+                    atomicCAS(&gfs_ctl.err, 0, r < 0 ? (int)r : -EIO);
+                } else {
+                    g->fbytes[gfs_ctl.frame[ln]] = (uint32_t)r;
+                }
             }
         }
+        __syncthreads();
+        for (uint64_t m = got; m; m &= m - 1) {
+            const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1;
+            if (uni((uint32_t)gfs_ctl.kind[i]) != GFS_K_OWN) continue;
+            const uint32_t f = gfs_ctl.frame[i];
+            const uint64_t bytes = g->fbytes[f];
+            copy_seg<true, false>(frame_ptr(g, f), staging_ptr(g, gfs_ctl.slot[i]), bytes, (t + T - rot) % T, T);
+            rot = (rot + (uint32_t)(bytes >> 4)) % T;
+            filled += bytes;
+        }
+        // every staging load of the round has returned (its data has been stored)
+        __syncthreads();
+        if (wave() == 0) {
+            const uint32_t ln = lane();
+            const bool give = ((got >> ln) & 1ull) && gfs_ctl.kind[ln] == GFS_K_OWN && gfs_ctl.slot[ln] != cq;
+            if (give) rpc_release(g, (int)gfs_ctl.slot[ln]);
+            const uint64_t gm = __ballot(give);
+            if (ln == 0) gfs_ctl.rel |= gm;
+        }
+        done |= got;
+        if (tmo) break;
     }
-    __syncthreads();
-    const uint32_t T = nthreads(), t = tid();
-    uint32_t rot = 0;
-    for (uint32_t i = 0; i < cnt; ++i) {
-        if (gfs_ctl.kind[i] != GFS_K_OWN || gfs_ctl.slot[i] == 0xFFFFFFFFu) continue;
-        uint32_t f = gfs_ctl.frame[i];
-        uint64_t bytes = g->fbytes[f];
-        copy_seg<true, false>(frame_ptr(g, f), staging_ptr(g, gfs_ctl.slot[i]), bytes, (t + T - rot) % T, T);
-        rot = (rot + (uint32_t)(bytes >> 4)) % T;
-        if (t == 0) count(&g->ctr.bytes_filled, bytes);
-    }
+    if (t == 0 && filled) count(&g->ctr.bytes_filled, filled);
     __threadfence();
     __syncthreads();
 }
@@ -1010,8 +1045,9 @@ __device__ inline size_t gwrite(int fd, size_t offset, size_t size, uchar* buffe
             fill_from_host(g, cnt, pb, fsize0);
             const uint32_t T = nthreads(), t = tid();
             // zero the never-filled part of host-less OWN pages, then copy the caller's bytes
+            const uint64_t posted = uni(gfs_ctl.own);
             for (uint32_t i = 0; i < cnt; ++i) {
-                if (gfs_ctl.kind[i] != GFS_K_OWN || gfs_ctl.slot[i] != 0xFFFFFFFFu) continue;
+                if (gfs_ctl.kind[i] != GFS_K_OWN || ((posted >> i) & 1ull)) continue;
                 uint8_t* fp = frame_ptr(g, gfs_ctl.frame[i]);
                 for (uint64_t k = t; k < (PS >> 4); k += T) *(v4u*)(fp + 16 * k) = v4u{0, 0, 0, 0};
             }
@@ -1130,7 +1166,7 @@ __device__ inline int flush_file(gfs_dev* g, int fd) {
                     rpc_post(g, (int)s, wronce ? GFS_OP_DIFF : GFS_OP_WRITE, file->host_fd, 0, poff, bytes, s);
                     count(&g->ctr.rpc_writes, 1);
                     count(&g->ctr.bytes_written, bytes);
-                    int w = rpc_wait(g, s, 1);
+                    int w = rpc_wait(g, s, 1ull);
                     int64_t r = w ? (int64_t)w : sys_ld(&g->slots[s].result);
                     if (r < 0 || (uint64_t)r != bytes) err = r < 0 ? (int)r : -EIO;
                     rpc_release(g, (int)s);
@@ -1265,7 +1301,7 @@ __device__ inline volatile void* gmmap(void* addr, size_t size, int prot, int fl
         __syncthreads();
         if (uni(gfs_ctl.err)) return GMAP_FAILED;
         fill_from_host(g, 1, page, fsize);
-        if (uni((uint32_t)gfs_ctl.kind[0]) == GFS_K_OWN && uni(gfs_ctl.slot[0]) == 0xFFFFFFFFu) {
+        if (uni((uint32_t)gfs_ctl.kind[0]) == GFS_K_OWN && (uni(gfs_ctl.own) & 1ull) == 0) {
             uint8_t* fp = frame_ptr(g, gfs_ctl.frame[0]);
             for (uint64_t k = tid(); k < (g->page_size >> 4); k += nthreads()) *(v4u*)(fp + 16 * k) = v4u{0, 0, 0, 0};
             __threadfence();
@@ -1358,7 +1394,7 @@ __device__ inline int gmsync(volatile void* addr, size_t length, int flags) {
         rpc_post(g, (int)s, wronce ? GFS_OP_DIFF : GFS_OP_WRITE, file->host_fd, 0, poff, bytes, s);
         count(&g->ctr.rpc_writes, 1);
         count(&g->ctr.bytes_written, bytes);
-        int w = rpc_wait(g, s, 1);
+        int w = rpc_wait(g, s, 1ull);
         int64_t r = w ? (int64_t)w : sys_ld(&g->slots[s].result);
         rpc_release(g, (int)s);
         gfs_ctl.ret = (r < 0 || (uint64_t)r != bytes) ? (r < 0 ? r : -EIO) : 0;

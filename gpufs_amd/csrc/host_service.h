@@ -8,8 +8,9 @@
 //  * the worker pread()s straight into the slot's pinned staging buffer (the
 //    GPU copies it into HBM), so there is no host-side batching thread and no
 //    H2D memcpy on the host;
-//  * completion = result field + a release increment of the batch's completion
-//    counter, which the GPU polls;
+//  * completion = result field + a release OR of the request's bit into its
+//    batch's 64-bit completion mask, which one thread of the workgroup polls,
+//    so a workgroup can use each page as soon as it has arrived;
 //  * the service runs independently of kernel completion (start/stop), which
 //    persistent and graph-launched kernels need; run_handler adds the
 //    reference's "until the stream is idle" behaviour on top.
@@ -34,7 +35,7 @@ namespace gfs_host {
 struct RingView {
     gfs_rpc_slot* slots = nullptr;
     uint64_t* sq = nullptr;
-    uint32_t* cq = nullptr;
+    uint64_t* cq = nullptr;     // completion masks
     uint8_t* staging = nullptr;
     uint32_t sq_depth = 0;      // power of two
     uint32_t ring_depth = 0;
@@ -149,8 +150,10 @@ static inline int64_t serve_slot(const RingView& rv, uint32_t s, ServiceCounters
     }
     if (res < 0) ctr.errors++;
     ctr.requests++;
-    __atomic_store_n(&r->result, res, __ATOMIC_RELAXED);
-    __atomic_fetch_add(&rv.cq[r->cq], 1u, __ATOMIC_RELEASE);
+    // result first, then the completion bit; a reader of either sees the staging data
+    const uint32_t cqw = r->cq;
+    __atomic_store_n(&r->result, res, __ATOMIC_RELEASE);
+    __atomic_fetch_or(&rv.cq[cqw & 0xFFFFu], 1ull << ((cqw >> 16) & 63u), __ATOMIC_RELEASE);
     return res;
 }
 

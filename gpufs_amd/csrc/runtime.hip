@@ -181,11 +181,11 @@ extern "C" int gpufs_init(const gpufs_config* cfg_in, gpufs_ctx** out) {
     const unsigned hflags = hipHostMallocMapped | hipHostMallocCoherent;
     TRY_ALLOC(hipHostMalloc((void**)&h.slots, sizeof(gfs_rpc_slot) * h.ring_depth, hflags));
     TRY_ALLOC(hipHostMalloc((void**)&h.sq, sizeof(uint64_t) * h.sq_depth, hflags));
-    TRY_ALLOC(hipHostMalloc((void**)&h.cq, sizeof(uint32_t) * h.ring_depth, hflags));
+    TRY_ALLOC(hipHostMalloc((void**)&h.cq, sizeof(uint64_t) * h.ring_depth, hflags));
     TRY_ALLOC(hipHostMalloc((void**)&h.staging, ((size_t)h.ring_depth << h.log_page) + kPad, hflags));
     memset(h.slots, 0, sizeof(gfs_rpc_slot) * h.ring_depth);
     memset(h.sq, 0xFF, sizeof(uint64_t) * h.sq_depth);
-    memset(h.cq, 0, sizeof(uint32_t) * h.ring_depth);
+    memset(h.cq, 0, sizeof(uint64_t) * h.ring_depth);
     {
         std::vector<uint64_t> bits(h.slot_words, 0);
         for (uint32_t s = 0; s < h.ring_depth; ++s) bits[s >> 6] |= 1ull << (s & 63);

@@ -1,8 +1,11 @@
 // CPU model of the GPU->host RPC ring: client threads play the device side of
 // the protocol (claim a slot, write the request, publish (ticket << 16 | slot)
-// in the submission queue, wait for the completion counter, read the result)
-// against the real host service (host_service.h).  Checks every result, the
-// data in staging, slot conservation, and runs under ThreadSanitizer.
+// in the submission queue, wait for the request's bit in a completion mask,
+// read the result) against the real host service (host_service.h).  Batched
+// clients post up to 64 reads sharing one completion word, take each page as
+// its bit appears and return its slot at once, keeping the word's slot to the
+// end, as a workgroup's page fill does.  Checks every result, the data in
+// staging, slot conservation, and runs under ThreadSanitizer.
 #include <atomic>
 #include <cstdio>
 #include <cstdlib>
@@ -23,7 +26,7 @@ struct Ring {
     gfs_host::RingView rv;
     std::vector<gfs_rpc_slot> slots;
     std::vector<uint64_t> sq;
-    std::vector<uint32_t> cq;
+    std::vector<uint64_t> cq;
     std::vector<uint8_t> staging;
     std::mutex mu;
     std::vector<uint32_t> free_slots;
@@ -43,11 +46,49 @@ struct Ring {
         r->op = op; r->host_fd = fd; r->flags = flags; r->cq = (uint32_t)s; r->offset = off; r->length = len;
         r->result = INT64_MIN;
         if (name) { memset(r->name, 0, GFS_NAME_MAX); strncpy(r->name, name, GFS_NAME_MAX - 1); }
-        __atomic_store_n(&cq[s], 0u, __ATOMIC_RELAXED);
+        __atomic_store_n(&cq[s], 0ull, __ATOMIC_RELAXED);
+        publish(s);
+        while ((__atomic_load_n(&cq[s], __ATOMIC_ACQUIRE) & 1ull) == 0) std::this_thread::yield();
+        return __atomic_load_n(&r->result, __ATOMIC_RELAXED);
+    }
+    void publish(int s) {
         uint64_t t = tail.fetch_add(1);
         __atomic_store_n(&sq[t & (rv.sq_depth - 1)], (t << 16) | (uint64_t)s, __ATOMIC_RELEASE);
-        while (__atomic_load_n(&cq[s], __ATOMIC_ACQUIRE) < 1) std::this_thread::yield();
-        return __atomic_load_n(&r->result, __ATOMIC_RELAXED);
+    }
+    // device side of a page fill: n <= 64 reads share slot ss[0]'s completion
+    // word; each page is checked as soon as its bit is set and its slot (other
+    // than ss[0]) goes back at once.  Returns the number of pages checked.
+    int batch(const std::vector<int>& ss, int fd, const std::vector<uint64_t>& offs, uint64_t len,
+              const std::vector<uint8_t>& data) {
+        const int n = (int)ss.size();
+        const uint32_t cqi = (uint32_t)ss[0];
+        __atomic_store_n(&cq[cqi], 0ull, __ATOMIC_RELAXED);
+        for (int i = 0; i < n; ++i) {
+            gfs_rpc_slot* r = &slots[ss[i]];
+            r->op = GFS_OP_READ; r->host_fd = fd; r->flags = 0; r->cq = cqi | ((uint32_t)i << 16);
+            r->offset = offs[i]; r->length = len; r->result = INT64_MIN;
+            publish(ss[i]);
+        }
+        const uint64_t all = n == 64 ? ~0ull : ((1ull << n) - 1);
+        uint64_t done = 0;
+        int checked = 0;
+        while (done != all) {
+            uint64_t m = __atomic_load_n(&cq[cqi], __ATOMIC_ACQUIRE) & all & ~done;
+            if (!m) { std::this_thread::yield(); continue; }
+            for (uint64_t b = m; b; b &= b - 1) {
+                const int i = __builtin_ctzll(b);
+                const int64_t got = __atomic_load_n(&slots[ss[i]].result, __ATOMIC_ACQUIRE);
+                const uint64_t off = offs[i];
+                const uint64_t exp = off >= data.size() ? 0 : std::min<uint64_t>(len, data.size() - off);
+                CHECK(got == (int64_t)exp, "batch read %d off=%llu got=%lld", i, (unsigned long long)off, (long long)got);
+                if (got > 0) CHECK(memcmp(&staging[(size_t)ss[i] << rv.log_page], &data[off], (size_t)got) == 0, "batch bytes");
+                if (i != 0) release(ss[i]);
+                ++checked;
+            }
+            done |= m;
+        }
+        release(ss[0]);
+        return checked;
     }
 };
 
@@ -95,6 +136,31 @@ int main(int argc, char** argv) {
     for (auto& t : cl) t.join();
     CHECK(done == 12 * 300, "done");
     CHECK(ring.free_slots.size() == depth, "slots conserved");
+    // batched fills: 4 clients, batches of 1..6 pages through the 8 slots
+    std::atomic<int> pages{0};
+    std::vector<std::thread> bl;
+    for (int c = 0; c < 4; ++c) {
+        bl.emplace_back([&, c]() {
+            std::mt19937_64 r(900 + c);
+            for (int k = 0; k < 100; ++k) {
+                const int n = 1 + (int)(r() % 6);
+                std::vector<int> ss;
+                while ((int)ss.size() < n) {   // all-or-nothing claim, as a slot-limited batch
+                    int s = ring.claim();
+                    if (s >= 0) { ss.push_back(s); continue; }
+                    for (int x : ss) ring.release(x);
+                    ss.clear();
+                    std::this_thread::yield();
+                }
+                std::vector<uint64_t> offs(n);
+                for (auto& o : offs) o = (r() % (data.size() / 4096 + 1)) * 4096;
+                pages += ring.batch(ss, (int)hfd, offs, 4096, data);
+            }
+        });
+    }
+    for (auto& t : bl) t.join();
+    CHECK(ring.free_slots.size() == depth, "slots conserved after batches");
+    const int batch_pages = pages.load();
     // write + OR-merge + truncate on a second file
     std::string wpath = std::string(dir) + "/ring_model_w.bin";
     { int fd = open(wpath.c_str(), O_CREAT | O_TRUNC | O_WRONLY, 0644); uint8_t b[8] = {1, 2, 4, 8, 0, 0, 0, 0}; CHECK(write(fd, b, 8) == 8, "w"); close(fd); }
@@ -118,7 +184,8 @@ int main(int argc, char** argv) {
     ring.release(s);
     { struct stat sb; stat(wpath.c_str(), &sb); CHECK(sb.st_size == 5, "size after trunc"); }
     svc.stop();
-    CHECK(svc.counters().requests.load() == 2 + 3600 + 7, "request count %llu", (unsigned long long)svc.counters().requests.load());
+    CHECK(svc.counters().requests.load() == (uint64_t)(2 + 3600 + batch_pages + 7), "request count %llu",
+          (unsigned long long)svc.counters().requests.load());
     unlink(path.c_str()); unlink(wpath.c_str());
     if (failures) { fprintf(stderr, "%d failures\n", failures); return 1; }
     printf("ring model OK\n");
