@@ -472,12 +472,11 @@ def bench_serve(args, torch, L, _lib, dg):
             nreq = max(100, int(rate * args.serve_seconds))
             ids = dg.zipf_sample(n, nreq, seed=dg.SEED_C4 + 10 + k, s=1.1)
             arr = np.cumsum(dg.poisson_gaps(rate, nreq, seed=dg.SEED_C4 + 20 + k))
-            s0 = ctx.stats()
+            h0, m0 = srv.pages()
             c, st, lat, el = srv.run(ids, arr)
-            s1 = ctx.stats()
+            h1, m1 = srv.pages()
             assert (st == 0).all()
-            hits = s1["page_hits"] - s0["page_hits"]
-            miss = s1["page_misses"] - s0["page_misses"]
+            hits, miss = h1 - h0, m1 - m0
             r = {"rate_req_s": rate, "requests": nreq, "files_per_s": round(nreq / el, 1),
                  "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 4),
                  "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 4),

@@ -77,6 +77,7 @@ struct wg_ctl {
     uint32_t tmo;            // wait-limit decision of thread 0, broadcast to the workgroup
     uint64_t a, b;
     uint32_t rot;
+    uint32_t hits, misses;   // running page-lookup counts of this workgroup (wraps; callers take differences)
 };
 
 __device__ __forceinline__ uint32_t tid() {
@@ -751,6 +752,10 @@ __device__ inline void resolve_batch(gfs_dev* g, int fd, uint64_t pb, uint32_t c
     wave_count(&g->ctr.page_waits, kind == GFS_K_WAIT && mine);
     wave_count(&g->ctr.page_hits, kind == GFS_K_HIT);
     wave_count(&g->ctr.page_misses, kind == GFS_K_OWN);
+    {
+        const uint64_t hm = __ballot(kind == GFS_K_HIT), om = __ballot(kind == GFS_K_OWN);
+        if (ln == 0) { gfs_ctl.hits += popc(hm); gfs_ctl.misses += popc(om); }
+    }
     fence_acq_agent();
     if (ln < 64) {
         gfs_ctl.kind[ln] = (uint8_t)kind;

@@ -36,11 +36,12 @@ struct serve_req {
 __global__ __launch_bounds__(1024) void k_serve(const char* names, uint32_t stride, uint32_t nfiles,
                                                 const char* dict, uint32_t nw, uint32_t chunk,
                                                 serve_req* ring, uint32_t ring_mask, uint32_t* results,
-                                                int32_t* res_status, uint64_t* done,
+                                                int32_t* res_status, uint32_t* res_pages, uint64_t* done,
                                                 unsigned long long* ticket, uint64_t idle_ticks) {
     __shared__ __attribute__((aligned(16))) grep_lds S;
     __shared__ uint32_t fid, slot;
     __shared__ uint64_t myseq;
+    __shared__ uint32_t h0, m0;
     grep_build_dict(S, dict, nw);
     const uint32_t t = gfs::tid();
     for (;;) {
@@ -60,6 +61,8 @@ __global__ __launch_bounds__(1024) void k_serve(const char* names, uint32_t stri
                 slot = gfs::sys_ld(&e->slot);
             }
             myseq = k + 1;
+            h0 = gfs_ctl.hits;
+            m0 = gfs_ctl.misses;
         }
         __syncthreads();
         const uint32_t f = gfs::uni(fid), s = gfs::uni(slot);
@@ -71,6 +74,8 @@ __global__ __launch_bounds__(1024) void k_serve(const char* names, uint32_t stri
         __threadfence_system();
         __syncthreads();
         if (t == 0) {
+            gfs::sys_st(&res_pages[2 * (size_t)s], gfs_ctl.hits - h0);
+            gfs::sys_st(&res_pages[2 * (size_t)s + 1], gfs_ctl.misses - m0);
             gfs::sys_st(&res_status[s], st);
             gfs::fence_rel_sys();
             gfs::sys_st_rel(&done[s], sq);
@@ -91,11 +96,15 @@ struct gpufs_server {
     uint32_t ring_mask;
     uint32_t* results;   // host coherent: slots x nw
     int32_t* rstatus;    // host coherent: slots
+    uint32_t* rpages;    // host coherent: slots x 2 (page hits, misses of the request)
     uint64_t* done;      // host coherent: slots
+    uint64_t hits, misses;   // summed over completed requests
     unsigned long long* ticket;   // device
     uint64_t next_ticket;
     bool running;
 };
+
+void gpufs_resident_add(gpufs_ctx* c, int d);
 
 static uint64_t serve_now_ns() {
     return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
@@ -142,11 +151,12 @@ extern "C" int gpufs_serve_start(gpufs_ctx* ctx, const gpufs_serve_config* cfg, 
     bool ok = hipHostMalloc((void**)&S->ring, sizeof(serve_req) * rs, hf) == hipSuccess &&
               hipHostMalloc((void**)&S->results, sizeof(uint32_t) * (size_t)cfg->slots * cfg->n_words, hf) == hipSuccess &&
               hipHostMalloc((void**)&S->rstatus, sizeof(int32_t) * cfg->slots, hf) == hipSuccess &&
+              hipHostMalloc((void**)&S->rpages, sizeof(uint32_t) * 2 * (size_t)cfg->slots, hf) == hipSuccess &&
               hipHostMalloc((void**)&S->done, sizeof(uint64_t) * cfg->slots, hf) == hipSuccess &&
               hipMalloc((void**)&S->ticket, sizeof(unsigned long long)) == hipSuccess;
     if (!ok) {
         (void)hipHostFree(S->ring); (void)hipHostFree(S->results); (void)hipHostFree(S->rstatus);
-        (void)hipHostFree(S->done); (void)hipFree(S->ticket);
+        (void)hipHostFree(S->rpages); (void)hipHostFree(S->done); (void)hipFree(S->ticket);
         delete S;
         return -ENOMEM;
     }
@@ -161,9 +171,10 @@ extern "C" int gpufs_serve_start(gpufs_ctx* ctx, const gpufs_serve_config* cfg, 
     const uint64_t idle = hd.wait_ticks * 2;
     hipLaunchKernelGGL(k_serve, dim3(S->workgroups), dim3(S->threads), 0, S->stream, names_dev, name_stride,
                        n_files, dict_dev, cfg->n_words, cfg->chunk_bytes, S->ring, S->ring_mask, S->results,
-                       S->rstatus, S->done, S->ticket, idle);
+                       S->rstatus, S->rpages, S->done, S->ticket, idle);
     if (hipGetLastError() != hipSuccess) return -EIO;
     S->running = true;
+    gpufs_resident_add(ctx, 1);
     *out = S;
     return 0;
 }
@@ -203,6 +214,8 @@ extern "C" int gpufs_serve_run(gpufs_server* S, const uint32_t* file_ids, const 
             const uint64_t tc = serve_now_ns();
             memcpy(counts_out + (size_t)i * S->nw, S->results + (size_t)s * S->nw, sizeof(uint32_t) * S->nw);
             status_out[i] = S->rstatus[s];
+            S->hits += S->rpages[2 * (size_t)s];
+            S->misses += S->rpages[2 * (size_t)s + 1];
             latency_s_out[i] = (double)(tc - t0) * 1e-9 - arrival_s[i];
             last_done = tc;
             reserved -= slot_frames[s];
@@ -248,12 +261,21 @@ extern "C" int gpufs_serve_stop(gpufs_server* S) {
         }
         if (e != hipSuccess) rc = -EIO;
         S->running = false;
+        gpufs_resident_add(S->ctx, -1);
     }
     (void)hipHostFree(S->ring);
     (void)hipHostFree(S->results);
     (void)hipHostFree(S->rstatus);
+    (void)hipHostFree(S->rpages);
     (void)hipHostFree(S->done);
     (void)hipFree(S->ticket);
     delete S;
     return rc;
+}
+
+extern "C" int gpufs_serve_pages(gpufs_server* S, uint64_t* page_hits, uint64_t* page_misses) {
+    if (!S || !page_hits || !page_misses) return -EINVAL;
+    *page_hits = S->hits;
+    *page_misses = S->misses;
+    return 0;
 }

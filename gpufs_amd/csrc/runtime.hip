@@ -31,7 +31,13 @@ struct gpufs_ctx {
     gfs_dev* dev = nullptr;  // device copy
     gfs_host::Service service;
     std::mutex mu;
+    std::atomic<int> resident{0};   // resident (serving) kernels running on the context
 };
+
+// Resident kernels hold every compute unit, so a device copy issued beside
+// them (done by a blit kernel) would wait until they exit.
+__attribute__((visibility("hidden"))) void gpufs_resident_add(gpufs_ctx* c, int d) { c->resident += d; }
+#define REFUSE_IF_RESIDENT(c) do { if ((c)->resident.load() > 0) return -EBUSY; } while (0)
 
 #define HIP_TRY(x)                                                        \
     do {                                                                  \
@@ -283,6 +289,7 @@ __global__ void gfs_reset_files_kernel(gfs_file* files, uint32_t n) {
 
 extern "C" int gpufs_cache_drop(gpufs_ctx* c) {
     if (!c) return -EINVAL;
+    REFUSE_IF_RESIDENT(c);
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
     gfs_dev cur;
@@ -312,6 +319,7 @@ extern "C" int gpufs_cache_drop(gpufs_ctx* c) {
 
 extern "C" int gpufs_stats_get(gpufs_ctx* c, gpufs_stats* o) {
     if (!c || !o) return -EINVAL;
+    REFUSE_IF_RESIDENT(c);
     HIP_TRY(hipSetDevice(c->device));
     gfs_dev cur;
     HIP_TRY(hipMemcpy(&cur, c->dev, sizeof(gfs_dev), hipMemcpyDeviceToHost));
@@ -346,6 +354,7 @@ extern "C" int gpufs_stats_get(gpufs_ctx* c, gpufs_stats* o) {
 
 extern "C" int gpufs_stats_reset(gpufs_ctx* c) {
     if (!c) return -EINVAL;
+    REFUSE_IF_RESIDENT(c);
     HIP_TRY(hipSetDevice(c->device));
     gfs_counters z;
     memset(&z, 0, sizeof(z));
@@ -356,6 +365,7 @@ extern "C" int gpufs_stats_reset(gpufs_ctx* c) {
 // Host-side audit of the page cache (idle context only).
 extern "C" int gpufs_check_invariants(gpufs_ctx* c) {
     if (!c) return -EINVAL;
+    REFUSE_IF_RESIDENT(c);
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
     gfs_dev cur;

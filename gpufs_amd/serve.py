@@ -56,6 +56,12 @@ class Server:
                                                                   P(lat), ctypes.byref(el)))
         return counts, status, lat, el.value
 
+    def pages(self):
+        """(page hits, page misses) of the requests completed since start."""
+        h, m = ctypes.c_uint64(), ctypes.c_uint64()
+        _lib.check("gpufs_serve_pages", _lib.lib().gpufs_serve_pages(self.h, ctypes.byref(h), ctypes.byref(m)))
+        return int(h.value), int(m.value)
+
     def stop(self):
         if self.h:
             rc = _lib.lib().gpufs_serve_stop(self.h)

@@ -177,6 +177,12 @@ int gpufs_serve_run(gpufs_server* srv, const uint32_t* file_ids_host, const doub
                     uint32_t n_requests, uint32_t* counts_out_host, int32_t* status_out_host,
                     double* latency_s_out_host, double* elapsed_s_out);
 int gpufs_serve_stop(gpufs_server* srv);
+/* Page lookups (cache hits, misses) of the requests completed since
+ * gpufs_serve_start, counted by the serving workgroups themselves.  While a
+ * server runs, gpufs_stats_get, gpufs_stats_reset, gpufs_cache_drop and
+ * gpufs_check_invariants return -EBUSY: their device copies cannot run
+ * beside workgroups that hold every compute unit. */
+int gpufs_serve_pages(gpufs_server* srv, uint64_t* page_hits, uint64_t* page_misses);
 
 /* y = A x with A (rows x cols fp32, row-major) gmmap()ed page by page from
  * `path_dev` (samples/bigmatrix_mmap/bigmatrix_mmap.cu:66-168 pattern): workgroup

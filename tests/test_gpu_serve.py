@@ -1,10 +1,12 @@
 """Serving form: a resident request loop with continuous admission.  Every
 request's counts equal the CPU path for its file, and are the same when the
 request runs alone and when it runs in a Poisson mix with others in flight."""
+import errno
+
 import numpy as np
 import pytest
 
-from gpufs_amd import datagen as dg
+from gpufs_amd import _lib, datagen as dg
 from tests.gpu_helpers import Server, make_ctx, say
 
 pytestmark = pytest.mark.gpu
@@ -37,8 +39,16 @@ def test_serve_alone_equals_mix_equals_cpu(tmp_path, orc):
                 if f in alone:
                     assert (c[i] == alone[f]).all()
             assert (lat > 0).all() and el > 0
-            s = ctx.stats()
-            assert s["page_hits"] > 0   # popular files are served from the resident cache
+            hits, misses = srv.pages()
+            assert hits > 0   # popular files are served from the resident cache
+            assert misses > 0
+            # device-copy calls refuse to run beside the resident workgroups
+            with pytest.raises(_lib.GpufsError) as ei:
+                ctx.stats()
+            assert ei.value.code == -errno.EBUSY
         finally:
             srv.stop()
+        s = ctx.stats()
+        # the workgroups' own counts agree with the context counters
+        assert s["page_hits"] == hits and s["page_misses"] == misses
         assert ctx.check_invariants() == 0
