@@ -59,6 +59,10 @@ def parse():
     ap.add_argument("--serve-files", type=int, default=100000)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-rates", default="1000,10000,50000")
+    ap.add_argument("--serve-saturation", type=int, default=100000,
+                    help="requests offered all at once to measure the serving loop's capacity")
+    ap.add_argument("--serve-plain", type=int, default=5000,
+                    help="requests replayed as one kernel launch each (the plain form) for comparison")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy (known bytes) for HBM counter calibration")
     ap.add_argument("--workdir", default=os.environ.get("GPUFS_BENCH_DIR", "/tmp/gpufs_bench"))
@@ -220,12 +224,19 @@ def main():
         timed_pass()
     torch.cuda.synchronize()
     ctx.stats_reset()
-    kt = []
+    # K launches back to back on the context stream; an event between each pair
+    # gives every launch's duration without a host round trip between steps
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        kt.append(timed_pass())
+    evs[0].record(stream)
+    for i in range(args.steps):
+        launch()
+        evs[i + 1].record(stream)
+    evs[-1].synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    kt = [evs[i].elapsed_time(evs[i + 1]) / 1e3 for i in range(args.steps)]
     s = ctx.stats()
     assert s["page_misses"] == 0, s
     warm_ok = None
@@ -485,6 +496,45 @@ def bench_serve(args, torch, L, _lib, dg):
             log(f"served @ {rate:.0f}/s: {r}")
             all_ids.append(ids)
             all_counts.append(c)
+        # saturation: every request eligible at once -> the loop's capacity
+        nsat = args.serve_saturation
+        ids = dg.zipf_sample(n, nsat, seed=dg.SEED_C4 + 30, s=1.1)
+        h0, m0 = srv.pages()
+        c, st, lat, el = srv.run(ids, np.zeros(nsat))
+        h1, m1 = srv.pages()
+        assert (st == 0).all()
+        sat = {"requests": nsat, "files_per_s": round(nsat / el, 1),
+               "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 4),
+               "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 4),
+               "hit_ratio": round((h1 - h0) / max(1, h1 - h0 + m1 - m0), 4)}
+        out["saturation"] = sat
+        log(f"served, all requests eligible at once: {sat}")
+        all_ids.append(ids)
+        all_counts.append(c)
+        srv.stop()
+        # plain form on the same requests: one launch per request, same cache
+        npl = min(nsat, args.serve_plain)
+        outs = torch.zeros((npl, q.shape[0]), dtype=torch.int32, device="cuda")
+        sts = torch.zeros(npl, dtype=torch.int32, device="cuda")
+        stream = torch.cuda.ExternalStream(ctx.stream)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for i in range(npl):
+            _lib.check("plain", L.gpufs_launch_grep(ctx.h, ctypes.c_void_p(srv.names[int(ids[i])].data_ptr()), 256,
+                                                    1, ctypes.c_void_p(srv.dict.data_ptr()), q.shape[0], 65536,
+                                                    ctypes.c_void_p(outs[i].data_ptr()),
+                                                    ctypes.c_void_p(sts[i:].data_ptr()), 1024, None))
+        ev = torch.cuda.Event()
+        ev.record(stream)
+        ev.synchronize()
+        t_plain = time.perf_counter() - t
+        assert (sts.cpu().numpy() == 0).all()
+        assert (outs.cpu().numpy().view(np.uint32) == c[:npl]).all(), "plain and served counts differ"
+        plain = {"requests": npl, "files_per_s": round(npl / t_plain, 1),
+                 "form": "one gpufs_launch_grep launch per request on the context stream, same cache"}
+        out["plain_per_file_launch"] = plain
+        out["served_over_plain"] = round(sat["files_per_s"] / plain["files_per_s"], 2)
+        log(f"plain per-file launches: {plain}; served/plain = {out['served_over_plain']}x")
     finally:
         srv.stop()
         ctx.close()
