@@ -925,6 +925,7 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
     using namespace gfs;
     gfs_dev* g = gfs_g;
     __syncthreads();
+    GFS_TRACE(5);
     if (tid() == 0) {
         int64_t r = 0;
         if (!fd_valid(g, fd) || fst_state(ld_acq(&g->files[fd].st)) != GFS_FS_OPEN) r = -EBADF;
@@ -999,6 +1000,7 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
         }
     }
     __syncthreads();
+    GFS_TRACE(6);
     return err ? (size_t)(int64_t)err : (size_t)n;
 }
 

@@ -215,6 +215,7 @@ __device__ inline int grep_one_file(grep_lds& S, const char* name, uint32_t nw, 
         cursor += (uint64_t)chunk + ov;
         grep_scan_window(S, dsz);
         __syncthreads();
+        GFS_TRACE(7);
     }
     __syncthreads();
     int c = gclose(fd);
