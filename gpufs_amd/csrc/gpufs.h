@@ -921,7 +921,7 @@ __device__ inline size_t fstat(int fd) {
 
 // libgpufs/fs_calls.cu:689-737.  Reads [offset, offset+size) clipped at end of
 // file into `buffer` (global or LDS); returns the bytes read.
-__device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer) {
+__device__ __attribute__((noinline)) size_t gread(int fd, size_t offset, size_t size, uchar* buffer) {
     using namespace gfs;
     gfs_dev* g = gfs_g;
     __syncthreads();

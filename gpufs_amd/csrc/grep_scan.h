@@ -4,12 +4,14 @@
 // Window scan: each thread owns 64-byte segments of the LDS window.  A
 // segment is read with four ds_read_b128, word bytes ([A-Za-z_]) are flagged
 // four at a time with SWAR arithmetic into a 64-bit mask, token starts are
-// mask & ~(mask << 1 | previous byte).  A token of at most 32 bytes is read as
-// up to four little-endian u64 words (funnel shifts of aligned 8-byte LDS
-// loads, bytes past the token zeroed); its hash picks a slot of an LDS table
-// of the dictionary, and a match is an exact comparison of length and the four
-// words.  A token's bytes and a query word's bytes compare equal iff the
-// strings are equal, so the counts are exact.
+// mask & ~(mask << 1 | previous byte).  A token is skipped at once when no
+// query word has its length; otherwise its first eight bytes (a funnel shift
+// of two aligned LDS words, bytes past the token zeroed) and its length are
+// hashed with 32-bit arithmetic into an LDS table whose entries carry the
+// word's length, and only a slot with the same length and the same first
+// eight bytes leads to reading and comparing the token's remaining words.  A
+// token's bytes and a query word's bytes compare equal iff the strings are
+// equal, so the counts are exact.
 #pragma once
 #include "gpufs.h"
 
@@ -20,10 +22,11 @@
 struct grep_lds {
     uint8_t corpus[GREP_MAX_CHUNK + 128];    // window + look-ahead + NUL + read padding
     uint64_t words[GREP_MAX_WORDS][4];       // query bytes, zero-padded, little-endian u64
-    uint32_t ht[GREP_HT];                    // canonical word index + 1
+    uint32_t ht[GREP_HT];                    // (canonical word index + 1) | length << 16; 0 = empty
     uint32_t counts[GREP_MAX_WORDS];
     uint16_t canon[GREP_MAX_WORDS];          // 0xFFFF: word can never match
     uint8_t wlen[GREP_MAX_WORDS];
+    uint32_t lenmask;                        // bit L-1: some query word has length L
     int ov;
     int status;
     uint64_t fsize;
@@ -46,38 +49,31 @@ __device__ __forceinline__ uint32_t alpha4(uint32_t d) {
     return ((v >> 7) & 1u) | ((v >> 14) & 2u) | ((v >> 21) & 4u) | ((v >> 28) & 8u);
 }
 
-__device__ __forceinline__ uint32_t tok_hash(uint64_t a, uint64_t b, uint64_t c, uint64_t d, uint32_t len) {
-    uint64_t h = a * 0x9E3779B97F4A7C15ull;
-    h ^= b * 0xC2B2AE3D27D4EB4Full;
-    h ^= c * 0x165667B19E3779F9ull;
-    h ^= d * 0xD6E8FEB86659FD93ull;
-    h ^= (uint64_t)len * 0x27D4EB2F165667C5ull;
-    h ^= h >> 29;
-    h *= 0xBF58476D1CE4E5B9ull;
-    return (uint32_t)(h >> 32);
+// Hash of a token's first eight bytes and its length, 32-bit arithmetic.
+__device__ __forceinline__ uint32_t tok_hash8(uint64_t w0, uint32_t len) {
+    uint32_t h = (uint32_t)w0 ^ ((uint32_t)(w0 >> 32) * 0x9E3779B1u) ^ (len << 26);
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 15;
+    h *= 0xC2B2AE35u;
+    return h ^ (h >> 16);
 }
 
-// Bytes [p, p+len) of the LDS window as four zero-padded u64 (len <= 32).
-// Reads the aligned words covering [p & ~7, p + 40): inside the padded buffer.
-__device__ __forceinline__ void load_tok(const uint8_t* corpus, uint32_t p, uint32_t len, uint64_t tk[4]) {
+// Eight bytes of the LDS window starting at byte p (two aligned loads).
+__device__ __forceinline__ uint64_t lds_u64_at(const uint8_t* corpus, uint32_t p) {
     const uint64_t* W = (const uint64_t*)corpus;
     const uint32_t a = p >> 3, sh = (p & 7) * 8;
-    uint64_t w[5];
-#pragma unroll
-    for (int i = 0; i < 5; ++i) w[i] = W[a + i];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        uint64_t v = sh ? ((w[i] >> sh) | (w[i + 1] << (64 - sh))) : w[i];
-        const int lo = 8 * i;
-        if ((int)len <= lo) v = 0;
-        else if ((int)len < lo + 8) v &= (1ull << (8 * (len - lo))) - 1;
-        tk[i] = v;
-    }
+    const uint64_t lo = W[a], hi = W[a + 1];
+    return sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
+}
+// The low n bytes of v (n may be <= 0 or >= 8).
+__device__ __forceinline__ uint64_t keep_bytes(uint64_t v, int n) {
+    return n >= 8 ? v : (n <= 0 ? 0ull : (v & ((1ull << (8 * n)) - 1)));
 }
 
 __device__ inline void grep_build_dict(grep_lds& S, const char* dict, uint32_t nw) {
     const uint32_t T = gfs::nthreads(), t = gfs::tid();
     for (uint32_t i = t; i < GREP_HT; i += T) S.ht[i] = 0;
+    if (t == 0) S.lenmask = 0;
     for (uint32_t q = t; q < nw; q += T) {
         uint32_t len = 0;
         bool ok = true;
@@ -104,11 +100,13 @@ __device__ inline void grep_build_dict(grep_lds& S, const char* dict, uint32_t n
     __syncthreads();
     for (uint32_t q = t; q < nw; q += T) {
         if (S.canon[q] == 0xFFFF) continue;
-        uint32_t pos = tok_hash(S.words[q][0], S.words[q][1], S.words[q][2], S.words[q][3], S.wlen[q]) & (GREP_HT - 1);
+        const uint32_t len = S.wlen[q];
+        atomicOr(&S.lenmask, 1u << (len - 1));
+        uint32_t pos = tok_hash8(S.words[q][0], len) & (GREP_HT - 1);
         for (;;) {
-            uint32_t old = atomicCAS(&S.ht[pos], 0u, q + 1);
+            uint32_t old = atomicCAS(&S.ht[pos], 0u, (q + 1) | (len << 16));
             if (old == 0) break;
-            uint32_t r = old - 1;
+            uint32_t r = (old & 0xFFFFu) - 1;
             if (S.wlen[r] == S.wlen[q] && S.words[r][0] == S.words[q][0] && S.words[r][1] == S.words[q][1] &&
                 S.words[r][2] == S.words[q][2] && S.words[r][3] == S.words[q][3]) {
                 S.canon[q] = (uint16_t)r;
@@ -134,6 +132,7 @@ __device__ inline void grep_scan_window(grep_lds& S, uint32_t dsz) {
     typedef __attribute__((address_space(3))) gfs::v4u lv4;
     const uint32_t T = gfs::nthreads(), t = gfs::tid();
     const uint32_t nseg = (dsz + 63) >> 6;
+    const uint32_t lm = S.lenmask;
     for (uint32_t sg = t; sg < nseg; sg += T) {
         const uint32_t base = sg << 6;
         const lv4* src = (const lv4*)(S.corpus + base);
@@ -159,18 +158,23 @@ __device__ inline void grep_scan_window(grep_lds& S, uint32_t dsz) {
                 while (q < p + 33 && g_alpha(S.corpus[q])) ++q;
                 len = q - p;
             }
-            if (len > 32) continue;
-            uint64_t tk[4];
-            load_tok(S.corpus, p, len, tk);
-            uint32_t pos = tok_hash(tk[0], tk[1], tk[2], tk[3], len) & (GREP_HT - 1);
+                if (len > 32 || !((lm >> (len - 1)) & 1u)) continue;
+            const uint64_t t0 = keep_bytes(lds_u64_at(S.corpus, p), (int)len);
+            uint32_t pos = tok_hash8(t0, len) & (GREP_HT - 1);
             for (;;) {
                 const uint32_t e = S.ht[pos];
                 if (!e) break;
-                const uint32_t w = e - 1;
-                if (S.wlen[w] == len && S.words[w][0] == tk[0] && S.words[w][1] == tk[1] && S.words[w][2] == tk[2] &&
-                    S.words[w][3] == tk[3]) {
-                    atomicAdd(&S.counts[w], 1u);
-                    break;
+                if ((e >> 16) == len) {
+                    const uint32_t w = (e & 0xFFFFu) - 1;
+                    if (S.words[w][0] == t0) {
+                        bool eq = true;
+                        for (uint32_t i = 1; 8 * i < len; ++i)
+                            eq = eq && S.words[w][i] == keep_bytes(lds_u64_at(S.corpus, p + 8 * i), (int)(len - 8 * i));
+                        if (eq) {
+                            atomicAdd(&S.counts[w], 1u);
+                            break;
+                        }
+                    }
                 }
                 pos = (pos + 1) & (GREP_HT - 1);
             }
