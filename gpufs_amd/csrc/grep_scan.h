@@ -7,22 +7,27 @@
 // mask & ~(mask << 1 | previous byte).  A token is skipped at once when no
 // query word has its length; otherwise its first eight bytes (a funnel shift
 // of two aligned LDS words, bytes past the token zeroed) and its length are
-// hashed with 32-bit arithmetic into an LDS table whose entries carry the
-// word's length, and only a slot with the same length and the same first
-// eight bytes leads to reading and comparing the token's remaining words.  A
-// token's bytes and a query word's bytes compare equal iff the strings are
-// equal, so the counts are exact.
+// hashed with 32-bit arithmetic.  The hash's top bits index a 2^18-bit
+// membership filter of the dictionary: a token whose bit is clear is no query
+// word (almost all tokens; measured: without the filter, walking the table's
+// probe chains for them took a third of the scan).  Its low bits pick a slot
+// of an LDS table whose entries carry the word's length; only a slot with the
+// same length and the same first eight bytes leads to reading and comparing
+// the token's remaining words.  A token's bytes and a query word's bytes
+// compare equal iff the strings are equal, so the counts are exact.
 #pragma once
 #include "gpufs.h"
 
 #define GREP_MAX_CHUNK 65536
 #define GREP_MAX_WORDS 1024
 #define GREP_HT 4096
+#define GREP_FILT_LOG 18   // membership filter bits (32 KiB of LDS)
 
 struct grep_lds {
     uint8_t corpus[GREP_MAX_CHUNK + 128];    // window + look-ahead + NUL + read padding
     uint64_t words[GREP_MAX_WORDS][4];       // query bytes, zero-padded, little-endian u64
     uint32_t ht[GREP_HT];                    // (canonical word index + 1) | length << 16; 0 = empty
+    uint32_t filt[(1u << GREP_FILT_LOG) / 32];   // bit (hash >> (32 - GREP_FILT_LOG)) set for each query word
     uint32_t counts[GREP_MAX_WORDS];
     uint16_t canon[GREP_MAX_WORDS];          // 0xFFFF: word can never match
     uint8_t wlen[GREP_MAX_WORDS];
@@ -73,6 +78,7 @@ __device__ __forceinline__ uint64_t keep_bytes(uint64_t v, int n) {
 __device__ inline void grep_build_dict(grep_lds& S, const char* dict, uint32_t nw) {
     const uint32_t T = gfs::nthreads(), t = gfs::tid();
     for (uint32_t i = t; i < GREP_HT; i += T) S.ht[i] = 0;
+    for (uint32_t i = t; i < (1u << GREP_FILT_LOG) / 32; i += T) S.filt[i] = 0;
     if (t == 0) S.lenmask = 0;
     for (uint32_t q = t; q < nw; q += T) {
         uint32_t len = 0;
@@ -102,7 +108,10 @@ __device__ inline void grep_build_dict(grep_lds& S, const char* dict, uint32_t n
         if (S.canon[q] == 0xFFFF) continue;
         const uint32_t len = S.wlen[q];
         atomicOr(&S.lenmask, 1u << (len - 1));
-        uint32_t pos = tok_hash8(S.words[q][0], len) & (GREP_HT - 1);
+        const uint32_t h = tok_hash8(S.words[q][0], len);
+        const uint32_t fi = h >> (32 - GREP_FILT_LOG);
+        atomicOr(&S.filt[fi >> 5], 1u << (fi & 31));
+        uint32_t pos = h & (GREP_HT - 1);
         for (;;) {
             uint32_t old = atomicCAS(&S.ht[pos], 0u, (q + 1) | (len << 16));
             if (old == 0) break;
@@ -160,7 +169,10 @@ __device__ inline void grep_scan_window(grep_lds& S, uint32_t dsz) {
             }
                 if (len > 32 || !((lm >> (len - 1)) & 1u)) continue;
             const uint64_t t0 = keep_bytes(lds_u64_at(S.corpus, p), (int)len);
-            uint32_t pos = tok_hash8(t0, len) & (GREP_HT - 1);
+            const uint32_t h = tok_hash8(t0, len);
+            const uint32_t fi = h >> (32 - GREP_FILT_LOG);
+            if (!((S.filt[fi >> 5] >> (fi & 31)) & 1u)) continue;
+            uint32_t pos = h & (GREP_HT - 1);
             for (;;) {
                 const uint32_t e = S.ht[pos];
                 if (!e) break;
