@@ -302,9 +302,14 @@ __device__ inline uint64_t node_alloc(gfs_dev* g) {
 
 // Returns the node index of the leaf covering pages [leaf*64, leaf*64+64), creating
 // interior nodes on the way when `create`.  0 = absent (or -ENOMEM in *err).
-__device__ inline uint64_t tree_leaf(gfs_dev* g, gfs_file* f, uint64_t leaf, bool create, int* err) {
+// `root_hint` (nonzero) is a root word read earlier by the caller: a tree only
+// grows by pushing the old root down as child 0, so an older root still holds
+// every page it covered; one too small for `leaf` fails the growth CAS below
+// and the current word is used.
+__device__ inline uint64_t tree_leaf(gfs_dev* g, gfs_file* f, uint64_t leaf, bool create, int* err,
+                                     uint64_t root_hint = 0) {
     const uint64_t page = leaf << GFS_FANOUT_LOG;
-    uint64_t rw = ld_acq(&f->root);
+    uint64_t rw = root_hint ? root_hint : ld_acq(&f->root);
     for (int guard = 0; guard < 16; ++guard) {
         uint32_t h = (uint32_t)(rw >> 40);
         if (rw != 0 && page < tree_cap(h)) break;
@@ -679,7 +684,7 @@ __device__ inline bool wg_timed_out(gfs_dev* g, uint64_t t0) {
 // need host data; they come back OWN with slot[i] = 0xFFFFFFFF.
 // ======================================================================
 __device__ inline void resolve_batch(gfs_dev* g, int fd, uint64_t pb, uint32_t cnt, uint64_t pending,
-                                     uint64_t nohost_mask, bool for_write) {
+                                     uint64_t nohost_mask, bool for_write, uint64_t root_hint = 0) {
     const uint32_t ln = lane();
     gfs_file* file = &g->files[fd];
     const bool mine = ((pending >> ln) & 1ull) != 0 && ln < cnt;
@@ -689,8 +694,8 @@ __device__ inline void resolve_batch(gfs_dev* g, int fd, uint64_t pb, uint32_t c
     uint64_t n0 = 0, n1 = 0;
     int err = 0;
     if (ln == 0) {
-        n0 = tree_leaf(g, file, leaf0, true, &err);
-        n1 = (leaf1 == leaf0) ? n0 : tree_leaf(g, file, leaf1, true, &err);
+        n0 = tree_leaf(g, file, leaf0, true, &err, root_hint);
+        n1 = (leaf1 == leaf0) ? n0 : tree_leaf(g, file, leaf1, true, &err, root_hint);
     }
     n0 = __shfl(n0, 0);
     n1 = __shfl(n1, 0);
@@ -713,7 +718,7 @@ __device__ inline void resolve_batch(gfs_dev* g, int fd, uint64_t pb, uint32_t c
             if (meta_gen(old) == vg && meta_state(old) == GFS_F_VALID) {
                 kind = GFS_K_HIT;
                 f = vf;
-                if (!g->fref[vf]) g->fref[vf] = 1;
+                g->fref[vf] = 1;   // CLOCK reference bit: a store, no load to wait for
                 if (for_write) __hip_atomic_fetch_or(&g->meta[vf], GFS_FL_DIRTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else {
                 fadd_rlx(&g->meta[vf], ~0ull);
@@ -921,24 +926,35 @@ __device__ inline size_t fstat(int fd) {
 
 // libgpufs/fs_calls.cu:689-737.  Reads [offset, offset+size) clipped at end of
 // file into `buffer` (global or LDS); returns the bytes read.
-__device__ __attribute__((noinline)) size_t gread(int fd, size_t offset, size_t size, uchar* buffer) {
+__device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer) {
     using namespace gfs;
     gfs_dev* g = gfs_g;
     __syncthreads();
     GFS_TRACE(5);
     if (tid() == 0) {
+        // state, size and radix root in one round trip (relaxed loads, one
+        // acquire fence); the caller's gopen already acquired what the opener
+        // published
         int64_t r = 0;
-        if (!fd_valid(g, fd) || fst_state(ld_acq(&g->files[fd].st)) != GFS_FS_OPEN) r = -EBADF;
+        uint64_t fs = 0, root = 0;
+        if (!fd_valid(g, fd)) r = -EBADF;
         else {
-            uint64_t fs = g->files[fd].size;
-            r = offset >= fs ? 0 : (int64_t)(size < fs - offset ? size : fs - offset);
+            gfs_file* e = &g->files[fd];
+            const uint64_t st = ld_rlx(&e->st);
+            fs = e->size;
+            root = ld_rlx(&e->root);
+            fence_acq_agent();   // the nodes under `root` are read after it
+            if (fst_state(st) != GFS_FS_OPEN) r = -EBADF;
+            else r = offset >= fs ? 0 : (int64_t)(size < fs - offset ? size : fs - offset);
         }
         gfs_ctl.ret = r;
-        gfs_ctl.a = r > 0 ? g->files[fd].size : 0;
+        gfs_ctl.a = r > 0 ? fs : 0;
+        gfs_ctl.b = r > 0 ? root : 0;
     }
     __syncthreads();
     const int64_t n = uni(gfs_ctl.ret);
     const uint64_t fsize = uni(gfs_ctl.a);
+    const uint64_t root_hint = uni(gfs_ctl.b);
     if (n <= 0) return (size_t)n;
     const uint32_t L = g->log_page;
     const uint64_t PS = g->page_size;
@@ -955,7 +971,7 @@ __device__ __attribute__((noinline)) size_t gread(int fd, size_t offset, size_t 
             __syncthreads();
             if (wave() == 0) {
                 if (lane() == 0) gfs_ctl.err = 0;
-                resolve_batch(g, fd, pb, cnt, pending, 0ull, false);
+                resolve_batch(g, fd, pb, cnt, pending, 0ull, false, root_hint);
             }
             __syncthreads();
             if (uni(gfs_ctl.err)) { err = uni(gfs_ctl.err); break; }
