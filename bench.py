@@ -56,6 +56,10 @@ def parse():
                     help="run the batched (persistent-kernel) leg; off by default in this build")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--batched-files", type=int, default=10000)
+    ap.add_argument("--gemv", action="store_true",
+                    help="also run the gmmap GEMV leg (65536 x 131072 fp32 = 32 GiB matrix file, 8 GiB cache)")
+    ap.add_argument("--gemv-rows", type=int, default=65536)
+    ap.add_argument("--gemv-page-kib", type=int, default=512)
     ap.add_argument("--serve-files", type=int, default=100000)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-rates", default="1000,10000,50000")
@@ -259,6 +263,7 @@ def main():
     del dst
     torch.cuda.empty_cache()
     batched_res = bench_batched(args, torch, L, _lib, dg) if args.batched else None
+    gemv_res = bench_gemv(args, torch, L, _lib, dg) if args.gemv else None
     serve_res = bench_serve(args, torch, L, _lib, dg) if not args.no_serve else None
     line = {
         "metric": METRIC,
@@ -288,6 +293,7 @@ def main():
         "cpu_baseline": cpu,
         "grep": grep_res,
         "batched": batched_res,
+        "gemv": gemv_res,
         "served": serve_res,
     }
     print(json.dumps(line), flush=True)
@@ -465,6 +471,74 @@ def bench_batched(args, torch, L, _lib, dg):
     log(f"batched: {res['batched_cold_files_per_s']} files/s cold, plain {res['plain_cold_files_per_s']} files/s, "
         f"x{res['speedup_vs_plain']}")
     return res
+
+
+def bench_gemv(args, torch, L, _lib, dg):
+    """y = A x with A (rows x 131072 fp32, row-major) gmmapped page by page
+    through an 8 GiB cache: the matrix does not fit, so the swapper evicts
+    continuously and every page crosses the host link."""
+    rows, cols, rpw = args.gemv_rows, 131072, 64
+    size = rows * cols * 4
+    path = os.path.join(args.workdir, f"matrix_{rows}x{cols}.f32")
+    if not os.path.exists(path) or os.path.getsize(path) != size:
+        t = time.time()
+        dg.matrix_file(path, rows, cols, dg.SEED_C2, threads=16)
+        log(f"generated {path} ({size / 2**30:.1f} GiB) in {time.time() - t:.1f}s")
+    _warm_host_cache([path])
+    x = dg.normal_f32(cols, dg.SEED_C2 + 1)
+    cache = 8 << 30
+    ctx = _lib.Context(cache_bytes=cache, page_size=args.gemv_page_kib << 10, ring_depth=1024,
+                       host_threads=args.host_threads, max_files=64)
+    ctx.host_start()
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    name = torch.zeros(256, dtype=torch.uint8)
+    b = os.fsencode(path)
+    name[:len(b)] = torch.frombuffer(bytearray(b), dtype=torch.uint8)
+    name = name.cuda()
+    xd = torch.from_numpy(x).cuda()
+    nwg = (rows + rpw - 1) // rpw
+    y = torch.zeros(rows, dtype=torch.float32, device="cuda")
+    st = torch.zeros(nwg, dtype=torch.int32, device="cuda")
+    cs = torch.zeros(nwg, dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()
+    times = []
+    try:
+        for _ in range(2):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            _lib.check("gpufs_launch_gemv_mmap", L.gpufs_launch_gemv_mmap(
+                ctx.h, ctypes.c_void_p(name.data_ptr()), rows, cols, ctypes.c_void_p(xd.data_ptr()),
+                ctypes.c_void_p(y.data_ptr()), rpw, ctypes.c_void_p(st.data_ptr()), ctypes.c_void_p(cs.data_ptr()),
+                1024, None))
+            e1.record(stream)
+            e1.synchronize()
+            times.append(e0.elapsed_time(e1) / 1e3)
+            assert (st.cpu().numpy() == 0).all()
+            log(f"gemv pass: {size / times[-1] / 1e9:.1f} GB/s ({times[-1] * 1e3:.1f} ms)")
+        s = ctx.stats()
+    finally:
+        ctx.host_stop()
+        ctx.close()
+    out = {"rows": rows, "cols": cols, "matrix_bytes": size, "cache_bytes": cache,
+           "page_bytes": args.gemv_page_kib << 10, "workgroups": nwg, "rows_per_workgroup": rpw,
+           "GBps_each": [round(size / t / 1e9, 2) for t in times], "evictions": s["evictions"]}
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        ncores = min(16, os.cpu_count() or 1)
+        y64, cs64 = orc.gemv_file(path, rows, cols, x, rpw, f64=True, threads=ncores)
+        yg = y.cpu().numpy().astype(np.float64)
+        err = float(np.abs(yg - y64).max() / np.abs(y64).max())
+        out["bytes_exact"] = bool((cs.cpu().numpy().view(np.uint64) == cs64).all())
+        out["rel_err_max_over_max"] = err
+        assert out["bytes_exact"], "mapped bytes differ from the file"
+        assert err <= 1e-5, err
+        samp = min(rows, 4096)
+        t = time.perf_counter()
+        orc.gemv_file(path, samp, cols, x, rpw, f64=False, threads=ncores)
+        dt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": round(samp * cols * 4 / dt / 1e9, 3), "unit": "GB/s", "cores": ncores,
+                               "kind": "port", "sample": f"first {samp} rows: pread + fp32 GEMV, {rpw} rows per task"}
+    return out
 
 
 def bench_serve(args, torch, L, _lib, dg):

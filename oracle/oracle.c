@@ -21,6 +21,8 @@
  *                    page), optional FNV-1a-64 per range, N threads.
  *   orc_gemv_f64     y = A x accumulated in double (the error reference for the
  *                    mapped-matrix GEMV, samples/bigmatrix_mmap/bigmatrix_mmap.cu:24-41).
+ *   orc_gemv_file    the same over a matrix file: threaded pread of row blocks,
+ *                    y (double or float accumulation) and per-block word sums.
  *   orc_or_merge     the O_GWRONCE write-back merge, libgpufs/host_loop.h:76-137.
  */
 #define _GNU_SOURCE
@@ -382,6 +384,76 @@ void orc_gemv_f64(const float* A, uint64_t rows, uint64_t cols, const float* x, 
         for (uint64_t c = 0; c < cols; c++) s += (double)a[c] * (double)x[c];
         y[r] = s;
     }
+}
+
+/* CPU path of the mapped-matrix GEMV over a file (samples/bigmatrix_mmap:
+ * bigmatrix_mmap.cu:66-168 reads rows page by page; bigmatrix_cpu.cpp:144-159
+ * is the reference's own CPU GEMV).  Rows are pread() in blocks of
+ * `rows_per_block`; each block gets y for its rows (accumulated in double when
+ * `f64`, else in float in column order) and the wrapping sum of the block's
+ * 8-byte words (the GPU kernel's per-workgroup checksum of the mapped bytes).
+ * N threads take blocks in turn.  Returns 0 or -errno. */
+struct gemv_job {
+    int fd;
+    uint64_t rows, cols, rpb, nblocks;
+    const float* x;
+    double* y;
+    uint64_t* csum;
+    int f64;
+    atomic_uint_fast64_t next;
+    atomic_int err;
+};
+static void* gemv_worker(void* arg) {
+    struct gemv_job* j = (struct gemv_job*)arg;
+    const uint64_t rb = j->cols * 4;
+    float* buf = (float*)malloc(rb);
+    if (!buf) { atomic_store(&j->err, -ENOMEM); return NULL; }
+    for (;;) {
+        uint64_t b = atomic_fetch_add(&j->next, 1);
+        if (b >= j->nblocks) break;
+        uint64_t cs = 0;
+        for (uint64_t r = b * j->rpb; r < (b + 1) * j->rpb && r < j->rows; r++) {
+            uint64_t done = 0;
+            while (done < rb) {
+                ssize_t n = pread(j->fd, (char*)buf + done, rb - done, (off_t)(r * rb + done));
+                if (n <= 0) { atomic_store(&j->err, n < 0 ? -errno : -EIO); free(buf); return NULL; }
+                done += (uint64_t)n;
+            }
+            if (j->f64) {
+                double s = 0.0;
+                for (uint64_t c = 0; c < j->cols; c++) s += (double)buf[c] * (double)j->x[c];
+                j->y[r] = s;
+            } else {
+                float s = 0.0f;
+                for (uint64_t c = 0; c < j->cols; c++) s += buf[c] * j->x[c];
+                j->y[r] = (double)s;
+            }
+            const uint64_t* w = (const uint64_t*)buf;
+            for (uint64_t k = 0; k < rb / 8; k++) cs += w[k];
+        }
+        j->csum[b] = cs;
+    }
+    free(buf);
+    return NULL;
+}
+int orc_gemv_file(const char* path, uint64_t rows, uint64_t cols, const float* x, uint64_t rows_per_block,
+                  double* y, uint64_t* csum, int f64, int threads) {
+    if (rows_per_block == 0 || (cols * 4) % 8) return -EINVAL;
+    int fd = open(path, O_RDONLY);
+    if (fd < 0) return -errno;
+    struct gemv_job j;
+    j.fd = fd; j.rows = rows; j.cols = cols; j.rpb = rows_per_block;
+    j.nblocks = (rows + rows_per_block - 1) / rows_per_block;
+    j.x = x; j.y = y; j.csum = csum; j.f64 = f64;
+    atomic_init(&j.next, 0);
+    atomic_init(&j.err, 0);
+    if (threads < 1) threads = 1;
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)threads);
+    for (int i = 0; i < threads; i++) pthread_create(&th[i], NULL, gemv_worker, &j);
+    for (int i = 0; i < threads; i++) pthread_join(th[i], NULL);
+    free(th);
+    close(fd);
+    return atomic_load(&j.err);
 }
 
 /* ---------------------------------------------------------------- write merge */

@@ -28,6 +28,8 @@ def lib():
         L.orc_fnv1a64.restype = u64
         L.orc_pread_ranges.argtypes = [ctypes.c_char_p, u32, u64, P, P, P, i32]
         L.orc_gemv_f64.argtypes = [P, u64, u64, P, P]
+        L.orc_gemv_file.argtypes = [ctypes.c_char_p, u64, u64, P, u64, P, P, i32, i32]
+        L.orc_gemv_file.restype = i32
         L.orc_or_merge.argtypes = [P, P, u64]
         for f in (L.orc_scan_naive, L.orc_scan_fast, L.orc_scan_files, L.orc_pread_ranges):
             f.restype = i32
@@ -99,6 +101,19 @@ def gemv_f64(A, x):
     y = np.zeros(A.shape[0], dtype=np.float64)
     lib().orc_gemv_f64(_ptr(A), A.shape[0], A.shape[1], _ptr(x), _ptr(y))
     return y
+
+
+def gemv_file(path, rows, cols, x, rows_per_block, f64=True, threads=1):
+    """(y float64[rows], per-block wrapping uint64 sums of the block's 8-byte words)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros(rows, dtype=np.float64)
+    nb = (rows + rows_per_block - 1) // rows_per_block
+    cs = np.zeros(nb, dtype=np.uint64)
+    rc = lib().orc_gemv_file(os.fsencode(path), rows, cols, _ptr(x), rows_per_block, _ptr(y), _ptr(cs),
+                             1 if f64 else 0, threads)
+    if rc != 0:
+        raise OSError(-rc, f"orc_gemv_file({path})")
+    return y, cs
 
 
 def or_merge(page, disk):

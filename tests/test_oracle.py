@@ -183,6 +183,34 @@ def test_gemv_f64_against_numpy(orc):
     assert (orc.gemv_f64(eye, x[:16]) == x[:16].astype(np.float64)).all()
 
 
+def test_gemv_file_against_numpy(orc, tmp_path):
+    """The file GEMV: y equals numpy's float64 product, the fp32 mode stays
+    within float32 rounding, and each block's checksum is the wrapping sum of
+    the block's 8-byte words (ragged last block included)."""
+    rng = np.random.default_rng(1)
+    rows, cols, rpb = 37, 96, 8
+    A = rng.standard_normal((rows, cols)).astype(np.float32)
+    x = rng.standard_normal(cols).astype(np.float32)
+    p = str(tmp_path / "m.bin")
+    A.tofile(p)
+    y, cs = orc.gemv_file(p, rows, cols, x, rpb, f64=True, threads=3)
+    ref = A.astype(np.float64) @ x.astype(np.float64)
+    assert np.allclose(y, ref, rtol=1e-12, atol=1e-12)
+    words = A.view(np.uint64).reshape(rows, cols // 2)
+    for b in range(len(cs)):
+        exp = np.sum(words[b * rpb:(b + 1) * rpb], dtype=np.uint64)
+        assert cs[b] == exp, b
+    y32, cs32 = orc.gemv_file(p, rows, cols, x, rpb, f64=False, threads=2)
+    assert (cs32 == cs).all()
+    scale = np.abs(A).astype(np.float64) @ np.abs(x).astype(np.float64)
+    assert (np.abs(y32 - ref) <= 1e-5 * scale).all()
+    # identity rows: y is x exactly
+    eye = np.eye(cols, dtype=np.float32)
+    eye.tofile(p)
+    ye, _ = orc.gemv_file(p, cols, cols, x, 5, f64=True, threads=4)
+    assert (ye == x.astype(np.float64)).all()
+
+
 def test_or_merge_known_answer(orc):
     page = np.array([0x01, 0x10, 0x00, 0xF0], dtype=np.uint8)
     disk = np.array([0x02, 0x01, 0x80], dtype=np.uint8)
