@@ -21,9 +21,12 @@
 //    second-chance reference bit.
 //  * host RPC: slots claimed from a device bitmap, requests published into a
 //    submission queue in coherent pinned host memory with system-scope release
-//    stores; the host service pread()s into pinned staging and bumps a per-batch
-//    completion counter; the workgroup copies staging into HBM frames with
-//    16-byte loads and stores.
+//    stores; the host service pread()s into pinned staging and sets the
+//    request's bit in its batch's 64-bit completion mask; the workgroup copies
+//    each page into its HBM frame with 16-byte loads and stores as soon as it
+//    has arrived and returns its slot.
+//  * file table: open/close by atomic adds on a per-file state word (fstate.h),
+//    no lock.
 //
 // Workgroups must have at least 64 threads (one full wave64): wave 0 resolves
 // up to 64 pages at a time, one page per lane.
