@@ -29,7 +29,8 @@
 #define E_FSTABLE_FULL (-1)
 #define E_IPC_OPEN_ERROR (-2)
 
-#define GFS_NAME_MAX 128          // bytes incl. NUL (reference: FILENAME_SIZE 64)
+#define GFS_NAME_MAX 128
+#define GFS_NAME_RESERVED 0xFFFFFFFFu   // name-index slot taken, entry not yet published          // bytes incl. NUL (reference: FILENAME_SIZE 64)
 #define GFS_FANOUT_LOG 6          // radix fan-out 64 = one wave64 lane per slot
 #define GFS_FANOUT 64
 #define GFS_MAX_HEIGHT 7          // 64^7 pages
@@ -50,7 +51,6 @@
 #define GFS_FS_OPEN    2u
 #define GFS_FS_CLOSING 3u
 #define GFS_FS_CLOSED  4u
-#define GFS_FS_DEAD    5u   // entry lost a name-index race; never used
 
 // ---- tuning switches
 #define GFS_TUNE_PAGE_SYNC 1u   // gread: workgroup barrier after each page copied to the caller

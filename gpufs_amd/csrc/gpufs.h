@@ -620,38 +620,50 @@ __device__ inline int open_single(const char* name, int flags) {
     if (nm.len >= GFS_NAME_MAX) return -ENAMETOOLONG;
     const uint32_t h = nm.hash;
     const uint32_t mask = g->name_cap - 1;
+    const uint64_t t0 = now_ticks();
     uint32_t pos = h & mask;
     int idx = -1;
-    int mine = -1;
     for (uint32_t probe = 0; probe <= mask; ++probe, pos = (pos + 1) & mask) {
         uint32_t v = ld_acq(&g->name_index[pos]);
         if (v == 0) {
-            if (mine < 0) {
-                unsigned long long m = fadd_rlx(&g->files_next, 1ull);
-                if (m >= g->max_files) return E_FSTABLE_FULL;
-                mine = (int)m;
-                gfs_file* e = &g->files[mine];
+            // reserve the slot first, so only the workgroup that publishes
+            // the name takes a file-table entry
+            uint32_t exp = 0;
+            if (cas(&g->name_index[pos], exp, GFS_NAME_RESERVED)) {
+                const unsigned long long m = fadd_rlx(&g->files_next, 1ull);
+                if (m >= g->max_files) {
+                    st_rel(&g->name_index[pos], 0u);
+                    return E_FSTABLE_FULL;
+                }
+                gfs_file* e = &g->files[m];
                 uint64_t* en = (uint64_t*)e->name;
                 for (int i = 0; i < GFS_NAME_MAX / 8; ++i) en[i] = nm.w[i];
                 e->name_hash = h;
                 e->root = 0;
                 e->host_fd = -1;
-                st_rlx(&e->st, fst_make(GFS_FS_OPENING, 1, 0));   // not yet published
-            }
-            uint32_t exp = 0;
-            if (cas(&g->name_index[pos], exp, (uint32_t)mine + 1u)) {
-                int r = open_host(g, &g->files[mine], flags, false);
-                return r < 0 ? E_IPC_OPEN_ERROR : mine;
+                st_rlx(&e->st, fst_make(GFS_FS_OPENING, 1, 0));
+                st_rel(&g->name_index[pos], (uint32_t)m + 1u);   // publish the entry
+                int r = open_host(g, e, flags, false);
+                return r < 0 ? E_IPC_OPEN_ERROR : (int)m;
             }
             v = exp;
+        }
+        while (v == GFS_NAME_RESERVED) {   // another workgroup is publishing this slot
+            if (now_ticks() - t0 > g->wait_ticks) return -ETIMEDOUT;
+            backoff();
+            v = ld_acq(&g->name_index[pos]);
+        }
+        if (v == 0) {   // the reservation was given up (table full): look at the slot again
+            --probe;
+            pos = (pos - 1) & mask;
+            continue;
         }
         gfs_file* e = &g->files[v - 1];
         if (e->name_hash == h && name_eq(e, nm)) { idx = (int)v - 1; break; }
     }
-    if (mine >= 0 && idx != mine) st_rel(&g->files[mine].st, fst_make(GFS_FS_DEAD, 0, 0));
     if (idx < 0) return E_FSTABLE_FULL;
     gfs_file* e = &g->files[idx];
-    fst_dev P{now_ticks(), g->wait_ticks};
+    fst_dev P{t0, g->wait_ticks};
     const int r = fst_open(&e->st, P);
     if (r == FST_JOINED) return idx;
     if (r < 0) return r;
