@@ -2,6 +2,7 @@
 
   gpufs_amd/libgpufs_amd.so     HIP device API kernels + host runtime (hipcc, gfx950)
   gpufs_amd/libgpufs_datagen.so synthetic workload generator (g++)
+  gpufs_amd/examples/compat_copy reference-style driver built against gpufs_compat.h
 
 Run as `python -m gpufs_amd.build` or through __graft_entry__.build().
 """
@@ -52,6 +53,17 @@ def build_hip(jobs=8):
     return lib
 
 
+def build_examples():
+    """gpufs_amd/examples/compat_copy: a reference-style driver (gpufs_compat.h)."""
+    src = os.path.join(HERE, "examples", "compat_copy.hip")
+    exe = os.path.join(HERE, "examples", "compat_copy")
+    tmp = exe + ".tmp"
+    flags = [f for f in HIP_FLAGS if f != "-fPIC"]
+    _run([HIPCC, *flags, "-I", CSRC, src, "-o", tmp, "-L", HERE, "-lgpufs_amd", "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(tmp, exe)
+    return exe
+
+
 def build_datagen():
     lib = os.path.join(HERE, "libgpufs_datagen.so")
     tmp = lib + ".tmp"
@@ -65,7 +77,8 @@ def build_all():
     with cf.ThreadPoolExecutor(max_workers=2) as ex:
         f1 = ex.submit(build_hip)
         f2 = ex.submit(build_datagen)
-        return f1.result(), f2.result()
+        out = (f1.result(), f2.result())
+    return out + (build_examples(),)
 
 
 if __name__ == "__main__":

@@ -64,3 +64,10 @@ def test_null_arguments_are_rejected_without_gpu():
     assert L.gpufs_destroy(None) == -22
     assert L.gpufs_run_handler(None) == -22
     assert L.gpufs_launch_gread_ranges(None, None, 0, 0, None, None, 64, None) == -22
+
+
+def test_compat_example_built():
+    """build() also builds the reference-style driver used by the GPU test."""
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpufs_amd", "examples",
+                       "compat_copy")
+    assert os.access(exe, os.X_OK)
