@@ -76,6 +76,7 @@ SIGNATURES = {
     "gpufs_serve_run": (I, [P, P, P, U32, P, P, P, ctypes.POINTER(ctypes.c_double)]),
     "gpufs_serve_stop": (I, [P]),
     "gpufs_serve_pages": (I, [P, P, P]),
+    "gpufs_launch_mmap_warp_records": (I, [P, P, U64, U64, P, P, U32, U32, P]),
     "gpufs_launch_gemv_mmap": (I, [P, P, U64, U64, P, P, U32, P, P, U32, P]),
     "gpufs_fnv1a64_ranges": (I, [P, U32, U64, P, P]),
     "gpufs_launch_read": (I, [P, P, I32, U64, U64, P, P, U32, P]),

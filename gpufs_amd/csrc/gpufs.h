@@ -1394,6 +1394,176 @@ __device__ inline int gmunmap(volatile void* addr, size_t length) {
     return 0;
 }
 
+
+// Wave-collective mapping (reference gmmap_warp / getRwLockedPage_warp,
+// libgpufs/fs_calls.cu:483-683): all 64 lanes of one wave call with the same
+// arguments, and different waves of a workgroup may map different pages at
+// the same time, so nothing here touches the workgroup's shared control block
+// or waits at a workgroup barrier.  Lane 0 looks the page up and pins it
+// (`ref` pins, dropped by gmunmap_warp), or claims a frame and a ring slot,
+// reads the page from the host and publishes it; values reach the other lanes
+// through v_readfirstlane; the page copy (or zero fill) uses all 64 lanes.
+// Wait limits are decided by lane 0, so the whole wave leaves together.
+__device__ inline volatile void* gmmap_warp(void* addr, size_t size, int prot, int flags, int fd, off_t offset,
+                                           int ref = 1) {
+    using namespace gfs;
+    (void)addr; (void)prot; (void)flags;
+    gfs_dev* g = gfs_g;
+    const uint32_t ln = lane();
+    if (ref < 1) return GMAP_FAILED;
+    int32_t bad = 0, oflags = 0, host_fd = -1;
+    uint64_t fsize = 0;
+    if (ln == 0) {
+        if (!fd_valid(g, fd) || fst_state(ld_acq(&g->files[fd].st)) != GFS_FS_OPEN || offset < 0) bad = -EBADF;
+        else if (((uint64_t)offset & (g->page_size - 1)) + size > g->page_size) bad = -EINVAL;
+        else {
+            fsize = g->files[fd].size;
+            oflags = g->files[fd].flags;
+            host_fd = g->files[fd].host_fd;
+        }
+    }
+    if (uni(bad)) return GMAP_FAILED;
+    fsize = uni(fsize);
+    oflags = uni(oflags);
+    host_fd = uni(host_fd);
+    const uint32_t L = g->log_page;
+    const bool writable = (oflags & O_ACCMODE) != O_RDONLY;
+    const bool wronce = (oflags & (O_CREAT | O_ACCMODE)) == O_GWRONCE;
+    const uint64_t page = (uint64_t)offset >> L;
+    const bool nohost = wronce || (page << L) >= fsize;
+    const uint64_t t0 = now_ticks();
+    uint32_t f = GFS_NOFRAME;
+    for (;;) {
+        // lane 0: leaf slot, pin
+        uint32_t kind = GFS_K_MISS;
+        int32_t err = 0;
+        uint64_t v = 0;
+        uint64_t* sp = nullptr;
+        if (ln == 0) {
+            int e = 0;
+            const uint64_t node = tree_leaf(g, &g->files[fd], page >> GFS_FANOUT_LOG, true, &e);
+            err = e;
+            if (!e) {
+                sp = &g->nodes[node * GFS_FANOUT + (page & (GFS_FANOUT - 1))];
+                v = ld_rlx(sp);
+                if (v) {
+                    const uint32_t vf = leaf_frame(v), vg = leaf_gen(v);
+                    const uint64_t old = fadd_rlx(&g->meta[vf], (uint64_t)ref);
+                    if (meta_gen(old) == vg && meta_state(old) == GFS_F_VALID) {
+                        kind = GFS_K_HIT;
+                        f = vf;
+                        g->fref[vf] = 1;
+                    } else {
+                        fadd_rlx(&g->meta[vf], (uint64_t)0 - (uint64_t)ref);
+                        if (meta_gen(old) == vg && meta_state(old) == GFS_F_FILLING) kind = GFS_K_WAIT;
+                    }
+                }
+            }
+        }
+        if (uni(err)) return GMAP_FAILED;
+        kind = uni(kind);
+        if (kind == GFS_K_HIT) {
+            fence_acq_agent();
+            if (ln == 0) count(&g->ctr.page_hits, 1);
+            f = uni(f);
+            break;
+        }
+        uint32_t late = 0;
+        if (kind == GFS_K_MISS) {
+            // a ring slot for the host read, then a frame
+            int32_t s = -1;
+            if (ln == 0 && !nohost) s = rpc_claim(g, true, blockIdx.x * 5u + wave());
+            s = uni(s);
+            if (nohost || s >= 0) {
+                uint32_t nf = uni(frame_alloc_wave(g, ln == 0, 2));
+                uint32_t own = 0;
+                if (nf != GFS_NOFRAME && ln == 0) {
+                    g->fkey[nf] = gfs_key((uint32_t)fd, page);
+                    g->fbytes[nf] = 0;
+                    const uint32_t gen = meta_gen(ld_rlx(&g->meta[nf]));
+                    uint64_t exp = v;
+                    own = cas(sp, exp, leaf_pack(nf, gen)) ? 1u : 0u;
+                    if (own) g->fref[nf] = 1;
+                    else frame_release(g, nf);
+                }
+                own = uni(own);
+                if (own) {
+                    int64_t r = 0;
+                    if (!nohost) {
+                        const uint64_t poff = page << L;
+                        uint64_t want = fsize > poff ? fsize - poff : 0;
+                        if (want > g->page_size) want = g->page_size;
+                        if (ln == 0) {
+                            sys_st(&g->cq[s], 0ull);
+                            rpc_post(g, s, GFS_OP_READ, host_fd, 0, poff, want, (uint32_t)s);
+                            count(&g->ctr.rpc_reads, 1);
+                            const int w = rpc_wait(g, (uint32_t)s, 1ull);
+                            r = w ? (int64_t)w : sys_ld(&g->slots[s].result);
+                            if (r >= 0 && (uint64_t)r != want) r = -EIO;
+                        }
+                        r = uni(r);
+                        if (r >= 0) copy_seg<true, false>(frame_ptr(g, nf), staging_ptr(g, (uint32_t)s), (uint64_t)r, ln, 64);
+                        // the wave's staging loads have all returned (their data was stored)
+                        if (ln == 0) rpc_release(g, s);
+                    } else {
+                        uint8_t* fp = frame_ptr(g, nf);
+                        for (uint64_t k = ln; k < (g->page_size >> 4); k += 64) *(v4u*)(fp + 16 * k) = v4u{0, 0, 0, 0};
+                    }
+                    if (r < 0) {
+                        if (ln == 0) {   // unpublish and free the frame
+                            uint64_t exp = leaf_pack(nf, meta_gen(ld_rlx(&g->meta[nf])));
+                            cas(sp, exp, 0ull);
+                            frame_release(g, nf);
+                        }
+                        return GMAP_FAILED;
+                    }
+                    __threadfence();   // frame bytes before VALID
+                    if (ln == 0) {
+                        g->fbytes[nf] = (uint32_t)r;
+                        count(&g->ctr.page_misses, 1);
+                        count(&g->ctr.bytes_filled, (unsigned long long)r);
+                        // FILLING -> VALID, the allocation's pin becomes `ref` pins
+                        fadd(&g->meta[nf], (GFS_F_VALID - GFS_F_FILLING) * (1ull << GFS_ST_SHIFT) + (uint64_t)(ref - 1));
+                    }
+                    f = nf;
+                    break;
+                }
+                if (ln == 0 && s >= 0) rpc_release(g, s);
+            }
+        } else if (ln == 0) {
+            count(&g->ctr.page_waits, 1);
+        }
+        if (ln == 0) late = (now_ticks() - t0 > g->wait_ticks) ? 1u : 0u;
+        if (uni(late)) {
+            if (ln == 0) count(&g->ctr.timeouts, 1);
+            return GMAP_FAILED;
+        }
+        backoff();
+    }
+    if (writable && ln == 0) {
+        __hip_atomic_fetch_or(&g->meta[f], GFS_FL_DIRTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        atomicMax(&g->fbytes[f], (uint32_t)(((uint64_t)offset & (g->page_size - 1)) + size));
+        g->files[fd].dirty = 1;
+        atomicMax((unsigned long long*)&g->files[fd].size, (unsigned long long)(offset + size));
+    }
+    return (volatile void*)uni((uint64_t)(uintptr_t)(frame_ptr(g, f) + ((uint64_t)offset & (g->page_size - 1))));
+}
+
+// Wave-collective unmap (reference gmunmap_warp, libgpufs/fs_calls.cu.h:45):
+// drops `ref` pins of the frame holding `addr` (any address inside the mapped
+// page); -1 if `addr` is not inside the page pool.
+__device__ inline int gmunmap_warp(volatile void* addr, size_t length, int ref = 1) {
+    using namespace gfs;
+    (void)length;
+    gfs_dev* g = gfs_g;
+    const uint8_t* a = (const uint8_t*)(uintptr_t)uni((uint64_t)(uintptr_t)addr);
+    const uint64_t f = (uint64_t)(a - g->pool) >> g->log_page;
+    if (a < g->pool || f >= g->n_frames || ref < 1) return -1;
+    __threadfence();   // the wave's accesses to the page come before the unpin
+    if (lane() == 0) fadd(&g->meta[f], (uint64_t)0 - (uint64_t)ref);
+    return 0;
+}
+
 // libgpufs/fs_calls.cu:804-843.  Writes back the page holding `addr` now.
 __device__ inline int gmsync(volatile void* addr, size_t length, int flags) {
     using namespace gfs;

@@ -106,3 +106,56 @@ extern "C" int gpufs_launch_gemv_mmap(gpufs_ctx* ctx, const char* path_dev, uint
                        rows_per_wg, status_dev, (unsigned long long*)checksum_dev);
     return hipGetLastError() == hipSuccess ? 0 : -EIO;
 }
+
+// Record checksums through wave-level mappings, the access pattern of
+// samples/mosaic/mosaic_gpufs.cu:612-651 (each warp maps one fixed-size record
+// with gmmap_warp, reads it and unmaps it; waves of a workgroup work on
+// different records at the same time and may meet on one page).  Record r
+// covers bytes [r*R, r*R+R) of the file and is handled by the wave whose
+// global index is r modulo the number of waves; its 64-bit wrapping sum of
+// 8-byte words goes to sums[r], 0 or a negative code to status[r].
+__global__ __launch_bounds__(1024) void k_mmap_warp_records(const char* path, uint64_t rec, uint64_t nrec,
+                                                            unsigned long long* sums, int32_t* status) {
+    const uint32_t ln = gfs::lane(), w = gfs::wave(), nw = (gfs::nthreads() + 63) >> 6;
+    const uint64_t gw = (uint64_t)blockIdx.x * nw + w, tw = (uint64_t)gridDim.x * nw;
+    int fd = gopen(path, O_GRDONLY);
+    if (fd < 0) {
+        for (uint64_t r = gw; r < nrec; r += tw)
+            if (ln == 0) status[r] = fd;
+        return;
+    }
+    for (uint64_t r = gw; r < nrec; r += tw) {
+        volatile unsigned long long* m =
+            (volatile unsigned long long*)gmmap_warp(nullptr, rec, 0, O_GRDONLY, fd, (off_t)(r * rec));
+        if (m == (volatile unsigned long long*)GMAP_FAILED) {
+            if (ln == 0) status[r] = -EIO;
+            continue;
+        }
+        unsigned long long acc = 0;
+        for (uint64_t k = ln; k < rec / 8; k += 64) acc += m[k];
+        acc = wave_sum_u64(acc);
+        gmunmap_warp((volatile void*)(m + ln), rec);   // any address inside the page
+        if (ln == 0) {
+            sums[r] = acc;
+            status[r] = 0;
+        }
+    }
+    gclose(fd);
+}
+
+extern "C" int gpufs_launch_mmap_warp_records(gpufs_ctx* ctx, const char* path_dev, uint64_t record_bytes,
+                                              uint64_t n_records, uint64_t* sums_dev, int32_t* status_dev,
+                                              uint32_t workgroups, uint32_t threads, void* stream) {
+    if (!ctx || !path_dev || !sums_dev || !status_dev || n_records == 0 || workgroups == 0) return -EINVAL;
+    if (threads < 64 || threads > 1024 || threads % 64) return -EINVAL;
+    gpufs_config cc;
+    gpufs_get_config(ctx, &cc);
+    if (record_bytes == 0 || record_bytes % 8 || record_bytes > cc.page_size || cc.page_size % record_bytes)
+        return -EINVAL;
+    int v = gpufs_bind(ctx);
+    if (v) return v;
+    hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)gpufs_stream(ctx);
+    hipLaunchKernelGGL(k_mmap_warp_records, dim3(workgroups), dim3(threads), 0, s, path_dev, record_bytes, n_records,
+                       (unsigned long long*)sums_dev, status_dev);
+    return hipGetLastError() == hipSuccess ? 0 : -EIO;
+}

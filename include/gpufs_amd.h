@@ -184,6 +184,16 @@ int gpufs_serve_stop(gpufs_server* srv);
  * beside workgroups that hold every compute unit. */
 int gpufs_serve_pages(gpufs_server* srv, uint64_t* page_hits, uint64_t* page_misses);
 
+/* Record checksums through the wave-level mapping gmmap_warp / gmunmap_warp
+ * (samples/mosaic/mosaic_gpufs.cu:612-651 pattern; device API
+ * libgpufs/fs_calls.cu:483-683): every wave maps fixed-size records
+ * (record_bytes divides the page size, multiple of 8) and stores the 64-bit
+ * wrapping sum of each record's 8-byte words in sums_dev[r]; status_dev[r]
+ * = 0 or a negative code.  Waves of one workgroup map records concurrently. */
+int gpufs_launch_mmap_warp_records(gpufs_ctx* ctx, const char* path_dev, uint64_t record_bytes,
+                                   uint64_t n_records, uint64_t* sums_dev, int32_t* status_dev,
+                                   uint32_t workgroups, uint32_t threads, void* stream);
+
 /* y = A x with A (rows x cols fp32, row-major) gmmap()ed page by page from
  * `path_dev` (samples/bigmatrix_mmap/bigmatrix_mmap.cu:66-168 pattern): workgroup
  * w owns rows [w*rows_per_wg, (w+1)*rows_per_wg).  x_dev has cols floats in
