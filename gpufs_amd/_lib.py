@@ -77,6 +77,7 @@ SIGNATURES = {
     "gpufs_serve_stop": (I, [P]),
     "gpufs_serve_pages": (I, [P, P, P]),
     "gpufs_launch_mmap_warp_records": (I, [P, P, U64, U64, P, P, U32, U32, P]),
+    "gpufs_grep_text": (I, [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, U32, U32, U32, P, P]),
     "gpufs_launch_gemv_mmap": (I, [P, P, U64, U64, P, P, U32, P, P, U32, P]),
     "gpufs_fnv1a64_ranges": (I, [P, U32, U64, P, P]),
     "gpufs_launch_read": (I, [P, P, I32, U64, U64, P, P, U32, P]),

@@ -194,16 +194,14 @@ __device__ inline void grep_scan_window(grep_lds& S, uint32_t dsz) {
     }
 }
 
-// One file, the whole chunk loop; S.counts is zeroed here.  Returns 0 or a
-// negative code (workgroup-uniform).
-__device__ inline int grep_one_file(grep_lds& S, const char* name, uint32_t nw, uint32_t chunk, uint32_t* row) {
-    const uint32_t T = gfs::nthreads(), t = gfs::tid();
-    for (uint32_t q = t; q < nw; q += T) S.counts[q] = 0;
-    int fd = gopen(name, O_GRDONLY);
-    if (fd < 0) {
-        for (uint32_t q = t; q < nw; q += T) row[q] = 0;
-        return fd;
-    }
+// The chunk loop of one open file (grep_text_gpu.cu:298-318): windows of
+// `chunk` bytes plus the look-ahead that completes a straddling word, each
+// read with gread into LDS and scanned into S.counts; after every window all
+// threads call on_window() (workgroup-collective).  Returns 0 or a negative
+// code (workgroup-uniform).
+template <class OnWindow>
+__device__ inline int grep_file_windows(grep_lds& S, int fd, uint32_t chunk, OnWindow&& on_window) {
+    const uint32_t t = gfs::tid();
     if (t == 0) {
         S.status = 0;
         S.fsize = fstat(fd);   // one read, broadcast: the loop bound must be workgroup-uniform
@@ -232,14 +230,31 @@ __device__ inline int grep_one_file(grep_lds& S, const char* name, uint32_t nw, 
         grep_scan_window(S, dsz);
         __syncthreads();
         GFS_TRACE(7);
+        on_window();
     }
     __syncthreads();
+    const int st = gfs::uni(S.status);
+    __syncthreads();
+    return st;
+}
+
+// One file, the whole chunk loop; S.counts is zeroed here and the file's
+// counts land in row[0..nw).  Returns 0 or a negative code (workgroup-uniform).
+__device__ inline int grep_one_file(grep_lds& S, const char* name, uint32_t nw, uint32_t chunk, uint32_t* row) {
+    const uint32_t T = gfs::nthreads(), t = gfs::tid();
+    for (uint32_t q = t; q < nw; q += T) S.counts[q] = 0;
+    int fd = gopen(name, O_GRDONLY);
+    if (fd < 0) {
+        for (uint32_t q = t; q < nw; q += T) row[q] = 0;
+        return fd;
+    }
+    const int sst = grep_file_windows(S, fd, chunk, [] {});
     int c = gclose(fd);
     for (uint32_t q = t; q < nw; q += T) {
         uint32_t k = S.canon[q];
         row[q] = k == 0xFFFF ? 0u : S.counts[k];
     }
-    int st = gfs::uni(S.status ? S.status : c);
+    int st = gfs::uni(sst ? sst : c);
     __syncthreads();
     return st;
 }

@@ -18,6 +18,11 @@
 // hash table of the dictionary (O(bytes)); equal query words share a counter.
 #include "grep_scan.h"
 
+#include <cstring>
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 __global__ __launch_bounds__(1024) void k_grep(const char* names, uint32_t stride, const char* dict, uint32_t nw,
                                                uint32_t chunk, uint32_t* counts_out, int32_t* status) {
     __shared__ __attribute__((aligned(16))) grep_lds S;
@@ -136,4 +141,200 @@ extern "C" int gpufs_graph_launch(void* graph_exec, void* stream) {
 extern "C" int gpufs_graph_destroy(void* graph_exec) {
     if (!graph_exec) return -EINVAL;
     return hipGraphExecDestroy((hipGraphExec_t)graph_exec) == hipSuccess ? 0 : -EIO;
+}
+
+// ---------------------------------------------------------------------------
+// The grep_text sample end to end (samples/grep_text/grep_text_gpu.cu:193-387):
+// three GPUfs files -- `src`, query words as 32-byte NUL-padded records
+// (dict.gpu), `dbs`, a list of database file names, and `out`, opened
+// O_GWRONCE.  Workgroup b takes query records [b*wpc, b*wpc+wpc) (the last
+// workgroup also takes the remainder), reads the list, and scans every
+// database in its chunk loop; after each window it formats a line
+// "<word> <count> <database>\n" for every record whose count in that window
+// is nonzero, reserves that many bytes of the output with a device atomic and
+// gwrites them there.  The line order depends on timing, as in the
+// reference; the set of lines does not.  Counts are printed in full decimal.
+struct grep_text_args {
+    const char* src;
+    const char* out;
+    const char* dbs;
+    uint64_t total_words;
+    uint32_t wpc;
+    uint32_t chunk;
+    uint8_t* scratch;            // per workgroup: words | list | name | lines
+    uint64_t stride, off_list, off_name, off_lines, list_cap;
+    unsigned long long* out_off;
+    int32_t* status;
+};
+
+// get_next() of grep_text_gpu.cu:163-186 on a NUL-terminated list: skips
+// ' ', '\t', ',', '\r', '\n'; a name ends at '\n', '\r', ',' or NUL; returns
+// its length (0 when the list is over or the name is 64 bytes or longer) and
+// advances *pos past the byte that ended it.
+__device__ inline uint32_t text_next_name(const char* list, uint64_t* pos, char* name) {
+    uint64_t i = *pos;
+    while (list[i] == ' ' || list[i] == '\t' || list[i] == ',' || list[i] == '\r' || list[i] == '\n') ++i;
+    const uint64_t beg = i;
+    while (list[i] != '\n' && list[i] != '\r' && list[i] != ',' && list[i] != 0 && i < beg + 64) ++i;
+    const uint64_t n = i - beg;
+    if (n == 0 || n >= 64) return 0;
+    for (uint64_t k = 0; k < n; ++k) name[k] = list[beg + k];
+    for (uint64_t k = n; k < 128; ++k) name[k] = 0;
+    *pos = list[i] ? i + 1 : i;
+    return (uint32_t)n;
+}
+
+__global__ __launch_bounds__(1024) void k_grep_text(grep_text_args a) {
+    __shared__ __attribute__((aligned(16))) grep_lds S;
+    __shared__ uint32_t line_bytes, name_len;
+    __shared__ int wst;   // first gwrite failure of the current database (thread 0 writes it)
+    __shared__ uint64_t list_pos, out_at;
+    const uint32_t T = gfs::nthreads(), t = gfs::tid();
+    uint8_t* base = a.scratch + (size_t)blockIdx.x * a.stride;
+    char* words = (char*)base;
+    char* list = (char*)(base + a.off_list);
+    char* name = (char*)(base + a.off_name);
+    char* lines = (char*)(base + a.off_lines);
+    const uint64_t first = (uint64_t)blockIdx.x * a.wpc;
+    const uint32_t nw = first >= a.total_words ? 0u
+                        : (blockIdx.x == gridDim.x - 1 ? (uint32_t)(a.total_words - first) : a.wpc);
+    int st = 0;
+    const int fd_dbs = gopen(a.dbs, O_GRDONLY);
+    const int fd_o = gopen(a.out, O_GWRONCE);
+    const int fd_src = gopen(a.src, O_GRDONLY);
+    if (fd_dbs < 0 || fd_o < 0 || fd_src < 0) st = fd_dbs < 0 ? fd_dbs : (fd_o < 0 ? fd_o : fd_src);
+    if (st == 0 && nw > 0) {
+        size_t got = gread(fd_src, first * 32, (size_t)nw * 32, (uchar*)words);
+        if (got != (size_t)nw * 32) st = (int64_t)got < 0 ? (int)(int64_t)got : -EIO;
+    }
+    if (st == 0 && nw > 0) {
+        size_t lsz = fstat(fd_dbs);
+        if (lsz > a.list_cap - 1) lsz = a.list_cap - 1;
+        size_t got = lsz ? gread(fd_dbs, 0, lsz, (uchar*)list) : 0;
+        if (got != lsz) st = (int64_t)got < 0 ? (int)(int64_t)got : -EIO;
+        if (t == 0) {
+            list[lsz] = 0;
+            list_pos = 0;
+            line_bytes = 0;
+        }
+    }
+    if (st == 0 && nw > 0) {
+        grep_build_dict(S, words, nw);
+        for (uint32_t q = t; q < nw; q += T) S.counts[q] = 0;
+        for (;;) {
+            if (t == 0) {
+                uint64_t p = list_pos;
+                name_len = text_next_name(list, &p, name);
+                list_pos = p;
+            }
+            __syncthreads();
+            const uint32_t nl = gfs::uni(name_len);
+            if (nl == 0) break;
+            const int fd = gopen(name, O_GRDONLY);
+            if (fd < 0) { st = fd; break; }
+            if (t == 0) wst = 0;
+            const int fst = grep_file_windows(S, fd, a.chunk, [&] {
+                // this window's lines, then the counts start again from zero
+                for (uint32_t q = t; q < nw; q += T) {
+                    const uint32_t k = S.canon[q];
+                    const uint32_t c = k == 0xFFFF ? 0u : S.counts[k];
+                    if (!c) continue;
+                    char num[10];
+                    uint32_t nd = 0;
+                    for (uint32_t v = c; v; v /= 10) num[nd++] = (char)('0' + v % 10);
+                    const uint32_t wl = S.wlen[q];
+                    const uint32_t len = wl + 1 + nd + 1 + nl + 1;
+                    char* o = lines + atomicAdd(&line_bytes, len);
+                    for (uint32_t i = 0; i < wl; ++i) o[i] = words[(size_t)q * 32 + i];
+                    o[wl] = ' ';
+                    for (uint32_t i = 0; i < nd; ++i) o[wl + 1 + i] = num[nd - 1 - i];
+                    o[wl + 1 + nd] = ' ';
+                    for (uint32_t i = 0; i < nl; ++i) o[wl + 2 + nd + i] = name[i];
+                    o[len - 1] = '\n';
+                }
+                __threadfence();
+                __syncthreads();
+                const uint32_t total = gfs::uni(line_bytes);
+                if (total) {
+                    if (t == 0) out_at = atomicAdd(a.out_off, (unsigned long long)total);
+                    __syncthreads();
+                    const size_t w = gwrite(fd_o, gfs::uni(out_at), total, (uchar*)lines);
+                    if (w != total && t == 0 && wst == 0) wst = (int64_t)w < 0 ? (int)(int64_t)w : -EIO;
+                }
+                for (uint32_t q = t; q < nw; q += T) S.counts[q] = 0;
+                if (t == 0) line_bytes = 0;
+                __syncthreads();
+            });
+            const int c = gclose(fd);   // collective: its barriers also publish wst
+            const int w = gfs::uni(wst);
+            st = fst ? fst : (w ? w : c);
+            if (st) break;
+        }
+    }
+    if (fd_src >= 0) gclose(fd_src);
+    if (fd_dbs >= 0) gclose(fd_dbs);
+    const int co = fd_o >= 0 ? gclose(fd_o) : 0;
+    if (t == 0 && (st || co)) atomicMin(a.status, st ? st : co);
+}
+
+extern "C" int gpufs_grep_text(gpufs_ctx* ctx, const char* src_path, const char* out_path, const char* dbs_path,
+                               uint32_t chunk_bytes, uint32_t workgroups, uint32_t threads, int32_t* status_dev,
+                               void* stream) {
+    if (!ctx || !src_path || !out_path || !dbs_path || !status_dev) return -EINVAL;
+    if (chunk_bytes == 0 || chunk_bytes > GREP_MAX_CHUNK) return -EINVAL;
+    if (threads < 64 || threads > 1024 || threads % 64 || workgroups == 0) return -EINVAL;
+    const size_t ln = strlen(src_path), lo = strlen(out_path), ld = strlen(dbs_path);
+    if (ln >= GFS_NAME_MAX || lo >= GFS_NAME_MAX || ld >= GFS_NAME_MAX) return -ENAMETOOLONG;
+    struct stat sb;
+    if (stat(src_path, &sb) != 0) return -errno;
+    const uint64_t total = (uint64_t)sb.st_size / 32;
+    if (total == 0) return -EINVAL;   // "empty input" in the reference
+    if (stat(dbs_path, &sb) != 0) return -errno;
+    const uint64_t list_cap = (uint64_t)sb.st_size + 1;
+    // words per workgroup as the reference splits them; more workgroups when a
+    // slice would not fit the 1024-word LDS table (the lines do not depend on
+    // how the records are split)
+    uint64_t grid = workgroups;
+    for (;; ++grid) {
+        uint64_t wpc = total / grid;
+        if (wpc == 0) wpc = 1;
+        const uint64_t last = total > wpc * (grid - 1) ? total - wpc * (grid - 1) : 0;
+        if (wpc <= GREP_MAX_WORDS && last <= GREP_MAX_WORDS) break;
+        if (grid > (1u << 20)) return -EINVAL;
+    }
+    uint64_t wpc = total / grid;
+    if (wpc == 0) wpc = 1;
+    if (grid > total) grid = total;   // every workgroup has a record
+    // the output starts empty (the reference truncates it from the GPU once)
+    int ofd = open(out_path, O_WRONLY | O_CREAT | O_TRUNC | O_CLOEXEC, 0644);
+    if (ofd < 0) return -errno;
+    close(ofd);
+    int v = gpufs_bind(ctx);
+    if (v) return v;
+    const uint64_t words_b = (uint64_t)GREP_MAX_WORDS * 32;
+    const uint64_t lines_b = (uint64_t)GREP_MAX_WORDS * (32 + 1 + 10 + 1 + 63 + 1);
+    auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
+    const uint64_t off_list = al(words_b), off_name = al(off_list + list_cap + 64), off_lines = al(off_name + 128);
+    const uint64_t stride = al(off_lines + lines_b);
+    hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)gpufs_stream(ctx);
+    uint8_t* scratch = nullptr;
+    char* names = nullptr;
+    unsigned long long* off = nullptr;
+    if (hipMallocAsync((void**)&scratch, stride * grid, s) != hipSuccess) return -ENOMEM;
+    if (hipMallocAsync((void**)&names, 3 * GFS_NAME_MAX + 8, s) != hipSuccess) return -ENOMEM;
+    off = (unsigned long long*)(names + 3 * GFS_NAME_MAX);
+    char hn[3 * GFS_NAME_MAX + 8];
+    memset(hn, 0, sizeof(hn));
+    memcpy(hn, src_path, ln);
+    memcpy(hn + GFS_NAME_MAX, out_path, lo);
+    memcpy(hn + 2 * GFS_NAME_MAX, dbs_path, ld);
+    if (hipMemcpyAsync(names, hn, sizeof(hn), hipMemcpyHostToDevice, s) != hipSuccess) return -EIO;
+    if (hipStreamSynchronize(s) != hipSuccess) return -EIO;   // hn is a stack buffer
+    grep_text_args a{names, names + GFS_NAME_MAX, names + 2 * GFS_NAME_MAX, total, (uint32_t)wpc, chunk_bytes,
+                     scratch, stride, off_list, off_name, off_lines, list_cap, off, status_dev};
+    hipLaunchKernelGGL(k_grep_text, dim3((uint32_t)grid), dim3(threads), 0, s, a);
+    if (hipGetLastError() != hipSuccess) return -EIO;
+    (void)hipFreeAsync(scratch, s);
+    (void)hipFreeAsync(names, s);
+    return (int)grid;
 }

@@ -184,6 +184,22 @@ int gpufs_serve_stop(gpufs_server* srv);
  * beside workgroups that hold every compute unit. */
 int gpufs_serve_pages(gpufs_server* srv, uint64_t* page_hits, uint64_t* page_misses);
 
+/* The grep_text sample end to end (samples/grep_text/grep_text_gpu.cu:193-387,
+ * driver samples/grep_text/fs.cu): host path strings of the query records
+ * (32-byte NUL-padded, dict.gpu format), the output file and the database list
+ * (names < 64 bytes separated by newlines, commas, spaces or tabs).  The
+ * output is truncated, then every workgroup scans every database for its
+ * slice of the records and appends "<word> <count> <database>\n" for each
+ * (record, window) with a nonzero count through gwrite (O_GWRONCE); line
+ * order follows timing, as in the reference.  Launches on `stream` (NULL =
+ * context stream); the caller then serves it (gpufs_run_handler).  Returns
+ * the number of workgroups launched (more than asked when a slice would
+ * exceed 1024 records) or a negative code; *status_dev (caller-zeroed)
+ * receives the first failure. */
+int gpufs_grep_text(gpufs_ctx* ctx, const char* src_path, const char* out_path, const char* dbs_path,
+                    uint32_t chunk_bytes, uint32_t workgroups, uint32_t threads, int32_t* status_dev,
+                    void* stream);
+
 /* Record checksums through the wave-level mapping gmmap_warp / gmunmap_warp
  * (samples/mosaic/mosaic_gpufs.cu:612-651 pattern; device API
  * libgpufs/fs_calls.cu:483-683): every wave maps fixed-size records

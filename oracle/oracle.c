@@ -24,6 +24,7 @@
  *   orc_gemv_file    the same over a matrix file: threaded pread of row blocks,
  *                    y (double or float accumulation) and per-block word sums.
  *   orc_or_merge     the O_GWRONCE write-back merge, libgpufs/host_loop.h:76-137.
+ *   orc_grep_text_lines  the grep_text sample's output lines (grep_text_gpu.cu:286-349).
  */
 #define _GNU_SOURCE
 #include <errno.h>
@@ -226,6 +227,101 @@ int orc_scan_fast(const uint8_t* data, size_t n, uint32_t chunk, const char* dic
     free(buf);
     dict_free(&m);
     return 0;
+}
+
+/* ---- the grep_text sample's output file (grep_text_gpu.cu:286-349):
+ * database names come from a list in the format get_next() parses
+ * (grep_text_gpu.cu:163-186: leading ' ', '\t', ',', '\r', '\n' skipped; a name
+ * ends at '\n', '\r', ',' or NUL; a name of 64 or more bytes, or an empty one,
+ * ends the list); for each database, each window of the chunk loop and each
+ * query record q in order whose count in that window is nonzero, the line
+ * "<word> <count> <database name>\n".  The reference emits these lines in an
+ * order set by thread timing, so callers compare sorted lines.  Counts are
+ * printed in full (the reference's print_uint, grep_text_gpu.cu:34-54, swaps
+ * the middle digits of 4-digit counts and overflows its buffer at 5).
+ * Returns the bytes of output (written up to `cap`) or -errno. */
+static int64_t pread_all(int fd, uint8_t* buf, size_t len, uint64_t off);
+
+typedef struct {
+    const dict_map* m;
+    uint32_t* acc;
+    const char* dict;
+    uint32_t nw;
+    const char* name;
+    int name_len;
+    char* out;
+    uint64_t cap, len;
+} lines_st;
+
+static void lines_window(void* p, const uint8_t* win, int dsz) {
+    lines_st* s = (lines_st*)p;
+    fast_st f = {s->m, s->acc};
+    memset(s->acc, 0, sizeof(uint32_t) * (s->nw + 1));
+    fast_window(&f, win, dsz);
+    for (uint32_t q = 0; q < s->nw; q++) {
+        if (s->m->canon[q] < 0) continue;
+        uint32_t c = s->acc[s->m->canon[q]];
+        if (!c) continue;
+        char num[16];
+        int nl = 0;
+        char rev[16];
+        do { rev[nl++] = (char)('0' + c % 10); c /= 10; } while (c);
+        for (int i = 0; i < nl; i++) num[i] = rev[nl - 1 - i];
+        const uint32_t wl = s->m->len[q];
+        const uint64_t ll = (uint64_t)wl + 1 + (uint64_t)nl + 1 + (uint64_t)s->name_len + 1;
+        if (s->len + ll <= s->cap) {
+            char* o = s->out + s->len;
+            memcpy(o, s->dict + (size_t)q * 32, wl);
+            o[wl] = ' ';
+            memcpy(o + wl + 1, num, (size_t)nl);
+            o[wl + 1 + nl] = ' ';
+            memcpy(o + wl + 2 + nl, s->name, (size_t)s->name_len);
+            o[ll - 1] = '\n';
+        }
+        s->len += ll;
+    }
+}
+
+int64_t orc_grep_text_lines(const char* dict, uint32_t nw, const char* dblist, uint64_t dblist_len, uint32_t chunk,
+                            char* out, uint64_t cap) {
+    if (chunk == 0) return -EINVAL;
+    dict_map m;
+    if (dict_build(&m, dict, nw)) return -ENOMEM;
+    uint32_t* acc = (uint32_t*)calloc(nw + 1, sizeof(uint32_t));
+    uint8_t* buf = (uint8_t*)malloc((size_t)chunk + 64);
+    if (!acc || !buf) return -ENOMEM;
+    lines_st st = {&m, acc, dict, nw, NULL, 0, out, cap, 0};
+    uint64_t i = 0;
+    int64_t rc = 0;
+    for (;;) {
+        while (i < dblist_len && (dblist[i] == ' ' || dblist[i] == '\t' || dblist[i] == ',' || dblist[i] == '\r' ||
+                                  dblist[i] == '\n'))
+            i++;
+        const uint64_t beg = i;
+        while (i < dblist_len && dblist[i] != '\n' && dblist[i] != '\r' && dblist[i] != ',' && dblist[i] != 0 && i < 64 + beg)
+            i++;
+        const uint64_t nlen = i - beg;
+        if (nlen == 0 || nlen >= 64) break;
+        char name[65];
+        memcpy(name, dblist + beg, nlen);
+        name[nlen] = 0;
+        int fd = open(name, O_RDONLY);
+        if (fd < 0) { rc = -errno; break; }
+        off_t sz = lseek(fd, 0, SEEK_END);
+        uint8_t* data = (uint8_t*)malloc((size_t)sz + 1);
+        if (!data || pread_all(fd, data, (size_t)sz, 0) != (int64_t)sz) { close(fd); free(data); rc = -EIO; break; }
+        close(fd);
+        st.name = name;
+        st.name_len = (int)nlen;
+        chunk_loop(data, (size_t)sz, chunk, buf, lines_window, &st);
+        free(data);
+        if (i >= dblist_len) break;
+        i++;   /* the byte that ended the name */
+    }
+    free(acc);
+    free(buf);
+    dict_free(&m);
+    return rc ? rc : (int64_t)st.len;
 }
 
 /* ---- CPU path over files: pread windows + fast scan, one file per task */

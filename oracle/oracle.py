@@ -30,6 +30,8 @@ def lib():
         L.orc_gemv_f64.argtypes = [P, u64, u64, P, P]
         L.orc_gemv_file.argtypes = [ctypes.c_char_p, u64, u64, P, u64, P, P, i32, i32]
         L.orc_gemv_file.restype = i32
+        L.orc_grep_text_lines.argtypes = [P, u32, ctypes.c_char_p, u64, u32, P, u64]
+        L.orc_grep_text_lines.restype = ctypes.c_int64
         L.orc_or_merge.argtypes = [P, P, u64]
         for f in (L.orc_scan_naive, L.orc_scan_fast, L.orc_scan_files, L.orc_pread_ranges):
             f.restype = i32
@@ -114,6 +116,20 @@ def gemv_file(path, rows, cols, x, rows_per_block, f64=True, threads=1):
     if rc != 0:
         raise OSError(-rc, f"orc_gemv_file({path})")
     return y, cs
+
+
+def grep_text_lines(dict_words, dblist, chunk):
+    """Output of the grep_text sample for a database list (bytes, deterministic order)."""
+    d = _dict_array(dict_words)
+    nw = d.size // 32
+    lst = dblist if isinstance(dblist, bytes) else dblist.encode()
+    n = lib().orc_grep_text_lines(_ptr(d), nw, lst, len(lst), chunk, None, 0)
+    if n < 0:
+        raise OSError(-n, "orc_grep_text_lines")
+    out = np.zeros(max(1, n), dtype=np.uint8)
+    m = lib().orc_grep_text_lines(_ptr(d), nw, lst, len(lst), chunk, _ptr(out), n)
+    assert m == n
+    return out[:n].tobytes()
 
 
 def or_merge(page, disk):

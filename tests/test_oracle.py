@@ -215,3 +215,65 @@ def test_or_merge_known_answer(orc):
     page = np.array([0x01, 0x10, 0x00, 0xF0], dtype=np.uint8)
     disk = np.array([0x02, 0x01, 0x80], dtype=np.uint8)
     assert list(orc.or_merge(page, disk)) == [0x03, 0x11, 0x80, 0xF0]
+
+
+def _dict(words):
+    d = np.zeros((len(words), 32), dtype=np.uint8)
+    for i, w in enumerate(words):
+        b = w.encode()
+        d[i, :len(b)] = np.frombuffer(b, dtype=np.uint8)
+    return d
+
+
+@pytest.fixture
+def short_dir():
+    """Database names must stay under 64 bytes (the sample's name limit)."""
+    import shutil
+    import tempfile
+    d = tempfile.mkdtemp(prefix="gt", dir="/tmp")
+    yield __import__("pathlib").Path(d)
+    shutil.rmtree(d, ignore_errors=True)
+
+
+def test_grep_text_lines_known_answers(orc, short_dir):
+    """grep_text output lines: format "<word> <count> <db>\\n", one line per
+    query record (duplicates included) and window with a nonzero count, full
+    decimal counts, get_next list parsing."""
+    a = short_dir / "a.txt"
+    a.write_bytes(b"cat dog cat\nbird")
+    b = short_dir / "b.txt"
+    b.write_bytes(b"ab " * 1234)
+    d = _dict(["cat", "dog", "cow", "cat", "ab"])
+    lst = f"  {a},{b}\r\n".encode()
+    out = orc.grep_text_lines(d, lst, 65536).decode()
+    assert out == (f"cat 2 {a}\ndog 1 {a}\ncat 2 {a}\n" f"ab 1234 {b}\n")
+    # tabs / newlines as separators, an empty list, a 64-byte name ends the list
+    assert orc.grep_text_lines(d, f"\t{b}\n\n{a}\n".encode(), 65536).decode() == (
+        f"ab 1234 {b}\ncat 2 {a}\ndog 1 {a}\ncat 2 {a}\n")
+    assert orc.grep_text_lines(d, b"\n , \n", 65536) == b""
+    long_name = "/" + "x" * 63
+    assert orc.grep_text_lines(d, f"{long_name}\n{a}\n".encode(), 65536) == b""
+
+
+def test_grep_text_lines_sum_to_file_counts(orc, short_dir):
+    """Per-window counts (several windows per file) add up to the file's
+    counts from the pinned scan, once per query record holding the word."""
+    from gpufs_amd import datagen as dg
+    voc = dg.vocab(300, seed=5)
+    q = np.ascontiguousarray(voc[dg.query(300, 40, seed=6)])
+    paths = [short_dir / f"f{i}.txt" for i in range(3)]
+    dg.text_files([str(p) for p in paths], [20000, 7777, 300], 9, voc, threads=2)
+    chunk = 1024
+    out = orc.grep_text_lines(q, ("\n".join(str(p) for p in paths) + "\n").encode(), chunk).decode()
+    words = [bytes(r).rstrip(b"\0").decode() for r in q]
+    for p in paths:
+        total = {}
+        for line in out.splitlines():
+            w, c, name = line.split(" ")
+            if name == str(p):
+                total[w] = total.get(w, 0) + int(c)
+        ref = orc.scan_fast(np.fromfile(p, dtype=np.uint8), q, chunk)
+        for i, w in enumerate(words):
+            assert total.get(w, 0) == ref[i] * words.count(w), (p, w)
+    nlines = len(out.splitlines())
+    assert nlines > 3 * len(set(words))   # several windows per file contribute lines
