@@ -32,7 +32,7 @@ STATS_FIELDS = [
     "page_hits", "page_misses", "page_waits", "evictions", "alloc_fail", "rpc_reads", "rpc_writes",
     "rpc_opens", "rpc_closes", "bytes_filled", "bytes_written", "nodes_used", "files_used",
     "host_requests", "host_bytes_read", "host_bytes_written", "frames", "frames_valid", "timeouts",
-    "host_busy_ns",
+    "host_busy_ns", "vm_errors",
 ]
 
 
@@ -78,6 +78,7 @@ SIGNATURES = {
     "gpufs_serve_pages": (I, [P, P, P]),
     "gpufs_launch_mmap_warp_records": (I, [P, P, U64, U64, P, P, U32, U32, P]),
     "gpufs_grep_text": (I, [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, U32, U32, U32, P, P]),
+    "gpufs_launch_vm_records": (I, [P, P, U64, P, U64, P, P, U32, U32, U32, P]),
     "gpufs_launch_gemv_mmap": (I, [P, P, U64, U64, P, P, U32, P, P, U32, P]),
     "gpufs_fnv1a64_ranges": (I, [P, U32, U64, P, P]),
     "gpufs_launch_read": (I, [P, P, I32, U64, U64, P, P, U32, P]),

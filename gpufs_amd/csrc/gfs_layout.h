@@ -101,6 +101,7 @@ typedef struct gfs_counters {
     unsigned long long rpc_reads, rpc_writes, rpc_opens, rpc_closes;
     unsigned long long bytes_filled, bytes_written;
     unsigned long long timeouts;
+    unsigned long long vm_errors;
 } gfs_counters;
 
 typedef struct gfs_dev {

@@ -7,6 +7,7 @@
 // reproducible run to run, and every mapped 8-byte word is added into a
 // per-workgroup 64-bit checksum so the mapped bytes can be verified exactly.
 #include "gpufs.h"
+#include "gpufs_vm.h"
 
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
@@ -157,5 +158,69 @@ extern "C" int gpufs_launch_mmap_warp_records(gpufs_ctx* ctx, const char* path_d
     hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)gpufs_stream(ctx);
     hipLaunchKernelGGL(k_mmap_warp_records, dim3(workgroups), dim3(threads), 0, s, path_dev, record_bytes, n_records,
                        (unsigned long long*)sums_dev, status_dev);
+    return hipGetLastError() == hipSuccess ? 0 : -EIO;
+}
+
+// Record sums through ActivePointers, the pattern of
+// samples/mosaic/mosaic_gpufs.cu:1010-1060 (one FatPointer per thread over
+// the whole file; for each record: moveTo the record, += lane, then steps of
+// 64 elements, dereferencing each).  Wave v handles order slots
+// [v*per_wave, v*per_wave + per_wave); slot j's record is order[j], and the
+// 64-bit sum of its uint32 elements goes to sums[j].  A TLB of N lines per
+// workgroup (N = 0: every page mapped directly).
+template <int N>
+__global__ __launch_bounds__(1024) void k_vm_records(const char* path, uint64_t rec_elems, const uint32_t* order,
+                                                     uint64_t per_wave, unsigned long long* sums, int32_t* status) {
+    __shared__ TLB<N> tlb;
+    tlb.init();
+    const uint32_t ln = gfs::lane(), w = gfs::wave(), nw = (gfs::nthreads() + 63) >> 6;
+    const uint64_t gw = (uint64_t)blockIdx.x * nw + w;
+    int fd = gopen(path, O_GRDONLY);
+    if (fd < 0) {
+        if (gfs::tid() == 0) atomicMin(status, fd);
+        return;
+    }
+    {
+        FatPointer<uint32_t, N> p = gvmmap<uint32_t, N>(nullptr, ~0ull >> 1, 0, O_GRDONLY, fd, 0, &tlb);
+        for (uint64_t j = 0; j < per_wave; ++j) {
+            const uint64_t slot = gw * per_wave + j;
+            const uint64_t r = order[slot];
+            p.moveTo(r * rec_elems * sizeof(uint32_t));
+            p += ln;
+            unsigned long long acc = 0;
+            for (uint64_t k = 0; k < rec_elems; k += 64) {
+                if (k) p += 64;
+                acc += *p;
+            }
+            acc = wave_sum_u64(acc);
+            if (ln == 0) sums[slot] = acc;
+        }
+    }   // the pointer is gone: what it held is dropped
+    tlb.release();
+    const int c = gclose(fd);
+    if (c && gfs::tid() == 0) atomicMin(status, c);
+}
+
+extern "C" int gpufs_launch_vm_records(gpufs_ctx* ctx, const char* path_dev, uint64_t record_elems,
+                                       const uint32_t* order_dev, uint64_t per_wave, uint64_t* sums_dev,
+                                       int32_t* status_dev, uint32_t tlb_lines, uint32_t workgroups,
+                                       uint32_t threads, void* stream) {
+    if (!ctx || !path_dev || !order_dev || !sums_dev || !status_dev || per_wave == 0 || workgroups == 0)
+        return -EINVAL;
+    if (record_elems == 0 || record_elems % 64) return -EINVAL;
+    if (threads < 64 || threads > 1024 || threads % 64) return -EINVAL;
+    int v = gpufs_bind(ctx);
+    if (v) return v;
+    hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)gpufs_stream(ctx);
+    unsigned long long* sums = (unsigned long long*)sums_dev;
+    switch (tlb_lines) {
+    case 0: hipLaunchKernelGGL(k_vm_records<0>, dim3(workgroups), dim3(threads), 0, s, path_dev, record_elems,
+                               order_dev, per_wave, sums, status_dev); break;
+    case 8: hipLaunchKernelGGL(k_vm_records<8>, dim3(workgroups), dim3(threads), 0, s, path_dev, record_elems,
+                               order_dev, per_wave, sums, status_dev); break;
+    case 32: hipLaunchKernelGGL(k_vm_records<32>, dim3(workgroups), dim3(threads), 0, s, path_dev, record_elems,
+                                order_dev, per_wave, sums, status_dev); break;
+    default: return -EINVAL;
+    }
     return hipGetLastError() == hipSuccess ? 0 : -EIO;
 }

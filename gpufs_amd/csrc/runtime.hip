@@ -349,6 +349,7 @@ extern "C" int gpufs_stats_get(gpufs_ctx* c, gpufs_stats* o) {
     o->frames_valid = valid;
     o->timeouts = cur.ctr.timeouts;
     o->host_busy_ns = hc.busy_ns.load();
+    o->vm_errors = cur.ctr.vm_errors;
     return 0;
 }
 

@@ -70,6 +70,7 @@ typedef struct gpufs_stats {
     uint64_t frames_valid;   /* frames VALID at the time of the call */
     uint64_t timeouts;       /* device waits that hit the wait limit (GPUFS_WAIT_TIMEOUT_S) */
     uint64_t host_busy_ns;   /* time the host workers spent serving requests, summed over workers */
+    uint64_t vm_errors;      /* FatPointer dereferences whose page could not be mapped */
 } gpufs_stats;
 
 /* ---- lifecycle: replaces initializer(GPUGlobals_ptr*) (libgpufs/fs_initializer.cu:116)
@@ -183,6 +184,18 @@ int gpufs_serve_stop(gpufs_server* srv);
  * gpufs_check_invariants return -EBUSY: their device copies cannot run
  * beside workgroups that hold every compute unit. */
 int gpufs_serve_pages(gpufs_server* srv, uint64_t* page_hits, uint64_t* page_misses);
+
+/* Record sums through ActivePointers (FatPointer / gvmmap with a per-workgroup
+ * TLB of tlb_lines = 0, 8 or 32 lines; libgpufs/fat_pointer.cu.h, pattern of
+ * samples/mosaic/mosaic_gpufs.cu:1010-1060): wave v reads the records
+ * order_dev[v*per_wave + j] (record_elems uint32 each, a multiple of 64) and
+ * stores each record's 64-bit element sum in sums_dev[v*per_wave + j].
+ * order_dev holds workgroups * (threads / 64) * per_wave record ids.
+ * *status_dev (caller-zeroed) receives the first failure. */
+int gpufs_launch_vm_records(gpufs_ctx* ctx, const char* path_dev, uint64_t record_elems,
+                            const uint32_t* order_dev, uint64_t per_wave, uint64_t* sums_dev,
+                            int32_t* status_dev, uint32_t tlb_lines, uint32_t workgroups,
+                            uint32_t threads, void* stream);
 
 /* The grep_text sample end to end (samples/grep_text/grep_text_gpu.cu:193-387,
  * driver samples/grep_text/fs.cu): host path strings of the query records
