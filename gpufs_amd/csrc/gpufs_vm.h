@@ -5,8 +5,11 @@
 // A FatPointer<T, N> holds a file offset.  Dereferencing it resolves the page
 // through a TLB<N> in shared memory: lanes of a wave that need the same page
 // are served together; a TLB line holding that page is shared by reference
-// count; a missing page takes over a line nobody holds (dropping the line's
-// previous page) and is mapped with gmmap_warp.  While the line of a page is
+// count; a missing page takes over a line nobody holds and is mapped with
+// gmmap_warp; the last holder to leave a line returns its page, so the pages
+// pinned through lines never outnumber the pages pointers hold (a line that
+// kept its page while idle could pin more pages, over all workgroups, than
+// the cache has frames, and starve every other mapping).  While the line of a page is
 // held for another page, the lanes map their page directly, each lane with a
 // pin of its own, instead of waiting for the line (a wave whose own lanes
 // hold the line would otherwise wait for itself).  N == 0 maps every page
@@ -135,8 +138,24 @@ private:
             const bool same = need && m_holder == h && m_frame == f;
             const uint32_t n = gfs::popc(__ballot(same));
             if (gfs::lane() == leader) {
-                if (h >= 0) atomicSub(&m_tlb->ref[h], (int32_t)n);
-                else gfs::fadd(&g->meta[f], (uint64_t)0 - (uint64_t)n);
+                if (h >= 0) {
+                    // the last holder returns the line's page (a line keeps no
+                    // page without holders, so the pages TLB lines pin never
+                    // outnumber the pages pointers hold), unless a holder
+                    // joined meanwhile: then the lock below fails
+                    const int32_t old = atomicSub(&m_tlb->ref[h], (int32_t)n);
+                    if (old == (int32_t)n && atomicCAS(&m_tlb->ref[h], 0, INT32_MIN) == 0) {
+                        typename TLB<N>::line_t& ln = m_tlb->lines[h];
+                        if (ln.fid != GFS_VM_NOFILE) {
+                            gfs::fadd(&g->meta[ln.frame], ~0ull);
+                            ln.fid = GFS_VM_NOFILE;
+                        }
+                        __threadfence_block();
+                        atomicAdd(&m_tlb->ref[h], (int32_t)0x80000000u);   // unlock: back to 0
+                    }
+                } else {
+                    gfs::fadd(&g->meta[f], (uint64_t)0 - (uint64_t)n);
+                }
             }
             if (same) need = false;
         }
