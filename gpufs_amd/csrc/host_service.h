@@ -23,8 +23,11 @@
 #include <cstring>
 #include <thread>
 #include <vector>
+#include <climits>
 #include <fcntl.h>
+#include <linux/futex.h>
 #include <sys/prctl.h>
+#include <sys/syscall.h>
 #include <sys/stat.h>
 #include <time.h>
 #include <unistd.h>
@@ -174,19 +177,41 @@ static inline uint64_t mono_ns() {
     return ts_ns(t);
 }
 
+// Sleeping workers wait on `word` (a futex) with a timeout; a worker that
+// takes a request while more are already queued wakes them, so a burst (the
+// page reads of a large cold file) is served by all workers at once instead
+// of by the one that was awake until the others' timers expire.
+struct WakeState {
+    std::atomic<uint32_t> word{0};
+    std::atomic<uint32_t> sleepers{0};
+};
+
+static inline void wake_all(WakeState* w) {
+    w->word.fetch_add(1, std::memory_order_release);
+    syscall(SYS_futex, (uint32_t*)&w->word, FUTEX_WAKE_PRIVATE, INT_MAX, nullptr, nullptr, 0);
+}
+
+// True when the submission queue holds a published request at `head`.
+static inline bool backlog(const RingView& rv, const std::atomic<uint64_t>& head) {
+    const uint64_t h = head.load(std::memory_order_relaxed);
+    const uint64_t e = __atomic_load_n(&rv.sq[h & (rv.sq_depth - 1)], __ATOMIC_ACQUIRE);
+    return (e >> 16) == (h & 0xFFFFFFFFFFFFull);
+}
+
 // Worker `index` of the service.  Worker 0 polls the queue without sleeping
 // while the service runs, so a lone request (an open between two launches)
 // is picked up within a poll; the others spin for a while after their last
-// request, then sleep in short steps (timer slack lowered to 1 us so a 2 us
-// sleep is not stretched to the default 50 us), then in 50 us steps.
+// request, then sleep on the wake futex (2 us steps, 50 us after a longer
+// idle; timer slack lowered to 1 us so short sleeps are not stretched).
 static inline void worker_loop(RingView rv, std::atomic<uint64_t>* head, std::atomic<bool>* stop,
-                               ServiceCounters* ctr, unsigned index) {
+                               ServiceCounters* ctr, WakeState* wk, unsigned index) {
     prctl(PR_SET_TIMERSLACK, 1000UL, 0, 0, 0);
     std::vector<uint8_t> tmp;
     uint64_t idle = 0;
     while (!stop->load(std::memory_order_relaxed)) {
         uint32_t s;
         if (try_take(rv, *head, &s)) {
+            if (wk->sleepers.load(std::memory_order_relaxed) && backlog(rv, *head)) wake_all(wk);
             const uint64_t t0 = mono_ns();
             serve_slot(rv, s, *ctr, tmp);
             ctr->busy_ns.fetch_add(mono_ns() - t0, std::memory_order_relaxed);
@@ -199,8 +224,13 @@ static inline void worker_loop(RingView rv, std::atomic<uint64_t>* head, std::at
             __builtin_ia32_pause();
 #endif
         } else {
-            struct timespec ts = {0, idle < 200000 ? 2000 : 50000};
-            nanosleep(&ts, nullptr);
+            const uint32_t seen = wk->word.load(std::memory_order_acquire);
+            wk->sleepers.fetch_add(1, std::memory_order_acq_rel);
+            if (!backlog(rv, *head)) {
+                struct timespec ts = {0, idle < 200000 ? 2000 : 50000};
+                syscall(SYS_futex, (uint32_t*)&wk->word, FUTEX_WAIT_PRIVATE, seen, &ts, nullptr, 0);
+            }
+            wk->sleepers.fetch_sub(1, std::memory_order_acq_rel);
         }
     }
 }
@@ -211,12 +241,13 @@ public:
         if (running_) return;
         stop_.store(false);
         for (unsigned i = 0; i < nthreads; ++i)
-            threads_.emplace_back(worker_loop, rv, &head_, &stop_, &ctr_, i);
+            threads_.emplace_back(worker_loop, rv, &head_, &stop_, &ctr_, &wake_, i);
         running_ = true;
     }
     void stop() {
         if (!running_) return;
         stop_.store(true);
+        wake_all(&wake_);
         for (auto& t : threads_) t.join();
         threads_.clear();
         running_ = false;
@@ -231,6 +262,7 @@ private:
     std::atomic<uint64_t> head_{0};
     std::atomic<bool> stop_{false};
     ServiceCounters ctr_;
+    WakeState wake_;
     bool running_ = false;
 };
 
