@@ -334,6 +334,7 @@ def bench_grep(args, ctx, torch, L, _lib, dg):
         e1.synchronize()
         return e0.elapsed_time(e1) / 1e3
 
+    run()   # first launch of the kernel in the process: not timed
     ctx.cache_drop()
     t_cold = run()
     c_cold = counts.cpu().numpy().view(np.uint32).reshape(nfiles, 1024).copy()
