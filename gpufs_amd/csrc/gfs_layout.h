@@ -54,6 +54,8 @@
 
 // ---- tuning switches
 #define GFS_TUNE_PAGE_SYNC 1u   // gread: workgroup barrier after each page copied to the caller
+#define GFS_TUNE_NO_READAHEAD 2u   // gread: no sequential read-ahead
+#define GFS_READAHEAD_BYTES (1u << 20)   // pages fetched ahead of a sequential reader (at most 63 pages)
 
 // ---- RPC opcodes
 #define GFS_OP_READ  1u

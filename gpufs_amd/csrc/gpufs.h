@@ -81,7 +81,27 @@ struct wg_ctl {
     uint64_t a, b;
     uint32_t rot;
     uint32_t hits, misses;   // running page-lookup counts of this workgroup (wraps; callers take differences)
+    uint32_t ra;             // gread: extra pages fetched after the request (read-ahead)
 };
+
+// Per-workgroup read history for sequential read-ahead.  It is advisory: it
+// only decides how many pages beyond a read are fetched together with it.
+// LDS keeps whatever the previous kernel on the compute unit left there, so
+// the history carries a tag of the launch and workgroup; a stale match can
+// cost one unneeded prefetch, never a wrong byte.
+struct wg_hist {
+    uint64_t tag;
+    int32_t fd;
+    uint32_t streak;    // forward-sequential reads in a row
+    uint64_t first;     // pages of the last read
+    uint64_t last;
+    uint64_t fetched;   // pages below this were read or fetched ahead
+};
+__device__ __forceinline__ uint64_t launch_tag() {
+    const uint64_t blk = blockIdx.x + (uint64_t)gridDim.x * (blockIdx.y + (uint64_t)gridDim.y * blockIdx.z);
+    return ((uint64_t)(uintptr_t)__builtin_amdgcn_dispatch_ptr() * 0x9E3779B97F4A7C15ull) ^
+           (uint64_t)(uintptr_t)__builtin_amdgcn_kernarg_segment_ptr() ^ (blk << 40);
+}
 
 __device__ __forceinline__ uint32_t tid() {
     return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
@@ -673,6 +693,7 @@ __device__ inline int open_single(const char* name, int flags) {
 } // namespace gfs
 
 static __shared__ gfs::wg_ctl gfs_ctl;
+static __shared__ gfs::wg_hist gfs_hist;
 
 namespace gfs {
 
@@ -962,14 +983,46 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
             if (fst_state(st) != GFS_FS_OPEN) r = -EBADF;
             else r = offset >= fs ? 0 : (int64_t)(size < fs - offset ? size : fs - offset);
         }
+        // read-ahead: the third and later forward-sequential reads of a file
+        // that reach a page not fetched yet take up to GFS_READAHEAD_BYTES of
+        // the following pages into their last batch (same round trip to the
+        // host; only the requested bytes are copied out)
+        uint32_t ext = 0;
+        if (r > 0) {
+            const uint32_t Lg = g->log_page;
+            const uint64_t pf = offset >> Lg, pl = (offset + (uint64_t)r - 1) >> Lg;
+            const uint64_t tag = launch_tag();
+            const bool known = gfs_hist.tag == tag && gfs_hist.fd == fd;
+            const bool seq = known && pf >= gfs_hist.first && pf <= gfs_hist.last + 1;
+            const uint32_t streak = seq ? gfs_hist.streak + 1 : 0;
+            uint64_t fetched = known ? gfs_hist.fetched : 0;
+            if (streak >= 2 && pl >= fetched && !(g->tune_flags & GFS_TUNE_NO_READAHEAD)) {
+                const uint64_t npages = (fs + g->page_size - 1) >> Lg;
+                uint64_t want = ((uint64_t)GFS_READAHEAD_BYTES) >> Lg;
+                if (want < 1) want = 1;
+                const uint64_t room = 64 - (((pl - pf) % 64) + 1);   // free lanes of the last batch
+                if (want > room) want = room;
+                if (want > npages - 1 - pl) want = npages - 1 - pl;
+                ext = (uint32_t)want;
+            }
+            if (pl + 1 + ext > fetched) fetched = pl + 1 + ext;
+            gfs_hist.tag = tag;
+            gfs_hist.fd = fd;
+            gfs_hist.streak = streak;
+            gfs_hist.first = pf;
+            gfs_hist.last = pl;
+            gfs_hist.fetched = fetched;
+        }
         gfs_ctl.ret = r;
         gfs_ctl.a = r > 0 ? fs : 0;
         gfs_ctl.b = r > 0 ? root : 0;
+        gfs_ctl.ra = ext;
     }
     __syncthreads();
     const int64_t n = uni(gfs_ctl.ret);
     const uint64_t fsize = uni(gfs_ctl.a);
     const uint64_t root_hint = uni(gfs_ctl.b);
+    const uint32_t ra_ext = uni(gfs_ctl.ra);
     if (n <= 0) return (size_t)n;
     const uint32_t L = g->log_page;
     const uint64_t PS = g->page_size;
@@ -978,7 +1031,10 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
     const uint32_t bmax = 64;
     int err = 0;
     for (uint64_t pb = p_first; pb <= p_last && !err; pb += bmax) {
-        const uint32_t cnt = (uint32_t)((p_last - pb + 1) < bmax ? (p_last - pb + 1) : bmax);
+        // requested pages of this batch, then the read-ahead pages after the last one
+        const uint32_t creq = (uint32_t)((p_last - pb + 1) < bmax ? (p_last - pb + 1) : bmax);
+        const uint32_t cnt = creq + (pb + creq - 1 == p_last ? ra_ext : 0u);
+        const uint64_t reqm = creq == 64 ? ~0ull : ((1ull << creq) - 1);
         uint64_t pending = cnt == 64 ? ~0ull : ((1ull << cnt) - 1);
         uint32_t idle = 0;
         uint64_t t0 = now_ticks();
@@ -993,10 +1049,10 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
             GFS_TRACE(1);
             fill_from_host(g, cnt, pb, fsize);
             GFS_TRACE(2);
-            // copy every pinned page (hit or freshly filled) into the caller's buffer
+            // copy every requested pinned page (hit or freshly filled) into the caller's buffer
             const uint32_t T = nthreads(), t = tid();
             uint32_t rot = 0;
-            for (uint32_t i = 0; i < cnt; ++i) {
+            for (uint32_t i = 0; i < creq; ++i) {
                 uint32_t k = uni((uint32_t)gfs_ctl.kind[i]);
                 if (k != GFS_K_HIT && k != GFS_K_OWN) continue;
                 uint64_t poff = (pb + i) << L;
@@ -1012,7 +1068,12 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
             GFS_TRACE(3);
             if (wave() == 0) {
                 const uint32_t ln = lane();
-                if (gfs_ctl.err) fail_own(g, fd, pb, cnt);
+                if (gfs_ctl.err) {
+                    fail_own(g, fd, pb, cnt);
+                    // a read-ahead page that failed does not fail the read
+                    const uint64_t errm = __ballot(ln < creq && gfs_ctl.kind[ln] == GFS_K_ERR);
+                    if (ln == 0 && errm == 0) gfs_ctl.err = 0;
+                }
                 finish_batch(g, cnt);
                 uint64_t waitm = __ballot(ln < cnt && gfs_ctl.kind[ln] == GFS_K_WAIT);
                 if (ln == 0) gfs_ctl.a = waitm;
@@ -1020,7 +1081,8 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
             __syncthreads();
             GFS_TRACE(4);
             if (uni(gfs_ctl.err)) { err = uni(gfs_ctl.err); break; }
-            uint64_t nxt = uni(gfs_ctl.a);
+            // read-ahead pages are never waited for
+            uint64_t nxt = uni(gfs_ctl.a) & reqm;
             if (nxt == pending) {
                 if (++idle > 4) backoff();
                 if ((idle & 63) == 0 && wg_timed_out(g, t0)) { err = -ETIMEDOUT; break; }

@@ -98,3 +98,24 @@ def test_grep_many_workgroups_one_file(tmp_path, orc):
         assert s["page_misses"] == 48
         assert s["rpc_opens"] >= 1
         assert ctx.check_invariants() == 0
+
+
+def test_grep_sequential_read_ahead(tmp_path, orc):
+    """Small windows over small pages: sequential reads fetch pages ahead in
+    the same round trip; every page is still read from the host exactly once
+    (nothing twice, nothing past the end) and the counts stay exact."""
+    say("test_grep_sequential_read_ahead")
+    voc = dg.vocab(2000, seed=51)
+    q = voc[dg.query(2000, 256, seed=52)]
+    size = (1 << 20) + 100
+    paths = [str(tmp_path / f"ra{i}.txt") for i in range(4)]
+    dg.text_files(paths, [size] * 4, 53, voc)
+    cpu, _ = orc.scan_files(paths, q, 4096)
+    with make_ctx(page_size=16 << 10, cache_bytes=32 << 20, ring_depth=128) as ctx:
+        counts, status = grep(ctx, paths, q, 4096)
+        assert (status == 0).all(), status
+        assert (counts == cpu).all()
+        s = ctx.stats()
+        pages = 4 * ((size + (16 << 10) - 1) // (16 << 10))
+        assert s["page_misses"] == pages and s["rpc_reads"] == pages, s
+        assert ctx.check_invariants() == 0
