@@ -30,7 +30,8 @@
 #define E_IPC_OPEN_ERROR (-2)
 
 #define GFS_NAME_MAX 128
-#define GFS_NAME_RESERVED 0xFFFFFFFFu   // name-index slot taken, entry not yet published          // bytes incl. NUL (reference: FILENAME_SIZE 64)
+#define GFS_NAME_RESERVED 0xFFFFFFFFu   // name-index slot taken, entry not yet published
+#define GFS_LEAF_RESERVED (~0ull)       // radix leaf slot claimed by a workgroup, frame not yet published          // bytes incl. NUL (reference: FILENAME_SIZE 64)
 #define GFS_FANOUT_LOG 6          // radix fan-out 64 = one wave64 lane per slot
 #define GFS_FANOUT 64
 #define GFS_MAX_HEIGHT 7          // 64^7 pages

@@ -748,7 +748,9 @@ __device__ inline void resolve_batch(gfs_dev* g, int fd, uint64_t pb, uint32_t c
     if (mine) {
         v = ld_rlx(sp);
         kind = GFS_K_MISS;
-        if (v) {
+        if (v == GFS_LEAF_RESERVED) {
+            kind = GFS_K_WAIT;   // another workgroup is claiming a frame for it
+        } else if (v) {
             uint32_t vf = leaf_frame(v), vg = leaf_gen(v);
             uint64_t old = fadd_rlx(&g->meta[vf], 1ull);
             if (meta_gen(old) == vg && meta_state(old) == GFS_F_VALID) {
@@ -762,32 +764,35 @@ __device__ inline void resolve_batch(gfs_dev* g, int fd, uint64_t pb, uint32_t c
             }
         }
     }
-    // misses that need host data claim a ring slot first
+    // A miss first reserves the leaf slot, so of several workgroups missing
+    // the same page only one claims a ring slot and a frame (claims made by
+    // the losers would churn frames and push useful pages out); the others
+    // wait as for a fill in flight.
+    bool resv = false;
+    if (kind == GFS_K_MISS) {
+        uint64_t exp = v;
+        resv = cas(sp, exp, GFS_LEAF_RESERVED);
+        if (!resv) kind = GFS_K_WAIT;
+    }
+    // misses that need host data claim a ring slot, then a frame
     const bool nohost = ((nohost_mask >> ln) & 1ull) != 0;
-    int s = rpc_claim(g, kind == GFS_K_MISS && !nohost, blockIdx.x * 5u + ln);
-    bool need_frame = kind == GFS_K_MISS && (nohost || s >= 0);
-    if (kind == GFS_K_MISS && !need_frame) kind = GFS_K_WAIT;
+    int s = rpc_claim(g, resv && !nohost, blockIdx.x * 5u + ln);
+    const bool need_frame = resv && (nohost || s >= 0);
     uint32_t nf = frame_alloc_wave(g, need_frame, 2);
-    if (need_frame) {
-        if (nf == GFS_NOFRAME) {
+    if (resv) {
+        if (!need_frame || nf == GFS_NOFRAME) {
             if (s >= 0) rpc_release(g, s);
             s = -1;
+            st_rel(sp, v);   // give the slot back as it was
             kind = GFS_K_WAIT;
         } else {
             g->fkey[nf] = gfs_key((uint32_t)fd, page);
             g->fbytes[nf] = 0;
-            uint32_t gen = meta_gen(ld_rlx(&g->meta[nf]));
-            uint64_t exp = v;
-            if (cas(sp, exp, leaf_pack(nf, gen))) {
-                kind = GFS_K_OWN;
-                f = nf;
-                if (!g->fref[nf]) g->fref[nf] = 1;
-            } else {
-                frame_release(g, nf);
-                if (s >= 0) rpc_release(g, s);
-                s = -1;
-                kind = GFS_K_WAIT;
-            }
+            const uint32_t gen = meta_gen(ld_rlx(&g->meta[nf]));
+            st_rel(sp, leaf_pack(nf, gen));   // the reservation is ours: publish the frame
+            kind = GFS_K_OWN;
+            f = nf;
+            g->fref[nf] = 1;
         }
     }
     wave_count(&g->ctr.page_waits, kind == GFS_K_WAIT && mine);
@@ -1508,7 +1513,9 @@ __device__ inline volatile void* gmmap_warp(void* addr, size_t size, int prot, i
             if (!e) {
                 sp = &g->nodes[node * GFS_FANOUT + (page & (GFS_FANOUT - 1))];
                 v = ld_rlx(sp);
-                if (v) {
+                if (v == GFS_LEAF_RESERVED) {
+                    kind = GFS_K_WAIT;
+                } else if (v) {
                     const uint32_t vf = leaf_frame(v), vg = leaf_gen(v);
                     const uint64_t old = fadd_rlx(&g->meta[vf], (uint64_t)ref);
                     if (meta_gen(old) == vg && meta_state(old) == GFS_F_VALID) {
@@ -1532,21 +1539,32 @@ __device__ inline volatile void* gmmap_warp(void* addr, size_t size, int prot, i
         }
         uint32_t late = 0;
         if (kind == GFS_K_MISS) {
-            // a ring slot for the host read, then a frame
+            // reserve the leaf slot (see resolve_batch), then a ring slot for
+            // the host read, then a frame
+            uint32_t resv = 0;
+            if (ln == 0) {
+                uint64_t exp = v;
+                resv = cas(sp, exp, GFS_LEAF_RESERVED) ? 1u : 0u;
+            }
+            resv = uni(resv);
             int32_t s = -1;
-            if (ln == 0 && !nohost) s = rpc_claim(g, true, blockIdx.x * 5u + wave());
+            if (resv && ln == 0 && !nohost) s = rpc_claim(g, true, blockIdx.x * 5u + wave());
             s = uni(s);
-            if (nohost || s >= 0) {
+            if (resv && !(nohost || s >= 0) && ln == 0) st_rel(sp, v);   // no slot: give the leaf back
+            if (resv && (nohost || s >= 0)) {
                 uint32_t nf = uni(frame_alloc_wave(g, ln == 0, 2));
                 uint32_t own = 0;
-                if (nf != GFS_NOFRAME && ln == 0) {
-                    g->fkey[nf] = gfs_key((uint32_t)fd, page);
-                    g->fbytes[nf] = 0;
-                    const uint32_t gen = meta_gen(ld_rlx(&g->meta[nf]));
-                    uint64_t exp = v;
-                    own = cas(sp, exp, leaf_pack(nf, gen)) ? 1u : 0u;
-                    if (own) g->fref[nf] = 1;
-                    else frame_release(g, nf);
+                if (ln == 0) {
+                    if (nf != GFS_NOFRAME) {
+                        g->fkey[nf] = gfs_key((uint32_t)fd, page);
+                        g->fbytes[nf] = 0;
+                        const uint32_t gen = meta_gen(ld_rlx(&g->meta[nf]));
+                        st_rel(sp, leaf_pack(nf, gen));
+                        g->fref[nf] = 1;
+                        own = 1u;
+                    } else {
+                        st_rel(sp, v);
+                    }
                 }
                 own = uni(own);
                 if (own) {

@@ -407,6 +407,7 @@ extern "C" int gpufs_check_invariants(gpufs_ctx* c) {
                     stack.push_back({v, it.lvl - 1, page});
                     continue;
                 }
+                if (v == GFS_LEAF_RESERVED) return -1009;                // a miss claimed the page and never finished
                 uint32_t fr = (uint32_t)(v & 0xFFFFFFFFu) - 1u;
                 uint32_t gen = (uint32_t)(v >> 32);
                 if (fr >= h.n_frames) return -1005;
@@ -435,6 +436,7 @@ extern "C" const char* gpufs_strerror(int code) {
     case -1006: return "invariant: live leaf frame not VALID";
     case -1007: return "invariant: frame key mismatch";
     case -1008: return "invariant: frame shared by two leaves";
+    case -1009: return "invariant: leaf left reserved";
     default: break;
     }
     if (code < 0 && code > -4096) return strerror(-code);
