@@ -764,26 +764,29 @@ __device__ inline void resolve_batch(gfs_dev* g, int fd, uint64_t pb, uint32_t c
             }
         }
     }
-    // A miss first reserves the leaf slot, so of several workgroups missing
-    // the same page only one claims a ring slot and a frame (claims made by
-    // the losers would churn frames and push useful pages out); the others
-    // wait as for a fill in flight.
+    // A miss that needs host data claims a ring slot; it then reserves the
+    // leaf slot, so of several workgroups missing the same page only one
+    // claims a frame (frames claimed by the losers would churn the pool and
+    // push useful pages out); a loser returns its ring slot and waits as for
+    // a fill in flight.  A miss without a ring slot leaves the leaf alone.
+    const bool nohost = ((nohost_mask >> ln) & 1ull) != 0;
+    int s = rpc_claim(g, kind == GFS_K_MISS && !nohost, blockIdx.x * 5u + ln);
     bool resv = false;
-    if (kind == GFS_K_MISS) {
+    if (kind == GFS_K_MISS && (nohost || s >= 0)) {
         uint64_t exp = v;
         resv = cas(sp, exp, GFS_LEAF_RESERVED);
-        if (!resv) kind = GFS_K_WAIT;
     }
-    // misses that need host data claim a ring slot, then a frame
-    const bool nohost = ((nohost_mask >> ln) & 1ull) != 0;
-    int s = rpc_claim(g, resv && !nohost, blockIdx.x * 5u + ln);
-    const bool need_frame = resv && (nohost || s >= 0);
-    uint32_t nf = frame_alloc_wave(g, need_frame, 2);
+    if (kind == GFS_K_MISS && !resv) {
+        if (s >= 0) rpc_release(g, s);
+        s = -1;
+        kind = GFS_K_WAIT;
+    }
+    uint32_t nf = frame_alloc_wave(g, resv, 2);
     if (resv) {
-        if (!need_frame || nf == GFS_NOFRAME) {
+        if (nf == GFS_NOFRAME) {
             if (s >= 0) rpc_release(g, s);
             s = -1;
-            st_rel(sp, v);   // give the slot back as it was
+            st_rel(sp, v);   // give the leaf back as it was
             kind = GFS_K_WAIT;
         } else {
             g->fkey[nf] = gfs_key((uint32_t)fd, page);
