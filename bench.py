@@ -49,7 +49,7 @@ def parse():
     ap.add_argument("--cache-gib", type=float, default=16.0)
     ap.add_argument("--ring-depth", type=int, default=1024)
     ap.add_argument("--host-threads", type=int, default=16)
-    ap.add_argument("--cold-steps", type=int, default=3)
+    ap.add_argument("--cold-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grep", action="store_true")
     ap.add_argument("--batched", action="store_true",
