@@ -137,6 +137,29 @@ __device__ inline void grep_build_dict(grep_lds& S, const char* dict, uint32_t n
 
 // Count tokens of window [0, dsz) of S.corpus.  corpus[dsz-1] is never a word
 // byte, so every token ends inside the window.
+// Probe the dictionary for a token that passed the filter: table slots whose
+// length and first eight bytes agree lead to comparing the remaining words.
+__device__ inline void grep_probe(grep_lds& S, uint32_t p, uint32_t len, uint64_t t0, uint32_t h) {
+    uint32_t pos = h & (GREP_HT - 1);
+    for (;;) {
+        const uint32_t e = S.ht[pos];
+        if (!e) return;
+        if ((e >> 16) == len) {
+            const uint32_t w = (e & 0xFFFFu) - 1;
+            if (S.words[w][0] == t0) {
+                bool eq = true;
+                for (uint32_t i = 1; 8 * i < len; ++i)
+                    eq = eq && S.words[w][i] == keep_bytes(lds_u64_at(S.corpus, p + 8 * i), (int)(len - 8 * i));
+                if (eq) {
+                    atomicAdd(&S.counts[w], 1u);
+                    return;
+                }
+            }
+        }
+        pos = (pos + 1) & (GREP_HT - 1);
+    }
+}
+
 __device__ inline void grep_scan_window(grep_lds& S, uint32_t dsz) {
     typedef __attribute__((address_space(3))) gfs::v4u lv4;
     const uint32_t T = gfs::nthreads(), t = gfs::tid();
@@ -167,29 +190,12 @@ __device__ inline void grep_scan_window(grep_lds& S, uint32_t dsz) {
                 while (q < p + 33 && g_alpha(S.corpus[q])) ++q;
                 len = q - p;
             }
-                if (len > 32 || !((lm >> (len - 1)) & 1u)) continue;
+            if (len > 32 || !((lm >> (len - 1)) & 1u)) continue;
             const uint64_t t0 = keep_bytes(lds_u64_at(S.corpus, p), (int)len);
             const uint32_t h = tok_hash8(t0, len);
             const uint32_t fi = h >> (32 - GREP_FILT_LOG);
             if (!((S.filt[fi >> 5] >> (fi & 31)) & 1u)) continue;
-            uint32_t pos = h & (GREP_HT - 1);
-            for (;;) {
-                const uint32_t e = S.ht[pos];
-                if (!e) break;
-                if ((e >> 16) == len) {
-                    const uint32_t w = (e & 0xFFFFu) - 1;
-                    if (S.words[w][0] == t0) {
-                        bool eq = true;
-                        for (uint32_t i = 1; 8 * i < len; ++i)
-                            eq = eq && S.words[w][i] == keep_bytes(lds_u64_at(S.corpus, p + 8 * i), (int)(len - 8 * i));
-                        if (eq) {
-                            atomicAdd(&S.counts[w], 1u);
-                            break;
-                        }
-                    }
-                }
-                pos = (pos + 1) & (GREP_HT - 1);
-            }
+            grep_probe(S, p, len, t0, h);
         }
     }
 }
