@@ -86,22 +86,19 @@ struct wg_ctl {
 
 // Per-workgroup read history for sequential read-ahead.  It is advisory: it
 // only decides how many pages beyond a read are fetched together with it.
-// LDS keeps whatever the previous kernel on the compute unit left there, so
-// the history carries a tag of the launch and workgroup; a stale match can
-// cost one unneeded prefetch, never a wrong byte.
+// LDS keeps whatever the previous workgroup on the compute unit left there, so
+// gopen clears the history; a read of a descriptor this workgroup did not open
+// may meet a stale entry, which can cost one unneeded prefetch (or skip one),
+// never a wrong byte.  (A tag built from the dispatch and kernel-argument
+// pointers kept four more SGPRs live through every caller's loops and pushed
+// the grep scan's window loop into scratch spills.)
 struct wg_hist {
-    uint64_t tag;
-    int32_t fd;
+    int32_t fd;         // -1: no history
     uint32_t streak;    // forward-sequential reads in a row
     uint64_t first;     // pages of the last read
     uint64_t last;
     uint64_t fetched;   // pages below this were read or fetched ahead
 };
-__device__ __forceinline__ uint64_t launch_tag() {
-    const uint64_t blk = blockIdx.x + (uint64_t)gridDim.x * (blockIdx.y + (uint64_t)gridDim.y * blockIdx.z);
-    return ((uint64_t)(uintptr_t)__builtin_amdgcn_dispatch_ptr() * 0x9E3779B97F4A7C15ull) ^
-           (uint64_t)(uintptr_t)__builtin_amdgcn_kernarg_segment_ptr() ^ (blk << 40);
-}
 
 __device__ __forceinline__ uint32_t tid() {
     return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
@@ -228,13 +225,24 @@ __device__ __forceinline__ void store16(uint8_t* p, v4u v, bool dst_lds, bool nt
     *(gv4u*)p = v;
 }
 
+// Address space of a copy operand, when the caller knows it.
+enum buf_space { SPACE_ANY = 0, SPACE_LDS = 1, SPACE_GLOBAL = 2 };
+
 // The non-temporal choices are template constants: as runtime flags the
-// compiler merges the two load forms and the nontemporal hint is lost.
-template <bool nt_load, bool nt_store>
+// compiler merges the two load forms and the nontemporal hint is lost.  The
+// operand spaces are template constants too: with SPACE_ANY every path below
+// is compiled into the caller, and the global->global body's data registers
+// then count against the caller's loops even where it never runs.  `depth` is
+// the number of 16-byte loads each lane keeps in flight on the global->global
+// path (eight for HBM-to-HBM streams, 4 where a copy sits inside
+// loops that need the registers).
+template <bool nt_load, bool nt_store, buf_space dst_space = SPACE_ANY, buf_space src_space = SPACE_ANY,
+          int depth = 8>
 __device__ inline void copy_seg(uint8_t* dst, const uint8_t* src, uint64_t len, uint32_t j, uint32_t T) {
+    static_assert(depth == 4 || depth == 8, "copy depth");
     if (len == 0) return;
-    const bool dl = is_lds(dst);
-    const bool sl = is_lds(src);
+    const bool dl = dst_space == SPACE_ANY ? is_lds(dst) : dst_space == SPACE_LDS;
+    const bool sl = src_space == SPACE_ANY ? is_lds(src) : src_space == SPACE_LDS;
     const uintptr_t d = (uintptr_t)dst, s = (uintptr_t)src;
     uint64_t head = (16 - (d & 15)) & 15;
     if (head > len) head = len;
@@ -248,18 +256,18 @@ __device__ inline void copy_seg(uint8_t* dst, const uint8_t* src, uint64_t len, 
     if (units == 0) return;
     if (((s ^ d) & 15) == 0) {
         if (!sl && !dl) {
-            // global -> global: eight 16-byte loads in flight per lane
+            // global -> global: `depth` 16-byte loads in flight per lane
             const gv4u* S4 = (const gv4u*)S;
             gv4u* D4 = (gv4u*)D;
             const uint64_t st = T;
             uint64_t u = j;
-            for (; u + 7 * st < units; u += 8 * st) {
-                v4u r[8];
+            for (; u + (depth - 1) * st < units; u += depth * st) {
+                v4u r[depth];
 #pragma unroll
-                for (int k = 0; k < 8; ++k)
+                for (int k = 0; k < depth; ++k)
                     r[k] = nt_load ? __builtin_nontemporal_load(S4 + u + k * st) : S4[u + k * st];
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
+                for (int k = 0; k < depth; ++k) {
                     if (nt_store) __builtin_nontemporal_store(r[k], D4 + u + k * st);
                     else D4[u + k * st] = r[k];
                 }
@@ -926,7 +934,10 @@ __device__ inline void fill_from_host(gfs_dev* g, uint32_t cnt, uint64_t pb, uin
             if (uni((uint32_t)gfs_ctl.kind[i]) != GFS_K_OWN) continue;
             const uint32_t f = gfs_ctl.frame[i];
             const uint64_t bytes = g->fbytes[f];
-            copy_seg<true, false>(frame_ptr(g, f), staging_ptr(g, gfs_ctl.slot[i]), bytes, (t + T - rot) % T, T);
+            // four loads in flight per lane: this copy is inlined into every
+            // read loop, and eight would cost the warm path register spills
+            copy_seg<true, false, SPACE_GLOBAL, SPACE_GLOBAL, 4>(frame_ptr(g, f), staging_ptr(g, gfs_ctl.slot[i]),
+                                                                 bytes, (t + T - rot) % T, T);
             rot = (rot + (uint32_t)(bytes >> 4)) % T;
             filled += bytes;
         }
@@ -956,7 +967,10 @@ __device__ inline void fill_from_host(gfs_dev* g, uint32_t cnt, uint64_t pb, uin
 // libgpufs/fs_calls.cu:227-236
 __device__ inline int gopen(const char* filename, int flags) {
     __syncthreads();
-    if (gfs::tid() == 0) gfs_ctl.ret = gfs::open_single(filename, flags);
+    if (gfs::tid() == 0) {
+        gfs_hist.fd = -1;
+        gfs_ctl.ret = gfs::open_single(filename, flags);
+    }
     __syncthreads();
     return (int)gfs::uni(gfs_ctl.ret);
 }
@@ -969,7 +983,10 @@ __device__ inline size_t fstat(int fd) {
 }
 
 // libgpufs/fs_calls.cu:689-737.  Reads [offset, offset+size) clipped at end of
-// file into `buffer` (global or LDS); returns the bytes read.
+// file into `buffer` (global or LDS); returns the bytes read.  A caller that
+// knows where `buffer` lives may say so (gread<gfs::SPACE_LDS>(...)): only that
+// copy path is then compiled into it.
+template <gfs::buf_space buffer_space = gfs::SPACE_ANY>
 __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer) {
     using namespace gfs;
     gfs_dev* g = gfs_g;
@@ -999,8 +1016,7 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
         if (r > 0) {
             const uint32_t Lg = g->log_page;
             const uint64_t pf = offset >> Lg, pl = (offset + (uint64_t)r - 1) >> Lg;
-            const uint64_t tag = launch_tag();
-            const bool known = gfs_hist.tag == tag && gfs_hist.fd == fd;
+            const bool known = gfs_hist.fd == fd;
             const bool seq = known && pf >= gfs_hist.first && pf <= gfs_hist.last + 1;
             const uint32_t streak = seq ? gfs_hist.streak + 1 : 0;
             uint64_t fetched = known ? gfs_hist.fetched : 0;
@@ -1014,7 +1030,6 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
                 ext = (uint32_t)want;
             }
             if (pl + 1 + ext > fetched) fetched = pl + 1 + ext;
-            gfs_hist.tag = tag;
             gfs_hist.fd = fd;
             gfs_hist.streak = streak;
             gfs_hist.first = pf;
@@ -1066,8 +1081,9 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
                 uint64_t poff = (pb + i) << L;
                 uint64_t lo = poff > offset ? poff : offset;
                 uint64_t hi = poff + PS < offset + n ? poff + PS : offset + n;
-                copy_seg<true, true>(buffer + (lo - offset), frame_ptr(g, gfs_ctl.frame[i]) + (lo - poff), hi - lo,
-                                     (t + T - rot) % T, T);
+                copy_seg<true, true, buffer_space, SPACE_GLOBAL>(buffer + (lo - offset),
+                                                                 frame_ptr(g, gfs_ctl.frame[i]) + (lo - poff), hi - lo,
+                                                                 (t + T - rot) % T, T);
                 rot = (rot + (uint32_t)((hi - lo) >> 4)) % T;
                 // keep the workgroup's waves on the same page (streams the pool in order)
                 if (page_sync) __syncthreads();
@@ -1166,7 +1182,8 @@ __device__ inline size_t gwrite(int fd, size_t offset, size_t size, uchar* buffe
                 uint64_t poff = (pb + i) << L;
                 uint64_t lo = poff > offset ? poff : offset;
                 uint64_t hi = poff + PS < offset + n ? poff + PS : offset + n;
-                copy_seg<false, false>(frame_ptr(g, gfs_ctl.frame[i]) + (lo - poff), buffer + (lo - offset), hi - lo, t, T);
+                copy_seg<false, false, SPACE_GLOBAL>(frame_ptr(g, gfs_ctl.frame[i]) + (lo - poff), buffer + (lo - offset),
+                                                     hi - lo, t, T);
                 if (t == 0) atomicMax(&g->fbytes[gfs_ctl.frame[i]], (uint32_t)(hi - poff));
             }
             __threadfence();

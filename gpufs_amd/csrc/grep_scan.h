@@ -218,7 +218,11 @@ __device__ inline int grep_file_windows(grep_lds& S, int fd, uint32_t chunk, OnW
     while (cursor < size) {
         const bool last = size - cursor < (uint64_t)chunk + 32;
         const uint32_t left = last ? (uint32_t)(size - cursor) : chunk + 32;
-        size_t got = gread(fd, cursor, left, S.corpus);
+        size_t got;
+        // Inlined here: as an outlined call, gread's prologue and epilogue save and
+        // restore about fifty registers through scratch on every window.  The
+        // window is in LDS, so only the global->LDS copy path is compiled in.
+        [[clang::always_inline]] got = gread<gfs::SPACE_LDS>(fd, cursor, left, S.corpus);
         if (got != left) {
             if (t == 0) S.status = (int64_t)got < 0 ? (int)(int64_t)got : -EIO;
             break;

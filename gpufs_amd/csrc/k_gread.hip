@@ -22,7 +22,8 @@ __global__ __launch_bounds__(1024) void k_gread_ranges(const char* path, uint64_
         if (gfs::tid() == 0) out[blockIdx.x] = fd;
         return;
     }
-    size_t r = gread(fd, (size_t)blockIdx.x * range, range, dst + (size_t)blockIdx.x * range);
+    size_t r;
+    [[clang::always_inline]] r = gread(fd, (size_t)blockIdx.x * range, range, dst + (size_t)blockIdx.x * range);
     int c = gclose(fd);
     if (gfs::tid() == 0) out[blockIdx.x] = (int64_t)r < 0 ? (int64_t)r : (c < 0 ? (int64_t)c : (int64_t)r);
 }

@@ -28,7 +28,8 @@ __global__ __launch_bounds__(1024) void k_grep(const char* names, uint32_t strid
     __shared__ __attribute__((aligned(16))) grep_lds S;
     const uint32_t f = blockIdx.x;
     grep_build_dict(S, dict, nw);
-    int st = grep_one_file(S, names + (size_t)f * stride, nw, chunk, counts_out + (size_t)f * nw);
+    int st;
+    [[clang::always_inline]] st = grep_one_file(S, names + (size_t)f * stride, nw, chunk, counts_out + (size_t)f * nw);
     if (gfs::tid() == 0) status[f] = st;
 }
 
@@ -48,7 +49,9 @@ __global__ __launch_bounds__(1024) void k_grep_batched(const char* names, uint32
         __syncthreads();
         if (f >= nfiles) break;
         const uint64_t t0 = gfs::now_ticks();
-        int st = grep_one_file(S, names + (size_t)f * stride, nw, chunk, counts_out + (size_t)f * nw);
+        int st;
+        [[clang::always_inline]] st = grep_one_file(S, names + (size_t)f * stride, nw, chunk,
+                                                    counts_out + (size_t)f * nw);
         if (gfs::tid() == 0) {
             status[f] = st;
             if (times) {
