@@ -32,6 +32,10 @@
 #define GFS_NAME_MAX 128
 #define GFS_NAME_RESERVED 0xFFFFFFFFu   // name-index slot taken, entry not yet published
 #define GFS_LEAF_RESERVED (~0ull)       // radix leaf slot claimed by a workgroup, frame not yet published          // bytes incl. NUL (reference: FILENAME_SIZE 64)
+// Ring slot bitmap words sit 128 bytes apart: every claim and release is an
+// atomic on them, and packed into one cache line they all queue at one L2
+// channel (which also carries its share of every other stream).
+#define GFS_SLOT_WORD_STRIDE 16u
 #define GFS_FANOUT_LOG 6          // radix fan-out 64 = one wave64 lane per slot
 #define GFS_FANOUT 64
 #define GFS_MAX_HEIGHT 7          // 64^7 pages
@@ -127,7 +131,7 @@ typedef struct gfs_dev {
     uint64_t* nodes;        // radix nodes, GFS_FANOUT words each; node 0 is null
     gfs_file* files;
     uint32_t* name_index;   // open addressing, value = file index + 1
-    uint64_t* slot_free;    // ring slot bitmap (1 = free)
+    uint64_t* slot_free;    // ring slot bitmap (1 = free); word w at [w * GFS_SLOT_WORD_STRIDE]
     // RPC ring (coherent pinned host memory, device-visible pointers)
     gfs_rpc_slot* slots;
     uint64_t* sq;           // submission queue: (ticket << 16) | slot

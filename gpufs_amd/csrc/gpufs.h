@@ -448,24 +448,97 @@ __device__ __forceinline__ void frame_release(gfs_dev* g, uint32_t f) {
 // ======================================================================
 // RPC client
 // ======================================================================
+__device__ __forceinline__ uint64_t* slot_word(gfs_dev* g, uint32_t w) {
+    return &g->slot_free[(size_t)w * GFS_SLOT_WORD_STRIDE];
+}
+// One ring slot for one thread, or -1 when none is free.
 __device__ inline int rpc_claim(gfs_dev* g, bool need, uint32_t hint) {
     if (!need) return -1;
     const uint32_t W = g->slot_words;
     for (uint32_t a = 0; a < W; ++a) {
         uint32_t w = (hint + a) % W;
-        uint64_t x = ld_rlx(&g->slot_free[w]);
+        uint64_t* wp = slot_word(g, w);
+        uint64_t x = ld_rlx(wp);
         while (x) {
             uint32_t b = first_lane(x);
             uint64_t bit = 1ull << b;
-            uint64_t old = __hip_atomic_fetch_and(&g->slot_free[w], ~bit, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            uint64_t old = __hip_atomic_fetch_and(wp, ~bit, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
             if (old & bit) return (int)(w * 64 + b);
             x = old & ~bit;
         }
     }
     return -1;
 }
+// The lowest k set bits of x.
+__device__ __forceinline__ uint64_t low_bits(uint64_t x, uint32_t k) {
+    if (popc(x) <= k) return x;
+    uint64_t m = 0;
+    for (uint32_t i = 0; i < k; ++i) {
+        const uint64_t b = x & (0ull - x);
+        m |= b;
+        x ^= b;
+    }
+    return m;
+}
+// Wave-collective claim (call with all 64 lanes): every lane with `need` gets
+// a slot, or -1 once none is left; slots go to the needing lanes in lane
+// order.  Lane 0 scans the bitmap and takes all the bits the wave wants from
+// a word with one atomic, so a wave costs one load and about one atomic per
+// word it visits instead of one per lane and word.
+__device__ inline int rpc_claim_wave(gfs_dev* g, bool need, uint32_t hint) {
+    const uint64_t needm = __ballot(need);
+    if (!needm) return -1;
+    const uint32_t ln = lane();
+    const uint32_t rank = mbcnt(needm);
+    const uint32_t want = popc(needm);
+    const uint32_t W = g->slot_words;
+    uint32_t base = 0;
+    int mine = -1;
+    for (uint32_t a = 0; a < W && base < want; ++a) {
+        const uint32_t w = (hint + a) % W;
+        uint64_t* wp = slot_word(g, w);
+        uint64_t x = 0;
+        if (ln == 0) x = ld_rlx(wp);
+        x = __shfl(x, 0);
+        while (x && base < want) {
+            const uint64_t m = low_bits(x, want - base);
+            uint64_t old = 0;
+            if (ln == 0) old = __hip_atomic_fetch_and(wp, ~m, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            old = __shfl(old, 0);
+            const uint64_t got = old & m;
+            const uint32_t c = popc(got);
+            if (need && rank >= base && rank < base + c) {
+                uint64_t t = got;
+                for (uint32_t i = base; i < rank; ++i) t &= t - 1;
+                mine = (int)(w * 64 + first_lane(t));
+            }
+            base += c;
+            x = old & ~m;   // bits still free that this round did not ask for
+        }
+    }
+    return mine;
+}
 __device__ __forceinline__ void rpc_release(gfs_dev* g, int s) {
-    __hip_atomic_fetch_or(&g->slot_free[s >> 6], 1ull << (s & 63), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_or(slot_word(g, (uint32_t)s >> 6), 1ull << (s & 63), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Wave-collective release (call with all 64 lanes): the slots of the lanes
+// with `give` are returned with one atomic per bitmap word.
+__device__ inline void rpc_release_wave(gfs_dev* g, bool give, int s) {
+    uint64_t pend = __ballot(give);
+    while (pend) {
+        const uint32_t w = (uint32_t)__shfl(s, (int)first_lane(pend)) >> 6;
+        const bool mine = ((pend >> lane()) & 1ull) && ((uint32_t)s >> 6) == w;
+        uint64_t v = mine ? (1ull << (s & 63)) : 0ull;
+        v |= __shfl_xor(v, 1);
+        v |= __shfl_xor(v, 2);
+        v |= __shfl_xor(v, 4);
+        v |= __shfl_xor(v, 8);
+        v |= __shfl_xor(v, 16);
+        v |= __shfl_xor(v, 32);
+        if (lane() == first_lane(pend))
+            __hip_atomic_fetch_or(slot_word(g, w), v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        pend &= ~__ballot(mine);
+    }
 }
 // Fill the request fields of slot s and publish it to the host.  `cq` is the
 // completion word index | tag << 16: when done the host sets bit `tag` of it.
@@ -778,7 +851,7 @@ __device__ inline void resolve_batch(gfs_dev* g, int fd, uint64_t pb, uint32_t c
     // push useful pages out); a loser returns its ring slot and waits as for
     // a fill in flight.  A miss without a ring slot leaves the leaf alone.
     const bool nohost = ((nohost_mask >> ln) & 1ull) != 0;
-    int s = rpc_claim(g, kind == GFS_K_MISS && !nohost, blockIdx.x * 5u + ln);
+    int s = rpc_claim_wave(g, kind == GFS_K_MISS && !nohost, blockIdx.x * 5u);
     bool resv = false;
     if (kind == GFS_K_MISS && (nohost || s >= 0)) {
         uint64_t exp = v;
@@ -911,6 +984,7 @@ __device__ inline void fill_from_host(gfs_dev* g, uint32_t cnt, uint64_t pb, uin
             gfs_ctl.a = m & own & ~done;
         }
         __syncthreads();
+        GFS_TRACE(7);
         const uint64_t got = uni(gfs_ctl.a);
         const bool tmo = uni(gfs_ctl.tmo) != 0u;
         if (wave() == 0) {
@@ -943,10 +1017,11 @@ __device__ inline void fill_from_host(gfs_dev* g, uint32_t cnt, uint64_t pb, uin
         }
         // every staging load of the round has returned (its data has been stored)
         __syncthreads();
+        GFS_TRACE(12);
         if (wave() == 0) {
             const uint32_t ln = lane();
             const bool give = ((got >> ln) & 1ull) && gfs_ctl.kind[ln] == GFS_K_OWN && gfs_ctl.slot[ln] != cq;
-            if (give) rpc_release(g, (int)gfs_ctl.slot[ln]);
+            rpc_release_wave(g, give, give ? (int)gfs_ctl.slot[ln] : 0);
             const uint64_t gm = __ballot(give);
             if (ln == 0) gfs_ctl.rel |= gm;
         }

@@ -32,6 +32,9 @@
 #include <time.h>
 #include <unistd.h>
 #include "gfs_layout.h"
+#if defined(__x86_64__)
+#include <emmintrin.h>
+#endif
 
 namespace gfs_host {
 
@@ -81,6 +84,58 @@ static inline int64_t pwrite_full(int fd, const uint8_t* buf, uint64_t len, uint
     return (int64_t)done;
 }
 
+// Per-worker buffers: the OR-merge's copy of the disk bytes, and the bounce
+// buffer of streamed page reads.
+struct WorkerBufs {
+    std::vector<uint8_t> merge;
+    std::vector<uint8_t> bounce;
+};
+
+// Page reads land in the staging page through a small bounce buffer that
+// stays in the core's cache: pread() copies the file bytes there, and they
+// are streamed on to staging with non-temporal stores.  The staging lines then
+// sit in DRAM, not dirty in a CPU cache, when the GPU reads them over PCIe
+// (measured on MI355X: cold sequential reads about 2.5 % faster than
+// pread() straight into staging, with less host time per page).
+static constexpr uint64_t kBounceBytes = 64 << 10;
+static inline void stream_copy(uint8_t* dst, const uint8_t* src, uint64_t n) {
+    uint64_t i = 0;
+#if defined(__x86_64__)
+    if ((((uintptr_t)dst | (uintptr_t)src) & 15) == 0) {
+        for (; i + 64 <= n; i += 64) {
+            const __m128i a = _mm_load_si128((const __m128i*)(src + i));
+            const __m128i b = _mm_load_si128((const __m128i*)(src + i + 16));
+            const __m128i c = _mm_load_si128((const __m128i*)(src + i + 32));
+            const __m128i d = _mm_load_si128((const __m128i*)(src + i + 48));
+            _mm_stream_si128((__m128i*)(dst + i), a);
+            _mm_stream_si128((__m128i*)(dst + i + 16), b);
+            _mm_stream_si128((__m128i*)(dst + i + 32), c);
+            _mm_stream_si128((__m128i*)(dst + i + 48), d);
+        }
+    }
+#endif
+    if (i < n) memcpy(dst + i, src + i, n - i);
+}
+// pread of [off, off+len) into `stage`: the bytes read, or -errno.  The
+// streamed stores are fenced before the caller publishes the result.
+static inline int64_t pread_staged(int fd, uint8_t* stage, uint64_t len, uint64_t off, std::vector<uint8_t>& bounce) {
+    if (bounce.size() < kBounceBytes + 64) bounce.resize(kBounceBytes + 64);
+    uint8_t* b = (uint8_t*)(((uintptr_t)bounce.data() + 63) & ~(uintptr_t)63);
+    uint64_t done = 0;
+    while (done < len) {
+        const uint64_t want = len - done < kBounceBytes ? len - done : kBounceBytes;
+        const int64_t r = pread_full(fd, b, want, off + done);
+        if (r < 0) return r;
+        stream_copy(stage + done, b, (uint64_t)r);
+        done += (uint64_t)r;
+        if ((uint64_t)r < want) break;   // end of file
+    }
+#if defined(__x86_64__)
+    _mm_sfence();
+#endif
+    return (int64_t)done;
+}
+
 // O_GWRONCE write-back: new bytes are OR-ed with the bytes already on disk
 // (libgpufs/host_loop.h:76-137).  The reference ORs whole 16-byte vectors
 // against a buffer that may hold stale bytes past the read; here exactly the
@@ -95,8 +150,7 @@ static inline int64_t diff_merge_write(int fd, uint8_t* page, uint64_t len, uint
 }
 
 // Serve one request in slot s.  Returns the result stored in the slot.
-static inline int64_t serve_slot(const RingView& rv, uint32_t s, ServiceCounters& ctr,
-                                 std::vector<uint8_t>& tmp) {
+static inline int64_t serve_slot(const RingView& rv, uint32_t s, ServiceCounters& ctr, WorkerBufs& wb) {
     gfs_rpc_slot* r = &rv.slots[s];
     const uint32_t op = r->op;
     const int fd = r->host_fd;
@@ -105,7 +159,7 @@ static inline int64_t serve_slot(const RingView& rv, uint32_t s, ServiceCounters
     int64_t res = -EINVAL;
     switch (op) {
     case GFS_OP_READ:
-        res = pread_full(fd, stage, len, off);
+        res = pread_staged(fd, stage, len, off, wb.bounce);
         if (res > 0) ctr.bytes_read += (uint64_t)res;
         break;
     case GFS_OP_WRITE:
@@ -113,7 +167,7 @@ static inline int64_t serve_slot(const RingView& rv, uint32_t s, ServiceCounters
         if (res > 0) ctr.bytes_written += (uint64_t)res;
         break;
     case GFS_OP_DIFF:
-        res = diff_merge_write(fd, stage, len, off, tmp);
+        res = diff_merge_write(fd, stage, len, off, wb.merge);
         if (res > 0) ctr.bytes_written += (uint64_t)res;
         break;
     case GFS_OP_OPEN: {
@@ -206,14 +260,14 @@ static inline bool backlog(const RingView& rv, const std::atomic<uint64_t>& head
 static inline void worker_loop(RingView rv, std::atomic<uint64_t>* head, std::atomic<bool>* stop,
                                ServiceCounters* ctr, WakeState* wk, unsigned index) {
     prctl(PR_SET_TIMERSLACK, 1000UL, 0, 0, 0);
-    std::vector<uint8_t> tmp;
+    WorkerBufs wb;
     uint64_t idle = 0;
     while (!stop->load(std::memory_order_relaxed)) {
         uint32_t s;
         if (try_take(rv, *head, &s)) {
             if (wk->sleepers.load(std::memory_order_relaxed) && backlog(rv, *head)) wake_all(wk);
             const uint64_t t0 = mono_ns();
-            serve_slot(rv, s, *ctr, tmp);
+            serve_slot(rv, s, *ctr, wb);
             ctr->busy_ns.fetch_add(mono_ns() - t0, std::memory_order_relaxed);
             idle = 0;
             continue;

@@ -183,7 +183,7 @@ extern "C" int gpufs_init(const gpufs_config* cfg_in, gpufs_ctx** out) {
     TRY_ALLOC(hipMalloc(&h.nodes, (size_t)h.node_cap * GFS_FANOUT * sizeof(uint64_t)));
     TRY_ALLOC(hipMalloc(&h.files, sizeof(gfs_file) * (size_t)h.max_files));
     TRY_ALLOC(hipMalloc(&h.name_index, sizeof(uint32_t) * (size_t)h.name_cap));
-    TRY_ALLOC(hipMalloc(&h.slot_free, sizeof(uint64_t) * h.slot_words));
+    TRY_ALLOC(hipMalloc(&h.slot_free, sizeof(uint64_t) * h.slot_words * GFS_SLOT_WORD_STRIDE));
     const unsigned hflags = hipHostMallocMapped | hipHostMallocCoherent;
     TRY_ALLOC(hipHostMalloc((void**)&h.slots, sizeof(gfs_rpc_slot) * h.ring_depth, hflags));
     TRY_ALLOC(hipHostMalloc((void**)&h.sq, sizeof(uint64_t) * h.sq_depth, hflags));
@@ -193,8 +193,8 @@ extern "C" int gpufs_init(const gpufs_config* cfg_in, gpufs_ctx** out) {
     memset(h.sq, 0xFF, sizeof(uint64_t) * h.sq_depth);
     memset(h.cq, 0, sizeof(uint64_t) * h.ring_depth);
     {
-        std::vector<uint64_t> bits(h.slot_words, 0);
-        for (uint32_t s = 0; s < h.ring_depth; ++s) bits[s >> 6] |= 1ull << (s & 63);
+        std::vector<uint64_t> bits((size_t)h.slot_words * GFS_SLOT_WORD_STRIDE, 0);
+        for (uint32_t s = 0; s < h.ring_depth; ++s) bits[(size_t)(s >> 6) * GFS_SLOT_WORD_STRIDE] |= 1ull << (s & 63);
         TRY_ALLOC(hipMemcpy(h.slot_free, bits.data(), bits.size() * 8, hipMemcpyHostToDevice));
     }
     TRY_ALLOC(hipMemset(h.files, 0, sizeof(gfs_file) * (size_t)h.max_files));
@@ -381,6 +381,15 @@ extern "C" int gpufs_check_invariants(gpufs_ctx* c) {
     uint32_t nf = (uint32_t)std::min<unsigned long long>(cur.files_next, h.max_files);
     std::vector<gfs_file> files(nf);
     if (nf) HIP_TRY(hipMemcpy(files.data(), h.files, sizeof(gfs_file) * nf, hipMemcpyDeviceToHost));
+    {
+        // every ring slot is free (a request that timed out may keep its slot:
+        // the host can still write its staging page)
+        std::vector<uint64_t> bits((size_t)h.slot_words * GFS_SLOT_WORD_STRIDE);
+        HIP_TRY(hipMemcpy(bits.data(), h.slot_free, bits.size() * 8, hipMemcpyDeviceToHost));
+        uint64_t nfree = 0;
+        for (uint32_t w = 0; w < h.slot_words; ++w) nfree += (uint64_t)__builtin_popcountll(bits[(size_t)w * GFS_SLOT_WORD_STRIDE]);
+        if (cur.ctr.timeouts == 0 && nfree != h.ring_depth) return -1010;
+    }
     for (uint32_t f = 0; f < h.n_frames; ++f)
         if (meta[f] & GFS_REF_MASK) return -1001;   // a frame is still pinned
     for (uint32_t f = 0; f < h.n_frames; ++f)
@@ -437,6 +446,7 @@ extern "C" const char* gpufs_strerror(int code) {
     case -1007: return "invariant: frame key mismatch";
     case -1008: return "invariant: frame shared by two leaves";
     case -1009: return "invariant: leaf left reserved";
+    case -1010: return "invariant: ring slot not returned";
     default: break;
     }
     if (code < 0 && code > -4096) return strerror(-code);

@@ -186,6 +186,39 @@ int main(int argc, char** argv) {
     svc.stop();
     CHECK(svc.counters().requests.load() == (uint64_t)(2 + 3600 + batch_pages + 7), "request count %llu",
           (unsigned long long)svc.counters().requests.load());
+    // 256 KiB pages: reads longer than the service's 64 KiB bounce buffer,
+    // at any offset, ending at or past end of file
+    {
+        const uint32_t big_log = 18, big_depth = 4;
+        Ring big(big_depth, big_log);
+        gfs_host::Service bsvc;
+        bsvc.start(big.rv, 2);
+        int so = big.claim();
+        const int64_t bfd = big.call(so, GFS_OP_OPEN, -1, O_RDONLY, 0, 0, path.c_str());
+        CHECK(bfd >= 0, "open big");
+        big.release(so);
+        std::vector<std::thread> bc;
+        for (int c = 0; c < 3; ++c) {
+            bc.emplace_back([&, c]() {
+                std::mt19937_64 r(500 + c);
+                for (int k = 0; k < 60; ++k) {
+                    int s;
+                    while ((s = big.claim()) < 0) std::this_thread::yield();
+                    const uint64_t off = k % 5 == 0 ? data.size() - (r() % 70000) : r() % (data.size() + 100);
+                    const uint64_t len = 1 + r() % (1u << big_log);
+                    const int64_t got = big.call(s, GFS_OP_READ, (int)bfd, 0, off, len);
+                    const uint64_t exp = off >= data.size() ? 0 : std::min<uint64_t>(len, data.size() - off);
+                    CHECK(got == (int64_t)exp, "big read off=%llu len=%llu got=%lld", (unsigned long long)off,
+                          (unsigned long long)len, (long long)got);
+                    if (got > 0) CHECK(memcmp(&big.staging[(size_t)s << big_log], &data[off], (size_t)got) == 0, "big bytes");
+                    big.release(s);
+                }
+            });
+        }
+        for (auto& t : bc) t.join();
+        bsvc.stop();
+        CHECK(big.free_slots.size() == big_depth, "big slots conserved");
+    }
     unlink(path.c_str()); unlink(wpath.c_str());
     if (failures) { fprintf(stderr, "%d failures\n", failures); return 1; }
     printf("ring model OK\n");
