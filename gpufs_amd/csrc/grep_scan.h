@@ -135,8 +135,6 @@ __device__ inline void grep_build_dict(grep_lds& S, const char* dict, uint32_t n
     __syncthreads();
 }
 
-// Count tokens of window [0, dsz) of S.corpus.  corpus[dsz-1] is never a word
-// byte, so every token ends inside the window.
 // Probe the dictionary for a token that passed the filter: table slots whose
 // length and first eight bytes agree lead to comparing the remaining words.
 __device__ inline void grep_probe(grep_lds& S, uint32_t p, uint32_t len, uint64_t t0, uint32_t h) {
@@ -160,35 +158,55 @@ __device__ inline void grep_probe(grep_lds& S, uint32_t p, uint32_t len, uint64_
     }
 }
 
-__device__ inline void grep_scan_window(grep_lds& S, uint32_t dsz) {
+// Word-byte mask of the 16*nq bytes at `base` (nq quads), cut at the window end.
+__device__ inline uint64_t grep_seg_mask(const uint8_t* corpus, uint32_t base, uint32_t dsz, int nq) {
     typedef __attribute__((address_space(3))) gfs::v4u lv4;
-    const uint32_t T = gfs::nthreads(), t = gfs::tid();
+    const lv4* src = (const lv4*)(corpus + base);
+    uint64_t am = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k >= nq) break;
+        gfs::v4u v = src[k];
+        am |= ((uint64_t)alpha4(v.x) | ((uint64_t)alpha4(v.y) << 4) | ((uint64_t)alpha4(v.z) << 8) |
+               ((uint64_t)alpha4(v.w) << 12)) << (16 * k);
+    }
+    const uint32_t valid = dsz - base;
+    if (valid < 64) am &= (1ull << valid) - 1;
+    return am;
+}
+
+// Count tokens of window [0, dsz) of S.corpus.  corpus[dsz-1] is never a word
+// byte, so every token ends inside the window.
+// Each lane classifies one 64-byte segment and walks its tokens.  The loop is
+// wave-uniform so that lanes can trade masks: a token that runs past its
+// segment's end takes the rest of its length from the next segment's mask (the
+// neighbour lane's, or lane 63's own classification of the next 32 bytes) and
+// the byte before a segment comes from the previous lane's mask.  A byte-wise
+// LDS walk there (one load and wait per byte, taken by almost every lane in the
+// same iteration) cost a third of the scan.
+__device__ inline void grep_scan_window(grep_lds& S, uint32_t dsz) {
+    const uint32_t T = gfs::nthreads(), t = gfs::tid(), ln = gfs::lane();
     const uint32_t nseg = (dsz + 63) >> 6;
     const uint32_t lm = S.lenmask;
-    for (uint32_t sg = t; sg < nseg; sg += T) {
+    for (uint32_t sg0 = t - ln; sg0 < nseg; sg0 += T) {
+        const uint32_t sg = sg0 + ln;
         const uint32_t base = sg << 6;
-        const lv4* src = (const lv4*)(S.corpus + base);
-        uint64_t am = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            gfs::v4u v = src[k];
-            am |= ((uint64_t)alpha4(v.x) | ((uint64_t)alpha4(v.y) << 4) | ((uint64_t)alpha4(v.z) << 8) |
-                   ((uint64_t)alpha4(v.w) << 12)) << (16 * k);
-        }
-        const uint32_t valid = dsz - base;
-        if (valid < 64) am &= (1ull << valid) - 1;
-        const uint64_t prev = (base > 0 && g_alpha(S.corpus[base - 1])) ? 1ull : 0ull;
-        uint64_t starts = am & ~((am << 1) | prev);
+        const uint64_t am = sg < nseg ? grep_seg_mask(S.corpus, base, dsz, 4) : 0ull;
+        uint64_t nx = __shfl_down(am, 1);
+        uint64_t pv = __shfl_up(am, 1) >> 63;
+        if (sg >= nseg) continue;
+        if (ln == 63) nx = sg + 1 < nseg ? grep_seg_mask(S.corpus, base + 64, dsz, 2) : 0ull;
+        if (ln == 0) pv = (base > 0 && g_alpha(S.corpus[base - 1])) ? 1ull : 0ull;
+        uint64_t starts = am & ~((am << 1) | pv);
         while (starts) {
             const uint32_t b = (uint32_t)__ffsll((unsigned long long)starts) - 1;
             starts &= starts - 1;
             const uint32_t p = base + b;
             const uint64_t rest = ~(am >> b);
             uint32_t len = rest ? (uint32_t)__ffsll((unsigned long long)rest) - 1 : 64;
-            if (b + len >= 64) {   // the token runs past the segment: count on in LDS (up to 33)
-                uint32_t q = base + 64;
-                while (q < p + 33 && g_alpha(S.corpus[q])) ++q;
-                len = q - p;
+            if (b + len >= 64) {   // the token runs past the segment: its length continues in the next mask
+                const uint64_t nr = ~nx;
+                len = (64 - b) + (nr ? (uint32_t)__ffsll((unsigned long long)nr) - 1 : 64);
             }
             if (len > 32 || !((lm >> (len - 1)) & 1u)) continue;
             const uint64_t t0 = keep_bytes(lds_u64_at(S.corpus, p), (int)len);
