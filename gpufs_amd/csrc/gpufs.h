@@ -82,6 +82,8 @@ struct wg_ctl {
     uint32_t rot;
     uint32_t hits, misses;   // running page-lookup counts of this workgroup (wraps; callers take differences)
     uint32_t ra;             // gread: extra pages fetched after the request (read-ahead)
+    uint32_t gen[64];        // resolve_batch: generation of each HIT / OWN frame
+    uint32_t memo_f;         // gread: frame pinned through the page memo at entry, or GFS_NOFRAME
 };
 
 // Per-workgroup read history for sequential read-ahead.  It is advisory: it
@@ -92,6 +94,20 @@ struct wg_ctl {
 // never a wrong byte.  (A tag built from the dispatch and kernel-argument
 // pointers kept four more SGPRs live through every caller's loops and pushed
 // the grep scan's window loop into scratch spills.)
+// The page the workgroup's last gread ended in: a following read inside the
+// same page pins that frame in gread's first round trip and skips the radix
+// walk (a grep-style loop reads a 256 KiB page in four 64 KiB windows).  The
+// pin only counts if the frame still carries `gen` and is VALID and the file
+// still has `version`: a frame that was evicted or reassigned has a new
+// generation, and truncation or a reopen after the file changed bumps the
+// version.  gopen clears it.
+struct wg_memo {
+    int32_t fd;         // -1: none
+    uint32_t frame;
+    uint32_t gen;
+    uint32_t version;
+    uint64_t page;
+};
 struct wg_hist {
     int32_t fd;         // -1: no history
     uint32_t streak;    // forward-sequential reads in a row
@@ -775,6 +791,7 @@ __device__ inline int open_single(const char* name, int flags) {
 
 static __shared__ gfs::wg_ctl gfs_ctl;
 static __shared__ gfs::wg_hist gfs_hist;
+static __shared__ gfs::wg_memo gfs_memo;
 
 namespace gfs {
 
@@ -824,7 +841,7 @@ __device__ inline void resolve_batch(gfs_dev* g, int fd, uint64_t pb, uint32_t c
     }
     uint64_t node = ((page >> GFS_FANOUT_LOG) == leaf0) ? n0 : n1;
     uint64_t* sp = &g->nodes[node * GFS_FANOUT + (page & (GFS_FANOUT - 1))];
-    uint32_t kind = GFS_K_DONE, f = GFS_NOFRAME;
+    uint32_t kind = GFS_K_DONE, f = GFS_NOFRAME, fgen = 0;
     uint64_t v = 0;
     if (mine) {
         v = ld_rlx(sp);
@@ -837,6 +854,7 @@ __device__ inline void resolve_batch(gfs_dev* g, int fd, uint64_t pb, uint32_t c
             if (meta_gen(old) == vg && meta_state(old) == GFS_F_VALID) {
                 kind = GFS_K_HIT;
                 f = vf;
+                fgen = vg;
                 g->fref[vf] = 1;   // CLOCK reference bit: a store, no load to wait for
                 if (for_write) __hip_atomic_fetch_or(&g->meta[vf], GFS_FL_DIRTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else {
@@ -876,6 +894,7 @@ __device__ inline void resolve_batch(gfs_dev* g, int fd, uint64_t pb, uint32_t c
             st_rel(sp, leaf_pack(nf, gen));   // the reservation is ours: publish the frame
             kind = GFS_K_OWN;
             f = nf;
+            fgen = gen;
             g->fref[nf] = 1;
         }
     }
@@ -890,6 +909,7 @@ __device__ inline void resolve_batch(gfs_dev* g, int fd, uint64_t pb, uint32_t c
     if (ln < 64) {
         gfs_ctl.kind[ln] = (uint8_t)kind;
         gfs_ctl.frame[ln] = f;
+        gfs_ctl.gen[ln] = fgen;
         gfs_ctl.slot[ln] = (kind == GFS_K_OWN && s >= 0) ? (uint32_t)s : 0xFFFFFFFFu;
     }
     // publish read requests for OWN pages that need host data
@@ -1044,6 +1064,7 @@ __device__ inline int gopen(const char* filename, int flags) {
     __syncthreads();
     if (gfs::tid() == 0) {
         gfs_hist.fd = -1;
+        gfs_memo.fd = -1;
         gfs_ctl.ret = gfs::open_single(filename, flags);
     }
     __syncthreads();
@@ -1067,21 +1088,36 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
     gfs_dev* g = gfs_g;
     __syncthreads();
     GFS_TRACE(5);
+    uint32_t version = 0;   // thread 0: the file's version at entry
     if (tid() == 0) {
         // state, size and radix root in one round trip (relaxed loads, one
         // acquire fence); the caller's gopen already acquired what the opener
-        // published
+        // published.  A read inside the page of the workgroup's previous read
+        // also pins that page's frame in the same round trip (wg_memo).
         int64_t r = 0;
         uint64_t fs = 0, root = 0;
+        uint32_t mf = GFS_NOFRAME;
         if (!fd_valid(g, fd)) r = -EBADF;
         else {
             gfs_file* e = &g->files[fd];
+            const uint64_t pin = g->page_size - (offset & (g->page_size - 1));
+            const bool try_memo = gfs_memo.fd == fd && size > 0 && size <= pin && (offset >> g->log_page) == gfs_memo.page;
             const uint64_t st = ld_rlx(&e->st);
             fs = e->size;
             root = ld_rlx(&e->root);
-            fence_acq_agent();   // the nodes under `root` are read after it
+            uint64_t mm = 0;
+            if (try_memo) mm = fadd_rlx(&g->meta[gfs_memo.frame], 1ull);
+            fence_acq_agent();   // the nodes under `root` (and the pinned frame) are read after it
+            version = fst_version(st);
             if (fst_state(st) != GFS_FS_OPEN) r = -EBADF;
             else r = offset >= fs ? 0 : (int64_t)(size < fs - offset ? size : fs - offset);
+            if (try_memo) {
+                if (r > 0 && version == gfs_memo.version && meta_gen(mm) == gfs_memo.gen &&
+                    meta_state(mm) == GFS_F_VALID)
+                    mf = gfs_memo.frame;
+                else
+                    fadd_rlx(&g->meta[gfs_memo.frame], ~0ull);
+            }
         }
         // read-ahead: the third and later forward-sequential reads of a file
         // that reach a page not fetched yet take up to GFS_READAHEAD_BYTES of
@@ -1095,7 +1131,7 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
             const bool seq = known && pf >= gfs_hist.first && pf <= gfs_hist.last + 1;
             const uint32_t streak = seq ? gfs_hist.streak + 1 : 0;
             uint64_t fetched = known ? gfs_hist.fetched : 0;
-            if (streak >= 2 && pl >= fetched && !(g->tune_flags & GFS_TUNE_NO_READAHEAD)) {
+            if (mf == GFS_NOFRAME && streak >= 2 && pl >= fetched && !(g->tune_flags & GFS_TUNE_NO_READAHEAD)) {
                 const uint64_t npages = (fs + g->page_size - 1) >> Lg;
                 uint64_t want = ((uint64_t)GFS_READAHEAD_BYTES) >> Lg;
                 if (want < 1) want = 1;
@@ -1115,12 +1151,14 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
         gfs_ctl.a = r > 0 ? fs : 0;
         gfs_ctl.b = r > 0 ? root : 0;
         gfs_ctl.ra = ext;
+        gfs_ctl.memo_f = mf;
     }
     __syncthreads();
     const int64_t n = uni(gfs_ctl.ret);
     const uint64_t fsize = uni(gfs_ctl.a);
     const uint64_t root_hint = uni(gfs_ctl.b);
     const uint32_t ra_ext = uni(gfs_ctl.ra);
+    uint32_t memo_f = uni(gfs_ctl.memo_f);   // n > 0 whenever a frame was pinned
     if (n <= 0) return (size_t)n;
     const uint32_t L = g->log_page;
     const uint64_t PS = g->page_size;
@@ -1139,9 +1177,27 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
         while (pending && !err) {
             __syncthreads();
             if (wave() == 0) {
-                if (lane() == 0) gfs_ctl.err = 0;
-                resolve_batch(g, fd, pb, cnt, pending, 0ull, false, root_hint);
+                const uint32_t ln = lane();
+                if (ln == 0) gfs_ctl.err = 0;
+                if (memo_f != GFS_NOFRAME) {
+                    // one page, pinned at entry: a hit without the radix walk
+                    gfs_ctl.kind[ln] = (uint8_t)(ln == 0 ? GFS_K_HIT : GFS_K_DONE);
+                    gfs_ctl.frame[ln] = ln == 0 ? memo_f : GFS_NOFRAME;
+                    gfs_ctl.slot[ln] = 0xFFFFFFFFu;
+                    gfs_ctl.gen[ln] = ln == 0 ? gfs_memo.gen : 0u;
+                    if (ln == 0) {
+                        gfs_ctl.own = 0;
+                        gfs_ctl.rel = 0;
+                        gfs_ctl.cq = 0;
+                        gfs_ctl.hits += 1;
+                        g->fref[memo_f] = 1;
+                        count(&g->ctr.page_hits, 1);
+                    }
+                } else {
+                    resolve_batch(g, fd, pb, cnt, pending, 0ull, false, root_hint);
+                }
             }
+            memo_f = GFS_NOFRAME;
             __syncthreads();
             if (uni(gfs_ctl.err)) { err = uni(gfs_ctl.err); break; }
             GFS_TRACE(1);
@@ -1165,6 +1221,17 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
             }
             __syncthreads();
             GFS_TRACE(3);
+            if (tid() == 0 && pb + creq - 1 == p_last) {
+                // the page this read ends in becomes the memo for the next one
+                const uint32_t k = gfs_ctl.kind[creq - 1];
+                if (k == GFS_K_HIT || k == GFS_K_OWN) {
+                    gfs_memo.fd = fd;
+                    gfs_memo.frame = gfs_ctl.frame[creq - 1];
+                    gfs_memo.gen = gfs_ctl.gen[creq - 1];
+                    gfs_memo.version = version;
+                    gfs_memo.page = p_last;
+                }
+            }
             if (wave() == 0) {
                 const uint32_t ln = lane();
                 if (gfs_ctl.err) {

@@ -119,3 +119,24 @@ def test_grep_sequential_read_ahead(tmp_path, orc):
         pages = 4 * ((size + (16 << 10) - 1) // (16 << 10))
         assert s["page_misses"] == pages and s["rpc_reads"] == pages, s
         assert ctx.check_invariants() == 0
+
+
+def test_grep_page_memo_under_eviction(tmp_path, orc):
+    """Reads inside the page of a workgroup's previous read reuse its frame
+    without the radix walk, but only while the frame still holds that page:
+    here the cache (2 MiB) is a third of the data and pages are evicted between
+    a workgroup's windows on them, so both the reuse and its fallback run."""
+    say("test_grep_page_memo_under_eviction")
+    voc = dg.vocab(2000, seed=61)
+    q = voc[dg.query(2000, 256, seed=62)]
+    size = (768 << 10) + 100
+    paths = [str(tmp_path / f"m{i}.txt") for i in range(8)]
+    dg.text_files(paths, [size] * 8, 63, voc)
+    cpu, _ = orc.scan_files(paths, q, 4096)
+    with make_ctx(page_size=16 << 10, cache_bytes=2 << 20, ring_depth=128) as ctx:
+        counts, status = grep(ctx, paths, q, 4096)
+        assert (status == 0).all(), status
+        assert (counts == cpu).all()
+        s = ctx.stats()
+        assert s["evictions"] > 0, s
+        assert ctx.check_invariants() == 0
