@@ -1216,8 +1216,12 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
                                                                  frame_ptr(g, gfs_ctl.frame[i]) + (lo - poff), hi - lo,
                                                                  (t + T - rot) % T, T);
                 rot = (rot + (uint32_t)((hi - lo) >> 4)) % T;
-                // keep the workgroup's waves on the same page (streams the pool in order)
-                if (page_sync) __syncthreads();
+                // keep the workgroup's waves on the same page (streams the pool in order);
+                // thread 0's agent-scope acquire before the barrier measured ~1 % faster
+                if (page_sync) {
+                    if (tid() == 0) fence_acq_agent();
+                    __syncthreads();
+                }
             }
             __syncthreads();
             GFS_TRACE(3);
