@@ -1158,10 +1158,26 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
     const uint64_t fsize = uni(gfs_ctl.a);
     const uint64_t root_hint = uni(gfs_ctl.b);
     const uint32_t ra_ext = uni(gfs_ctl.ra);
-    uint32_t memo_f = uni(gfs_ctl.memo_f);   // n > 0 whenever a frame was pinned
+    const uint32_t memo_f = uni(gfs_ctl.memo_f);   // n > 0 whenever a frame was pinned
     if (n <= 0) return (size_t)n;
     const uint32_t L = g->log_page;
     const uint64_t PS = g->page_size;
+    if (memo_f != GFS_NOFRAME) {
+        // One page, pinned and checked at entry (thread 0's acquire, then the
+        // barrier): copy, meet once, and thread 0 drops the pin.  The memo
+        // already names this page.
+        copy_seg<true, true, buffer_space, SPACE_GLOBAL>(buffer, frame_ptr(g, memo_f) + (offset & (PS - 1)),
+                                                         (uint64_t)n, tid(), nthreads());
+        __syncthreads();
+        if (tid() == 0) {
+            fadd(&g->meta[memo_f], ~0ull);
+            g->fref[memo_f] = 1;
+            gfs_ctl.hits += 1;
+            count(&g->ctr.page_hits, 1);
+        }
+        GFS_TRACE(6);
+        return (size_t)n;
+    }
     const bool page_sync = (uni(g->tune_flags) & GFS_TUNE_PAGE_SYNC) != 0;
     const uint64_t p_first = offset >> L, p_last = (offset + n - 1) >> L;
     const uint32_t bmax = 64;
@@ -1177,27 +1193,9 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
         while (pending && !err) {
             __syncthreads();
             if (wave() == 0) {
-                const uint32_t ln = lane();
-                if (ln == 0) gfs_ctl.err = 0;
-                if (memo_f != GFS_NOFRAME) {
-                    // one page, pinned at entry: a hit without the radix walk
-                    gfs_ctl.kind[ln] = (uint8_t)(ln == 0 ? GFS_K_HIT : GFS_K_DONE);
-                    gfs_ctl.frame[ln] = ln == 0 ? memo_f : GFS_NOFRAME;
-                    gfs_ctl.slot[ln] = 0xFFFFFFFFu;
-                    gfs_ctl.gen[ln] = ln == 0 ? gfs_memo.gen : 0u;
-                    if (ln == 0) {
-                        gfs_ctl.own = 0;
-                        gfs_ctl.rel = 0;
-                        gfs_ctl.cq = 0;
-                        gfs_ctl.hits += 1;
-                        g->fref[memo_f] = 1;
-                        count(&g->ctr.page_hits, 1);
-                    }
-                } else {
-                    resolve_batch(g, fd, pb, cnt, pending, 0ull, false, root_hint);
-                }
+                if (lane() == 0) gfs_ctl.err = 0;
+                resolve_batch(g, fd, pb, cnt, pending, 0ull, false, root_hint);
             }
-            memo_f = GFS_NOFRAME;
             __syncthreads();
             if (uni(gfs_ctl.err)) { err = uni(gfs_ctl.err); break; }
             GFS_TRACE(1);

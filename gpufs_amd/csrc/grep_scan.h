@@ -74,6 +74,15 @@ __device__ __forceinline__ uint64_t lds_u64_at(const uint8_t* corpus, uint32_t p
 __device__ __forceinline__ uint64_t keep_bytes(uint64_t v, int n) {
     return n >= 8 ? v : (n <= 0 ? 0ull : (v & ((1ull << (8 * n)) - 1)));
 }
+// Eight bytes of the LDS window starting at byte p, without a branch on the
+// alignment (reads both aligned words; the second shift is split so that an
+// aligned p shifts the high word out entirely).
+__device__ __forceinline__ uint64_t lds_u64_nb(const uint8_t* corpus, uint32_t p) {
+    const uint64_t* W = (const uint64_t*)corpus;
+    const uint32_t a = p >> 3, sh = (p & 7) * 8;
+    const uint64_t lo = W[a], hi = W[a + 1];
+    return (lo >> sh) | ((hi << 1) << (63 - sh));
+}
 
 __device__ inline void grep_build_dict(grep_lds& S, const char* dict, uint32_t nw) {
     const uint32_t T = gfs::nthreads(), t = gfs::tid();
@@ -208,8 +217,11 @@ __device__ inline void grep_scan_window(grep_lds& S, uint32_t dsz) {
                 const uint64_t nr = ~nx;
                 len = (64 - b) + (nr ? (uint32_t)__ffsll((unsigned long long)nr) - 1 : 64);
             }
-            if (len > 32 || !((lm >> (len - 1)) & 1u)) continue;
-            const uint64_t t0 = keep_bytes(lds_u64_at(S.corpus, p), (int)len);
+            // len in [1, 32] and some query word has this length, tested with one branch
+            const uint32_t ok = (uint32_t)(len <= 32) & (lm >> ((len - 1) & 31));
+            if (!(ok & 1u)) continue;
+            const uint32_t kb = len < 8 ? len : 8u;
+            const uint64_t t0 = lds_u64_nb(S.corpus, p) & (~0ull >> (64 - 8 * kb));
             const uint32_t h = tok_hash8(t0, len);
             const uint32_t fi = h >> (32 - GREP_FILT_LOG);
             if (!((S.filt[fi >> 5] >> (fi & 31)) & 1u)) continue;
