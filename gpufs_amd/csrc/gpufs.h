@@ -1004,7 +1004,7 @@ __device__ inline void fill_from_host(gfs_dev* g, uint32_t cnt, uint64_t pb, uin
             gfs_ctl.a = m & own & ~done;
         }
         __syncthreads();
-        GFS_TRACE(7);
+        GFS_TRACE(13);
         const uint64_t got = uni(gfs_ctl.a);
         const bool tmo = uni(gfs_ctl.tmo) != 0u;
         if (wave() == 0) {
