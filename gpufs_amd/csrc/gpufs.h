@@ -100,7 +100,9 @@ struct wg_ctl {
 // pin only counts if the frame still carries `gen` and is VALID and the file
 // still has `version`: a frame that was evicted or reassigned has a new
 // generation, and truncation or a reopen after the file changed bumps the
-// version.  gopen clears it.
+// version.  gopen clears it; a memo left in LDS by an earlier launch is checked
+// the same way, and one left by other code fails the frame-range and owner-key
+// checks.
 struct wg_memo {
     int32_t fd;         // -1: none
     uint32_t frame;
@@ -1101,19 +1103,26 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
         else {
             gfs_file* e = &g->files[fd];
             const uint64_t pin = g->page_size - (offset & (g->page_size - 1));
-            const bool try_memo = gfs_memo.fd == fd && size > 0 && size <= pin && (offset >> g->log_page) == gfs_memo.page;
+            // LDS is not cleared between launches: a memo this launch did not
+            // write is only used if its frame index is in range, and the
+            // frame's owner key must name this page
+            const bool try_memo = gfs_memo.fd == fd && size > 0 && size <= pin &&
+                                  (offset >> g->log_page) == gfs_memo.page && gfs_memo.frame < g->n_frames;
             const uint64_t st = ld_rlx(&e->st);
             fs = e->size;
             root = ld_rlx(&e->root);
-            uint64_t mm = 0;
-            if (try_memo) mm = fadd_rlx(&g->meta[gfs_memo.frame], 1ull);
+            uint64_t mm = 0, mk = 0;
+            if (try_memo) {
+                mm = fadd_rlx(&g->meta[gfs_memo.frame], 1ull);
+                mk = g->fkey[gfs_memo.frame];
+            }
             fence_acq_agent();   // the nodes under `root` (and the pinned frame) are read after it
             version = fst_version(st);
             if (fst_state(st) != GFS_FS_OPEN) r = -EBADF;
             else r = offset >= fs ? 0 : (int64_t)(size < fs - offset ? size : fs - offset);
             if (try_memo) {
                 if (r > 0 && version == gfs_memo.version && meta_gen(mm) == gfs_memo.gen &&
-                    meta_state(mm) == GFS_F_VALID)
+                    meta_state(mm) == GFS_F_VALID && mk == gfs_key((uint32_t)fd, gfs_memo.page))
                     mf = gfs_memo.frame;
                 else
                     fadd_rlx(&g->meta[gfs_memo.frame], ~0ull);
