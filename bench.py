@@ -125,8 +125,9 @@ def link_peak_gbs(torch, nbytes=1 << 30, trials=5):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the read path shards per GPU with no exchange, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the read path would shard "
+              "per GPU with no exchange: one context, cache and host service per GPU, files or ranges split "
+              "by index)", file=sys.stderr)
         sys.exit(2)
     import torch
     from gpufs_amd import _lib, datagen as dg

@@ -531,7 +531,7 @@ __device__ inline int rpc_claim_wave(gfs_dev* g, bool need, uint32_t hint) {
                 mine = (int)(w * 64 + first_lane(t));
             }
             base += c;
-            x = old & ~m;   // bits still free that this round did not ask for
+            x = old & ~m;   // bits still free that this pass did not ask for
         }
     }
     return mine;
