@@ -8,7 +8,10 @@
 // the plain and batched forms), write the per-word counts straight into the
 // request's result row in pinned host memory and publish a completion word.
 // Page cache and file table stay resident across requests (and across runs
-// of the same server).
+// of the same server), and so do open files: the first request for a file
+// keeps one open reference for the server (up to a cap), so later requests
+// join the open file without a host round trip and their close is never the
+// last one; the server's references are dropped when it stops.
 //
 // Host (gpufs_serve_run): releases requests at their arrival times, admits
 // them FIFO while a request slot is free and the frames they may need fit the
@@ -20,12 +23,17 @@
 
 #include <chrono>
 #include <cstring>
+#include <cstdlib>
+#include <fcntl.h>
+#include <sys/resource.h>
+#include <unistd.h>
 #include <deque>
 #include <new>
 #include <thread>
 #include <vector>
 
 #define SERVE_STOP 0xFFFFFFFFu
+#define SERVE_HOLD_CLAIM 0xFFFFFFFFu   // held[f] while a workgroup opens f for the server
 
 struct serve_req {
     uint64_t seq;       // ticket + 1 once published
@@ -37,11 +45,12 @@ __global__ __launch_bounds__(1024) void k_serve(const char* names, uint32_t stri
                                                 const char* dict, uint32_t nw, uint32_t chunk,
                                                 serve_req* ring, uint32_t ring_mask, uint32_t* results,
                                                 int32_t* res_status, uint32_t* res_pages, uint64_t* done,
-                                                unsigned long long* ticket, uint64_t idle_ticks) {
+                                                unsigned long long* ticket, uint64_t idle_ticks, uint32_t* held,
+                                                unsigned int* held_n, uint32_t held_cap) {
     __shared__ __attribute__((aligned(16))) grep_lds S;
     __shared__ uint32_t fid, slot;
     __shared__ uint64_t myseq;
-    __shared__ uint32_t h0, m0;
+    __shared__ uint32_t h0, m0, hold;
     grep_build_dict(S, dict, nw);
     const uint32_t t = gfs::tid();
     for (;;) {
@@ -68,7 +77,55 @@ __global__ __launch_bounds__(1024) void k_serve(const char* names, uint32_t stri
         const uint32_t f = gfs::uni(fid), s = gfs::uni(slot);
         const uint64_t sq = gfs::uni(myseq);
         __syncthreads();
-        if (f == SERVE_STOP) break;
+        if (f == SERVE_STOP) {
+            // drop the server's open references of this workgroup's share of the files
+            for (uint32_t x = blockIdx.x; x < nfiles; x += gridDim.x) {
+                if (t == 0) {
+                    // a claim resolves as soon as its workgroup's open returns (bounded wait)
+                    const uint64_t c0 = gfs::now_ticks();
+                    uint32_t hv;
+                    while ((hv = __hip_atomic_load(&held[x], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) ==
+                               SERVE_HOLD_CLAIM &&
+                           gfs::now_ticks() - c0 < idle_ticks)
+                        __builtin_amdgcn_s_sleep(64);
+                    hold = hv;
+                }
+                __syncthreads();
+                const uint32_t hv = gfs::uni(hold);
+                __syncthreads();
+                if (hv != 0 && hv != SERVE_HOLD_CLAIM) {
+                    gclose((int)hv - 1);
+                    if (t == 0) __hip_atomic_store(&held[x], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+            break;
+        }
+        if (f < nfiles && held_cap) {
+            // the first request for a file keeps an open reference for the server
+            if (t == 0) {
+                hold = 0;
+                uint32_t exp = 0;
+                if (__hip_atomic_load(&held[f], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
+                    __hip_atomic_load(held_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < held_cap &&
+                    __hip_atomic_compare_exchange_strong(&held[f], &exp, SERVE_HOLD_CLAIM, __ATOMIC_ACQ_REL,
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    if (atomicAdd(held_n, 1u) < held_cap) hold = 1;
+                    else {
+                        atomicSub(held_n, 1u);
+                        __hip_atomic_store(&held[f], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                }
+            }
+            __syncthreads();
+            if (gfs::uni(hold)) {
+                const int hfd = gopen(names + (size_t)f * stride, O_GRDONLY);
+                if (t == 0) {
+                    if (hfd < 0) atomicSub(held_n, 1u);
+                    __hip_atomic_store(&held[f], hfd >= 0 ? (uint32_t)hfd + 1u : 0u, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        }
         int st = f < nfiles ? grep_one_file(S, names + (size_t)f * stride, nw, chunk, results + (size_t)s * nw)
                             : -EINVAL;
         __threadfence_system();
@@ -100,9 +157,44 @@ struct gpufs_server {
     uint64_t* done;      // host coherent: slots
     uint64_t hits, misses;   // summed over completed requests
     unsigned long long* ticket;   // device
+    uint32_t* held;               // device, per file: 0, SERVE_HOLD_CLAIM, or the server's open fd + 1
+    unsigned int* held_n;         // device: files held open by the server
     uint64_t next_ticket;
     bool running;
 };
+
+// Files the server keeps open at most: about half the process's descriptor
+// limit (the host service opens one descriptor per open file), at most 4096;
+// GPUFS_SERVE_HOLD sets it explicitly.
+// Grow the process's descriptor table once, up front, to cover `fds` more
+// descriptors: the kernel grows it by doubling, and in a threaded process each
+// growth waits for an RCU grace period (about 10-20 ms measured), during which
+// every host open queues.  A server that keeps files open would otherwise hit
+// those stalls in the middle of requests.
+static void serve_reserve_fds(uint32_t fds) {
+    if (fds == 0) return;
+    struct rlimit rl;
+    if (getrlimit(RLIMIT_NOFILE, &rl) != 0) return;
+    const uint64_t soft = rl.rlim_cur == RLIM_INFINITY ? 1ull << 20 : (uint64_t)rl.rlim_cur;
+    const uint64_t want = 2ull * fds + 1024;
+    const int target = (int)(want < soft ? want : soft - 1);
+    const int fd = ::open("/dev/null", O_RDONLY | O_CLOEXEC);
+    if (fd < 0) return;
+    if (target > fd && ::dup2(fd, target) == target) ::close(target);
+    ::close(fd);
+}
+
+static uint32_t serve_hold_cap() {
+    if (const char* e = getenv("GPUFS_SERVE_HOLD")) {   // explicit cap (0: the server holds nothing)
+        const long v = strtol(e, nullptr, 10);
+        if (v >= 0) return (uint32_t)(v < 4096 ? v : 4096);
+    }
+    struct rlimit rl;
+    if (getrlimit(RLIMIT_NOFILE, &rl) != 0) return 0;
+    const uint64_t soft = rl.rlim_cur == RLIM_INFINITY ? 1ull << 20 : (uint64_t)rl.rlim_cur;
+    const uint64_t cap = soft > 256 ? soft / 2 - 64 : 0;
+    return (uint32_t)(cap < 4096 ? cap : 4096);
+}
 
 void gpufs_resident_add(gpufs_ctx* c, int d);
 
@@ -153,25 +245,33 @@ extern "C" int gpufs_serve_start(gpufs_ctx* ctx, const gpufs_serve_config* cfg, 
               hipHostMalloc((void**)&S->rstatus, sizeof(int32_t) * cfg->slots, hf) == hipSuccess &&
               hipHostMalloc((void**)&S->rpages, sizeof(uint32_t) * 2 * (size_t)cfg->slots, hf) == hipSuccess &&
               hipHostMalloc((void**)&S->done, sizeof(uint64_t) * cfg->slots, hf) == hipSuccess &&
-              hipMalloc((void**)&S->ticket, sizeof(unsigned long long)) == hipSuccess;
+              hipMalloc((void**)&S->ticket, sizeof(unsigned long long)) == hipSuccess &&
+              hipMalloc((void**)&S->held, sizeof(uint32_t) * (size_t)n_files) == hipSuccess &&
+              hipMalloc((void**)&S->held_n, sizeof(unsigned int)) == hipSuccess;
     if (!ok) {
         (void)hipHostFree(S->ring); (void)hipHostFree(S->results); (void)hipHostFree(S->rstatus);
         (void)hipHostFree(S->rpages); (void)hipHostFree(S->done); (void)hipFree(S->ticket);
+        (void)hipFree(S->held); (void)hipFree(S->held_n);
         delete S;
         return -ENOMEM;
     }
     memset(S->ring, 0, sizeof(serve_req) * rs);
     memset(S->done, 0, sizeof(uint64_t) * cfg->slots);
-    if (hipMemsetAsync(S->ticket, 0, sizeof(unsigned long long), S->stream) != hipSuccess) return -EIO;
+    if (hipMemsetAsync(S->ticket, 0, sizeof(unsigned long long), S->stream) != hipSuccess ||
+        hipMemsetAsync(S->held, 0, sizeof(uint32_t) * (size_t)n_files, S->stream) != hipSuccess ||
+        hipMemsetAsync(S->held_n, 0, sizeof(unsigned int), S->stream) != hipSuccess)
+        return -EIO;
     S->next_ticket = 0;
     gpufs_host_start(ctx);
     // idle workgroups give up after twice the RPC wait limit without a request
     gfs_dev hd;
     if (hipMemcpy(&hd, gpufs_device_state(ctx), sizeof(hd), hipMemcpyDeviceToHost) != hipSuccess) return -EIO;
     const uint64_t idle = hd.wait_ticks * 2;
+    const uint32_t hold_cap = serve_hold_cap();
+    serve_reserve_fds(hold_cap);
     hipLaunchKernelGGL(k_serve, dim3(S->workgroups), dim3(S->threads), 0, S->stream, names_dev, name_stride,
                        n_files, dict_dev, cfg->n_words, cfg->chunk_bytes, S->ring, S->ring_mask, S->results,
-                       S->rstatus, S->rpages, S->done, S->ticket, idle);
+                       S->rstatus, S->rpages, S->done, S->ticket, idle, S->held, S->held_n, hold_cap);
     if (hipGetLastError() != hipSuccess) return -EIO;
     S->running = true;
     gpufs_resident_add(ctx, 1);
@@ -269,6 +369,8 @@ extern "C" int gpufs_serve_stop(gpufs_server* S) {
     (void)hipHostFree(S->rpages);
     (void)hipHostFree(S->done);
     (void)hipFree(S->ticket);
+    (void)hipFree(S->held);
+    (void)hipFree(S->held_n);
     delete S;
     return rc;
 }

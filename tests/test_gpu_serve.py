@@ -2,6 +2,7 @@
 request's counts equal the CPU path for its file, and are the same when the
 request runs alone and when it runs in a Poisson mix with others in flight."""
 import errno
+import resource
 
 import numpy as np
 import pytest
@@ -51,4 +52,13 @@ def test_serve_alone_equals_mix_equals_cpu(tmp_path, orc):
         s = ctx.stats()
         # the workgroups' own counts agree with the context counters
         assert s["page_hits"] == hits and s["page_misses"] == misses
+        # the server keeps each file open from its first request on (while under its cap of half
+        # the descriptor limit, at most 4096), so every distinct file is opened on the host once,
+        # and it closes them all when it stops
+        soft = resource.getrlimit(resource.RLIMIT_NOFILE)[0]
+        cap = 4096 if soft == resource.RLIM_INFINITY else (min(4096, soft // 2 - 64) if soft > 256 else 0)
+        distinct = len(set(ids.tolist()))
+        if cap >= distinct:
+            assert s["rpc_opens"] == distinct, (s, distinct)
+        assert s["rpc_opens"] == s["rpc_closes"], s
         assert ctx.check_invariants() == 0
