@@ -47,7 +47,9 @@ def parse():
     ap.add_argument("--threads", type=int, default=1024)
     ap.add_argument("--page-kib", type=int, default=256)
     ap.add_argument("--cache-gib", type=float, default=16.0)
-    ap.add_argument("--ring-depth", type=int, default=1024)
+    ap.add_argument("--ring-depth", type=int, default=2048,
+                    help="RPC ring slots (pinned staging = slots x page size); 2048 measured 4-5 %% faster "
+                         "cold than 1024, 4096 only 1 %% more")
     ap.add_argument("--host-threads", type=int, default=16)
     ap.add_argument("--cold-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -281,7 +283,8 @@ def main():
         "data": "synthetic (seeded uniform random bytes; seeded Zipf text for grep)",
         "config": {"workload": "sequential gread: 1 file, 256 workgroups x 32 MiB ranges, 256 KiB pages, 16 GiB page cache, warm",
                    "file_bytes": size, "workgroups": n, "range_bytes": rng_bytes, "page_bytes": args.page_kib << 10,
-                   "cache_bytes": int(args.cache_gib * (1 << 30)), "threads_per_workgroup": args.threads},
+                   "cache_bytes": int(args.cache_gib * (1 << 30)), "threads_per_workgroup": args.threads,
+                   "ring_slots": args.ring_depth, "host_threads": args.host_threads},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _profiled_traffic(),
                      "kernel": "k_gread_ranges", "kernel_avg_ms": round(avg_k * 1e3, 4),
