@@ -62,6 +62,7 @@ def parse():
                     help="also run the gmmap GEMV leg (65536 x 131072 fp32 = 32 GiB matrix file, 8 GiB cache)")
     ap.add_argument("--gemv-rows", type=int, default=65536)
     ap.add_argument("--gemv-page-kib", type=int, default=512)
+    ap.add_argument("--gemv-ring", type=int, default=1024, help="RPC ring slots of the GEMV context")
     ap.add_argument("--serve-files", type=int, default=100000)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-rates", default="1000,10000,50000")
@@ -492,7 +493,7 @@ def bench_gemv(args, torch, L, _lib, dg):
     _warm_host_cache([path])
     x = dg.normal_f32(cols, dg.SEED_C2 + 1)
     cache = 8 << 30
-    ctx = _lib.Context(cache_bytes=cache, page_size=args.gemv_page_kib << 10, ring_depth=1024,
+    ctx = _lib.Context(cache_bytes=cache, page_size=args.gemv_page_kib << 10, ring_depth=args.gemv_ring,
                        host_threads=args.host_threads, max_files=64)
     ctx.host_start()
     stream = torch.cuda.ExternalStream(ctx.stream)
