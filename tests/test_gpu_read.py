@@ -20,7 +20,7 @@ def _rand_file(path, size, seed):
 @pytest.mark.parametrize("page", [4096, 65536])
 def test_read_whole_files(tmp_path, page):
     say(f"test_read_whole_files page={page}")
-    sizes = [1, 100, 4095, 4096, 4097, 3 * 65536 + 5, 1 << 20]
+    sizes = [0, 1, 100, 4095, 4096, 4097, 3 * 65536 + 5, 1 << 20]
     with make_ctx(page_size=page, cache_bytes=16 << 20) as ctx:
         for i, size in enumerate(sizes):
             p = tmp_path / f"w{i}.bin"
@@ -49,6 +49,68 @@ def test_read_offsets_eof_and_misaligned_dst(tmp_path):
             exp = data[off:off + ln] if off < size else b""
             assert got == len(exp), (off, ln, got)
             assert out == exp, (off, ln)
+        assert ctx.check_invariants() == 0
+
+
+def _sparse_with_blobs(path, size, offsets, seed):
+    """A sparse file of `size` bytes: seeded random 1 MiB blobs at `offsets`,
+    zeros (holes) elsewhere.  Returns {offset: bytes}."""
+    rng = np.random.default_rng(seed)
+    blobs = {}
+    with open(path, "wb") as f:
+        f.truncate(size)
+        for off in offsets:
+            b = rng.integers(0, 256, size=1 << 20, dtype=np.uint8).tobytes()
+            f.seek(off)
+            f.write(b)
+            blobs[off] = b
+    return blobs
+
+
+def _sparse_bytes(blobs, off, n):
+    """Bytes [off, off + n) of a file made by _sparse_with_blobs."""
+    buf = bytearray(n)
+    for bo, b in blobs.items():
+        lo, hi = max(off, bo), min(off + n, bo + len(b))
+        if lo < hi:
+            buf[lo - off:hi - off] = b[lo - bo:hi - bo]
+    return bytes(buf)
+
+
+def test_read_past_4gib_offsets(tmp_path):
+    """64-bit offsets: reads that straddle 2 GiB and 4 GiB, start past 4 GiB and
+    end at EOF of a 6 GiB sparse file return its bytes (zeros in the holes)."""
+    say("test_read_past_4gib_offsets")
+    G = 1 << 30
+    size = 6 * G + 12345
+    p = tmp_path / "big.bin"
+    blobs = _sparse_with_blobs(p, size, [0, 2 * G - (512 << 10), 4 * G - (300 << 10), size - (1 << 20)], 21)
+    cases = [(2 * G - 1000, 5000), (4 * G - 100000, 300000), (4 * G, 4096), (4 * G + 1, 1 << 20),
+             (size - 70000, 100000), (size - 1, 1), (size, 10), (5 * G + 7, 65536)]
+    with make_ctx(page_size=65536, cache_bytes=64 << 20) as ctx:
+        for k, (off, ln) in enumerate(cases):
+            n = max(0, min(ln, size - off))
+            got, fsize, crc, out = read_file(ctx, str(p), off, ln, dst_pad=k % 3)
+            assert (got, fsize, crc) == (n, size, 0), (off, ln, got, fsize, crc)
+            assert out == _sparse_bytes(blobs, off, n), (off, ln)
+        assert ctx.check_invariants() == 0
+
+
+def test_single_read_larger_than_2gib(tmp_path):
+    """One gread call of 2.25 GiB that crosses the 4 GiB mark, through a cache
+    smaller than the read (the reference counts a read's bytes in an int, so
+    it stops at 2 GiB; here the count is 64-bit)."""
+    say("test_single_read_larger_than_2gib")
+    G = 1 << 30
+    size = 5 * G
+    p = tmp_path / "huge.bin"
+    off, ln = 2 * G - (1 << 20), (9 * G) // 4
+    blobs = _sparse_with_blobs(p, size, [off, 3 * G + 12345, 4 * G - (1 << 19), off + ln - (1 << 20)], 22)
+    with make_ctx(page_size=1 << 20, cache_bytes=512 << 20, ring_depth=128) as ctx:
+        got, fsize, crc, out = read_file(ctx, str(p), off, ln, threads=1024)
+        assert (got, fsize, crc) == (ln, size, 0), (got, fsize, crc)
+        assert out == _sparse_bytes(blobs, off, ln)
+        assert ctx.stats()["evictions"] > 0
         assert ctx.check_invariants() == 0
 
 

@@ -51,6 +51,27 @@ def test_write_extends_file(tmp_path):
         assert p.read_bytes() == exp
 
 
+def test_write_past_4gib_into_new_file(tmp_path):
+    """gwrite at a 64-bit offset: 100,000 bytes at 5 GiB + 3 of a new file
+    extend it there (a hole of zeros before them) and read back through the
+    cache."""
+    say("test_write_past_4gib_into_new_file")
+    p = tmp_path / "sparse_w.bin"
+    off = (5 << 30) + 3
+    data = np.random.default_rng(4).integers(0, 256, size=100000, dtype=np.uint8).tobytes()
+    with make_ctx(page_size=65536) as ctx:
+        r = write_file(ctx, str(p), off, data, O_GWRONLY | O_GCREAT)
+        assert r[0] == len(data) and r[1] == off + len(data) and r[2] == 0, r
+        assert os.path.getsize(p) == off + len(data)
+        with open(p, "rb") as f:
+            f.seek(off - 10)
+            assert f.read() == b"\0" * 10 + data
+        got, fsize, _, out = read_file(ctx, str(p), off - 10, len(data) + 10)
+        assert got == len(data) + 10 and fsize == off + len(data)
+        assert out == b"\0" * 10 + data
+        assert ctx.check_invariants() == 0
+
+
 def test_wronce_or_merges_with_disk(tmp_path):
     say("test_wronce_or_merges_with_disk")
     p = tmp_path / "once.bin"
