@@ -13,10 +13,10 @@
 // published in the name index every change is an atomic add of a delta (or a
 // CAS from the value read to that value plus a delta), so a count added by an
 // opener that is still deciding is never overwritten:
-//   * an opener adds one open, then looks at the old state: OPEN -> joined;
-//     OPENING -> it keeps the count and waits, the host open's outcome
-//     carries the count over; CLOSING / CLOSED -> it takes the count back,
-//     then reopens a CLOSED file by CAS or waits for a closer to finish.
+//   * an opener looks at the word first: OPEN -> it adds one open and is
+//     joined if the state was still OPEN (else it takes the count back and
+//     looks again); CLOSED -> it moves the state to OPENING with its open by
+//     CAS and does the host open; OPENING / CLOSING -> it waits on loads.
 //   * a closer subtracts its open; the one that leaves OPEN with zero opens
 //     moves it to CLOSING by CAS, which fails when an opener joined meanwhile.
 // When every workgroup of a launch opens the same file, an open or close of
@@ -44,35 +44,28 @@ GFS_HD inline uint64_t fst_delta(uint32_t from, uint32_t to) { return (uint64_t)
 #define FST_MUST_OPEN 1  // state is OPENING with this open counted: do the host open, then fst_open_done
 
 template <class P> GFS_HD int fst_open(uint64_t* w, P& p) {
-    uint64_t old = p.fadd(w, FST_ONE);   // optimistic: the file is usually open already
+    // Look before adding: when a launch's workgroups all open the same closed
+    // file, one of them wins the CAS to OPENING and the others wait on plain
+    // loads, then join with one add each (every atomic on the word queues
+    // behind the others at one L2 channel).
+    uint64_t cur = p.load(w);
     for (;;) {
-        const uint32_t s = fst_state(old);
-        if (s == GFS_FS_OPEN) return FST_JOINED;
-        if (s == GFS_FS_OPENING) {
-            uint64_t cur;
-            while (fst_state(cur = p.load(w)) == GFS_FS_OPENING) {
-                if (p.expired()) {
-                    p.fadd(w, FST_NEG);
-                    return -ETIMEDOUT;
-                }
-                p.pause();
-            }
-            if (fst_state(cur) == GFS_FS_OPEN) return FST_JOINED;
-            // the host open failed and the entry is CLOSED again: give the count back
+        const uint32_t s = fst_state(cur);
+        if (s == GFS_FS_OPEN) {
+            const uint64_t old = p.fadd(w, FST_ONE);
+            if (fst_state(old) == GFS_FS_OPEN) return FST_JOINED;
+            // a closer moved it to CLOSING first: give the count back, look again
+            cur = p.fadd(w, FST_NEG) + FST_NEG;
+            continue;
         }
-        uint64_t st = p.fadd(w, FST_NEG) + FST_NEG;   // the word right after taking the count back
-        for (;;) {
-            const uint32_t c = fst_state(st);
-            if (c == GFS_FS_OPEN || c == GFS_FS_OPENING) break;
-            if (c == GFS_FS_CLOSED) {
-                if (p.cas(w, st, st + fst_delta(GFS_FS_CLOSED, GFS_FS_OPENING) + FST_ONE)) return FST_MUST_OPEN;
-                continue;   // st holds the current word
-            }
-            if (p.expired()) return -ETIMEDOUT;   // CLOSING: the closer is still at the host
-            p.pause();
-            st = p.load(w);
+        if (s == GFS_FS_CLOSED) {
+            if (p.cas(w, cur, cur + fst_delta(GFS_FS_CLOSED, GFS_FS_OPENING) + FST_ONE)) return FST_MUST_OPEN;
+            continue;   // cur holds the current word
         }
-        old = p.fadd(w, FST_ONE);
+        // OPENING or CLOSING: another workgroup is at the host
+        if (p.expired()) return -ETIMEDOUT;
+        p.pause();
+        cur = p.load(w);
     }
 }
 

@@ -587,9 +587,11 @@ __device__ inline int rpc_wait(gfs_dev* g, uint32_t cq, uint64_t want) {
     return 0;
 }
 // One synchronous request from a single thread (open / close / truncate / write).
+static_assert(offsetof(gfs_file, name) == 0 && sizeof(gfs_file) % 8 == 0, "file-table names are 8-byte aligned");
 __device__ inline int64_t rpc_call_single(gfs_dev* g, uint32_t op, int32_t host_fd, int32_t flags,
                                           uint64_t off, uint64_t len, const char* name, int* slot_out) {
     int s = -1;
+    GFS_TRACE(23);
     uint64_t t0 = now_ticks();
     uint32_t hint = blockIdx.x * 7u + blockIdx.y * 13u;
     while ((s = rpc_claim(g, true, hint)) < 0) {
@@ -597,16 +599,22 @@ __device__ inline int64_t rpc_call_single(gfs_dev* g, uint32_t op, int32_t host_
         backoff();
     }
     if (name) {
+        // a file-table entry's name: 8-byte aligned, GFS_NAME_MAX bytes, zero
+        // padded; all its words are loaded before the first store
         gfs_rpc_slot* r = &g->slots[s];
-        for (int i = 0; i < GFS_NAME_MAX; i += 8) {
-            uint64_t w = 0;
-            for (int b = 0; b < 8; ++b) w |= (uint64_t)(uint8_t)name[i + b] << (8 * b);
-            sys_st((uint64_t*)(r->name + i), w);
-        }
+        const uint64_t* src = (const uint64_t*)name;
+        uint64_t w[GFS_NAME_MAX / 8];
+#pragma unroll
+        for (int i = 0; i < GFS_NAME_MAX / 8; ++i) w[i] = src[i];
+#pragma unroll
+        for (int i = 0; i < GFS_NAME_MAX / 8; ++i) sys_st((uint64_t*)(r->name + 8 * i), w[i]);
     }
     sys_st(&g->cq[s], 0u);
+    GFS_TRACE(20);
     rpc_post(g, s, op, host_fd, flags, off, len, (uint32_t)s);
+    GFS_TRACE(21);
     int w = rpc_wait(g, (uint32_t)s, 1ull);
+    GFS_TRACE(22);
     int64_t res = w ? (int64_t)w : sys_ld(&g->slots[s].result);
     if (slot_out) *slot_out = s;   // caller reads extra results, then releases
     else rpc_release(g, s);
@@ -688,11 +696,22 @@ __device__ inline bool name_eq(const gfs_file* e, const gname& n) {
 // fstate.h policy: agent-scope atomics, the context's wait limit.
 struct fst_dev {
     uint64_t t0, limit;
+    uint32_t spins = 0;
     __device__ uint64_t fadd(uint64_t* w, uint64_t v) { return gfs::fadd(w, v); }
     __device__ uint64_t load(uint64_t* w) { return ld_acq(w); }
     __device__ bool cas(uint64_t* w, uint64_t& exp, uint64_t v) { return gfs::cas(w, exp, v); }
     __device__ bool expired() { return now_ticks() - t0 > limit; }
-    __device__ void pause() { backoff(); }
+    // Waits for another workgroup's host open or close: after a few short
+    // pauses, longer ones (about 1.5 us), so that hundreds of waiting
+    // workgroups do not crowd the L2 channel that holds the opener's word.
+    __device__ void pause() {
+        if (spins < 4) {
+            ++spins;
+            backoff();
+        } else {
+            __builtin_amdgcn_s_sleep(48);
+        }
+    }
 };
 
 // Host open of entry e (new or reopened), whose state is OPENING.  Fills the
@@ -708,6 +727,7 @@ __device__ inline int open_host(gfs_dev* g, gfs_file* e, int flags, bool reopen)
         size = sys_ld(&sl->size), ct = sys_ld(&sl->ctime_ns), mt = sys_ld(&sl->mtime_ns), ino = sys_ld(&sl->ino);
         rpc_release(g, s);
     }
+    GFS_TRACE(24);
     fst_dev P{0, 0};
     if (s < 0 || r < 0) {
         fst_open_done(&e->st, P, false, false);
@@ -783,7 +803,9 @@ __device__ inline int open_single(const char* name, int flags) {
     if (idx < 0) return E_FSTABLE_FULL;
     gfs_file* e = &g->files[idx];
     fst_dev P{t0, g->wait_ticks};
+    GFS_TRACE(25);
     const int r = fst_open(&e->st, P);
+    GFS_TRACE(26);
     if (r == FST_JOINED) return idx;
     if (r < 0) return r;
     return open_host(g, e, flags, true) < 0 ? E_IPC_OPEN_ERROR : idx;
