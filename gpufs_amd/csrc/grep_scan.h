@@ -257,11 +257,16 @@ __device__ inline int grep_file_windows(grep_lds& S, int fd, uint32_t chunk, OnW
             if (t == 0) S.status = (int64_t)got < 0 ? (int)(int64_t)got : -EIO;
             break;
         }
-        if (t == 0) {
-            S.corpus[left] = 0;
-            int ov = 0;
-            if (!last) while (ov < 32 && g_alpha(S.corpus[chunk + ov])) ++ov;
-            S.ov = ov;
+        // the word that straddles the chunk end: its bytes in the look-ahead
+        // (at most 32), found by one ballot of wave 0 over the look-ahead
+        if (gfs::wave() == 0) {
+            const uint32_t ln = gfs::lane();
+            const bool stop = last || ln >= 32 || !g_alpha(S.corpus[chunk + ln]);
+            const uint64_t m = __ballot(stop) | (1ull << 32);
+            if (ln == 0) {
+                S.corpus[left] = 0;
+                S.ov = (int)__builtin_ctzll(m);
+            }
         }
         __syncthreads();
         const uint32_t ov = gfs::uni((uint32_t)S.ov);
