@@ -24,7 +24,7 @@
 #define GREP_FILT_LOG 18   // membership filter bits (32 KiB of LDS)
 
 struct grep_lds {
-    uint8_t corpus[GREP_MAX_CHUNK + 128];    // window + look-ahead + NUL + read padding
+    uint8_t corpus[GREP_MAX_CHUNK + 128];    // alignment (< 16) + window + look-ahead + NUL + read padding
     uint64_t words[GREP_MAX_WORDS][4];       // query bytes, zero-padded, little-endian u64
     uint32_t ht[GREP_HT];                    // (canonical word index + 1) | length << 16; 0 = empty
     uint32_t filt[(1u << GREP_FILT_LOG) / 32];   // bit (hash >> (32 - GREP_FILT_LOG)) set for each query word
@@ -193,14 +193,17 @@ __device__ inline uint64_t grep_seg_mask(const uint8_t* corpus, uint32_t base, u
 // the byte before a segment comes from the previous lane's mask.  A byte-wise
 // LDS walk there (one load and wait per byte, taken by almost every lane in the
 // same iteration) cost a third of the scan.
-__device__ inline void grep_scan_window(grep_lds& S, uint32_t dsz) {
+// The window's first `skip` bytes (< 64) are not part of it: they are masked
+// out as if they were separators.
+__device__ inline void grep_scan_window(grep_lds& S, uint32_t dsz, uint32_t skip = 0) {
     const uint32_t T = gfs::nthreads(), t = gfs::tid(), ln = gfs::lane();
     const uint32_t nseg = (dsz + 63) >> 6;
     const uint32_t lm = S.lenmask;
     for (uint32_t sg0 = t - ln; sg0 < nseg; sg0 += T) {
         const uint32_t sg = sg0 + ln;
         const uint32_t base = sg << 6;
-        const uint64_t am = sg < nseg ? grep_seg_mask(S.corpus, base, dsz, 4) : 0ull;
+        uint64_t am = sg < nseg ? grep_seg_mask(S.corpus, base, dsz, 4) : 0ull;
+        if (sg == 0) am &= ~0ull << skip;
         uint64_t nx = __shfl_down(am, 1);
         uint64_t pv = __shfl_up(am, 1) >> 63;
         if (sg >= nseg) continue;
@@ -248,12 +251,17 @@ __device__ inline int grep_file_windows(grep_lds& S, int fd, uint32_t chunk, OnW
     while (cursor < size) {
         const bool last = size - cursor < (uint64_t)chunk + 32;
         const uint32_t left = last ? (uint32_t)(size - cursor) : chunk + 32;
+        // The read starts at the 16-byte boundary at or below the cursor, so
+        // that it copies with aligned 16-byte loads (a window starting
+        // anywhere else takes five dword loads per 16 bytes); the `mis` bytes
+        // before the cursor are masked out of the scan.
+        const uint32_t mis = (uint32_t)(cursor & 15);
         size_t got;
         // Inlined here: as an outlined call, gread's prologue and epilogue save and
         // restore about fifty registers through scratch on every window.  The
         // window is in LDS, so only the global->LDS copy path is compiled in.
-        [[clang::always_inline]] got = gread<gfs::SPACE_LDS>(fd, cursor, left, S.corpus);
-        if (got != left) {
+        [[clang::always_inline]] got = gread<gfs::SPACE_LDS>(fd, cursor - mis, left + mis, S.corpus);
+        if (got != left + mis) {
             if (t == 0) S.status = (int64_t)got < 0 ? (int)(int64_t)got : -EIO;
             break;
         }
@@ -261,18 +269,18 @@ __device__ inline int grep_file_windows(grep_lds& S, int fd, uint32_t chunk, OnW
         // (at most 32), found by one ballot of wave 0 over the look-ahead
         if (gfs::wave() == 0) {
             const uint32_t ln = gfs::lane();
-            const bool stop = last || ln >= 32 || !g_alpha(S.corpus[chunk + ln]);
+            const bool stop = last || ln >= 32 || !g_alpha(S.corpus[mis + chunk + ln]);
             const uint64_t m = __ballot(stop) | (1ull << 32);
             if (ln == 0) {
-                S.corpus[left] = 0;
+                S.corpus[mis + left] = 0;
                 S.ov = (int)__builtin_ctzll(m);
             }
         }
         __syncthreads();
         const uint32_t ov = gfs::uni((uint32_t)S.ov);
-        const uint32_t dsz = last ? left + 1 : chunk + ov + 1;
+        const uint32_t dsz = mis + (last ? left + 1 : chunk + ov + 1);
         cursor += (uint64_t)chunk + ov;
-        grep_scan_window(S, dsz);
+        [[clang::always_inline]] grep_scan_window(S, dsz, mis);
         __syncthreads();
         GFS_TRACE(7);
         on_window();
