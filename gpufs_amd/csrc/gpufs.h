@@ -299,6 +299,21 @@ __device__ inline void copy_seg(uint8_t* dst, const uint8_t* src, uint64_t len, 
         }
         uint64_t u = j;
         const uint64_t st = T;
+        if (dst_space == SPACE_LDS && !sl) {
+            // into LDS (a window of at most 64 KiB plus a look-ahead): five
+            // loads per lane in flight, the ones past the end switched off, so
+            // a window a little over 4 x 16 x T bytes still takes one round trip
+            for (; u < units; u += 5 * st) {
+                v4u r[5];
+#pragma unroll
+                for (int k = 0; k < 5; ++k)
+                    if (u + k * st < units) r[k] = load16(S + 16 * (u + k * st), false, nt_load);
+#pragma unroll
+                for (int k = 0; k < 5; ++k)
+                    if (u + k * st < units) store16(D + 16 * (u + k * st), r[k], true, nt_store);
+            }
+            return;
+        }
         for (; u + 3 * st < units; u += 4 * st) {
             v4u a = load16(S + 16 * u, sl, nt_load);
             v4u b = load16(S + 16 * (u + st), sl, nt_load);
