@@ -63,6 +63,7 @@
 #define GFS_READAHEAD_BYTES (1u << 20)   // pages fetched ahead of a sequential reader (at most 63 pages)
 
 // ---- RPC opcodes
+#define GFS_SQ_NOSLOT 0xFFFFu   // submission-queue slot field of a close that carries no slot (ring_depth <= 65535)
 #define GFS_OP_READ  1u
 #define GFS_OP_WRITE 2u
 #define GFS_OP_DIFF  3u     // write OR-merged with on-disk bytes (O_GWRONCE pages)
@@ -117,7 +118,7 @@ typedef struct gfs_dev {
     uint32_t n_frames;
     uint32_t max_files;
     uint32_t ring_depth;
-    uint32_t sq_depth;      // power of two >= ring_depth
+    uint32_t sq_depth;      // power of two >= ring_depth + max_files (room for every slot's request and one close per file)
     uint32_t name_cap;      // power of two >= 2*max_files
     uint32_t node_cap;
     uint32_t slot_words;    // ceil(ring_depth / 64)
@@ -137,6 +138,7 @@ typedef struct gfs_dev {
     uint64_t* sq;           // submission queue: (ticket << 16) | slot
     uint64_t* cq;           // completion masks, one per slot (a batch of <= 64 requests shares its first slot's)
     uint8_t*  staging;      // ring_depth << log_page bytes (+ padding)
+    int64_t*  closeq;       // sq_depth entries: host fd of the close posted with ticket t at [t & (sq_depth - 1)]
     // allocation cursors (device atomics)
     unsigned long long fresh_next;   // frames handed out from the never-used range
     unsigned long long clock_hand;

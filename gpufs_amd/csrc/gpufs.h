@@ -589,6 +589,17 @@ __device__ inline void rpc_post(gfs_dev* g, int s, uint32_t op, int32_t host_fd,
     unsigned long long t = fadd_rlx(&g->sq_tail, 1ull);
     sys_st_rel(&g->sq[t & (g->sq_depth - 1)], (uint64_t)((t << 16) | (uint64_t)s));
 }
+// Close of a host fd, posted without a slot and not waited for: the fd goes
+// into closeq under the request's ticket.  The submission queue has room for a
+// request from every slot plus one such close per file-table entry (an entry
+// is reopened only through a later ticket, which the host takes after this
+// one), so a ticket never lands on an entry the host has not taken yet.
+__device__ inline void rpc_post_close(gfs_dev* g, int32_t host_fd) {
+    const unsigned long long t = fadd_rlx(&g->sq_tail, 1ull);
+    sys_st(&g->closeq[t & (g->sq_depth - 1)], (int64_t)host_fd);
+    fence_rel_sys();
+    sys_st_rel(&g->sq[t & (g->sq_depth - 1)], (uint64_t)((t << 16) | (uint64_t)GFS_SQ_NOSLOT));
+}
 // Wait until every bit of `want` is set in completion mask `cq`.  Returns 0
 // or -ETIMEDOUT.
 __device__ inline int rpc_wait(gfs_dev* g, uint32_t cq, uint64_t want) {
@@ -1539,9 +1550,10 @@ __device__ inline int gclose(int fd) {
     int r = uni(gfs_ctl.flag) ? flush_file(g, fd) : 0;
     __syncthreads();
     if (tid() == 0) {
-        int64_t c = rpc_call_single(g, GFS_OP_CLOSE, e->host_fd, 0, 0, 0, nullptr, nullptr);
+        // the host closes its fd after the workgroup has moved on: nothing is
+        // written through it any more (the flush above waited for its writes)
+        rpc_post_close(g, e->host_fd);
         count(&g->ctr.rpc_closes, 1);
-        if (r == 0 && c < 0) r = (int)c;
         e->host_fd = -1;
         e->dirty = 0;
         fst_dev P{0, 0};

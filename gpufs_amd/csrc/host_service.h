@@ -43,6 +43,7 @@ struct RingView {
     uint64_t* sq = nullptr;
     uint64_t* cq = nullptr;     // completion masks
     uint8_t* staging = nullptr;
+    int64_t* closeq = nullptr;  // host fd of the slot-less close with ticket t at [t & (sq_depth - 1)]
     uint32_t sq_depth = 0;      // power of two
     uint32_t ring_depth = 0;
     uint32_t log_page = 0;
@@ -215,13 +216,22 @@ static inline int64_t serve_slot(const RingView& rv, uint32_t s, ServiceCounters
 }
 
 // Try to take the next published request.  Returns true and sets *slot.
-static inline bool try_take(const RingView& rv, std::atomic<uint64_t>& head, uint32_t* slot) {
+static inline bool try_take(const RingView& rv, std::atomic<uint64_t>& head, uint32_t* slot, uint64_t* ticket) {
     uint64_t h = head.load(std::memory_order_relaxed);
     uint64_t e = __atomic_load_n(&rv.sq[h & (rv.sq_depth - 1)], __ATOMIC_ACQUIRE);
     if ((e >> 16) != (h & 0xFFFFFFFFFFFFull)) return false;
     if (!head.compare_exchange_strong(h, h + 1, std::memory_order_acq_rel)) return false;
     *slot = (uint32_t)(e & 0xFFFF);
+    *ticket = h;
     return true;
+}
+
+// A close posted without a slot (the device does not wait for it): the host
+// fd sits in closeq under the request's ticket.
+static inline void serve_slotless_close(const RingView& rv, uint64_t ticket, ServiceCounters& ctr) {
+    const int64_t fd = __atomic_load_n(&rv.closeq[ticket & (rv.sq_depth - 1)], __ATOMIC_RELAXED);
+    if (fd < 0 || ::close((int)fd) != 0) ctr.errors.fetch_add(1, std::memory_order_relaxed);
+    ctr.requests.fetch_add(1, std::memory_order_relaxed);
 }
 
 // Worker loop: spin while work arrives, back off to short sleeps when idle.
@@ -262,16 +272,21 @@ static inline void worker_loop(RingView rv, std::atomic<uint64_t>* head, std::at
     prctl(PR_SET_TIMERSLACK, 1000UL, 0, 0, 0);
     WorkerBufs wb;
     uint64_t idle = 0;
-    while (!stop->load(std::memory_order_relaxed)) {
+    for (;;) {
         uint32_t s;
-        if (try_take(rv, *head, &s)) {
+        uint64_t ticket;
+        if (try_take(rv, *head, &s, &ticket)) {
             if (wk->sleepers.load(std::memory_order_relaxed) && backlog(rv, *head)) wake_all(wk);
             const uint64_t t0 = mono_ns();
-            serve_slot(rv, s, *ctr, wb);
+            if (s == GFS_SQ_NOSLOT) serve_slotless_close(rv, ticket, *ctr);
+            else serve_slot(rv, s, *ctr, wb);
             ctr->busy_ns.fetch_add(mono_ns() - t0, std::memory_order_relaxed);
             idle = 0;
             continue;
         }
+        // stopping: only once every published request is taken, so closes the
+        // device posted without waiting are not left behind
+        if (stop->load(std::memory_order_relaxed)) break;
         ++idle;
         if (index == 0 || idle < 20000) {
 #if defined(__x86_64__)

@@ -68,6 +68,7 @@ static gfs_host::RingView ring_view(gpufs_ctx* c) {
     rv.sq = c->host.sq;
     rv.cq = c->host.cq;
     rv.staging = c->host.staging;
+    rv.closeq = c->host.closeq;
     rv.sq_depth = c->host.sq_depth;
     rv.ring_depth = c->host.ring_depth;
     rv.log_page = c->host.log_page;
@@ -87,6 +88,7 @@ static void free_all(gpufs_ctx* c) {
     (void)hipFree(h.slot_free);
     (void)hipHostFree(h.slots);
     (void)hipHostFree(h.sq);
+    (void)hipHostFree(h.closeq);
     (void)hipHostFree(h.cq);
     (void)hipHostFree(h.staging);
     (void)hipFree(c->dev);
@@ -154,7 +156,7 @@ extern "C" int gpufs_init(const gpufs_config* cfg_in, gpufs_ctx** out) {
     h.n_frames = (uint32_t)(cfg.cache_bytes / cfg.page_size);
     h.max_files = cfg.max_files;
     h.ring_depth = cfg.ring_depth;
-    h.sq_depth = pow2_at_least(std::max<uint32_t>(cfg.ring_depth, 64));
+    h.sq_depth = pow2_at_least(std::max<uint64_t>((uint64_t)cfg.ring_depth + cfg.max_files, 64));
     h.name_cap = pow2_at_least(2ull * cfg.max_files);
     h.slot_words = (cfg.ring_depth + 63) / 64;
     uint64_t auto_nodes = 4096ull + 2ull * cfg.max_files + 4ull * ((uint64_t)h.n_frames / GFS_FANOUT + 1);
@@ -187,6 +189,7 @@ extern "C" int gpufs_init(const gpufs_config* cfg_in, gpufs_ctx** out) {
     const unsigned hflags = hipHostMallocMapped | hipHostMallocCoherent;
     TRY_ALLOC(hipHostMalloc((void**)&h.slots, sizeof(gfs_rpc_slot) * h.ring_depth, hflags));
     TRY_ALLOC(hipHostMalloc((void**)&h.sq, sizeof(uint64_t) * h.sq_depth, hflags));
+    TRY_ALLOC(hipHostMalloc((void**)&h.closeq, sizeof(int64_t) * h.sq_depth, hflags));
     TRY_ALLOC(hipHostMalloc((void**)&h.cq, sizeof(uint64_t) * h.ring_depth, hflags));
     TRY_ALLOC(hipHostMalloc((void**)&h.staging, ((size_t)h.ring_depth << h.log_page) + kPad, hflags));
     memset(h.slots, 0, sizeof(gfs_rpc_slot) * h.ring_depth);

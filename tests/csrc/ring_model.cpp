@@ -26,15 +26,18 @@ struct Ring {
     gfs_host::RingView rv;
     std::vector<gfs_rpc_slot> slots;
     std::vector<uint64_t> sq;
+    std::vector<int64_t> closeq;
     std::vector<uint64_t> cq;
     std::vector<uint8_t> staging;
     std::mutex mu;
     std::vector<uint32_t> free_slots;
     std::atomic<uint64_t> tail{0};
-    Ring(uint32_t depth, uint32_t log_page) : slots(depth), sq(64 < depth ? depth : 64, ~0ull), cq(depth, 0),
+    // the submission queue holds a request per slot plus up to 64 slot-less closes
+    Ring(uint32_t depth, uint32_t log_page) : slots(depth), sq(depth + 64, ~0ull), cq(depth, 0),
         staging(((size_t)depth << log_page) + 64) {
-        uint32_t q = 1; while (q < sq.size()) q <<= 1; sq.assign(q, ~0ull);
+        uint32_t q = 1; while (q < sq.size()) q <<= 1; sq.assign(q, ~0ull); closeq.assign(q, -1);
         rv.slots = slots.data(); rv.sq = sq.data(); rv.cq = cq.data(); rv.staging = staging.data();
+        rv.closeq = closeq.data();
         rv.sq_depth = q; rv.ring_depth = depth; rv.log_page = log_page;
         for (uint32_t s = 0; s < depth; ++s) free_slots.push_back(s);
     }
@@ -54,6 +57,12 @@ struct Ring {
     void publish(int s) {
         uint64_t t = tail.fetch_add(1);
         __atomic_store_n(&sq[t & (rv.sq_depth - 1)], (t << 16) | (uint64_t)s, __ATOMIC_RELEASE);
+    }
+    // device side of gclose: the fd under the ticket, no slot, no wait
+    void post_close(int64_t fd) {
+        uint64_t t = tail.fetch_add(1);
+        __atomic_store_n(&closeq[t & (rv.sq_depth - 1)], fd, __ATOMIC_RELAXED);
+        __atomic_store_n(&sq[t & (rv.sq_depth - 1)], (t << 16) | (uint64_t)GFS_SQ_NOSLOT, __ATOMIC_RELEASE);
     }
     // device side of a page fill: n <= 64 reads share slot ss[0]'s completion
     // word; each page is checked as soon as its bit is set and its slot (other
@@ -183,8 +192,25 @@ int main(int argc, char** argv) {
     CHECK(ring.call(s, 99, -1, 0, 0, 0) == -EINVAL, "bad op");
     ring.release(s);
     { struct stat sb; stat(wpath.c_str(), &sb); CHECK(sb.st_size == 5, "size after trunc"); }
+    // slot-less closes: 40 opens through the ring, each closed by a request
+    // that carries no slot and is not waited for, from 4 client threads at
+    // once; stopping the service drains them, so every fd is closed after it
+    std::vector<int64_t> lfds;
+    for (int k = 0; k < 40; ++k) {
+        int so = ring.claim();
+        lfds.push_back(ring.call(so, GFS_OP_OPEN, -1, O_RDONLY, 0, 0, path.c_str()));
+        CHECK(lfds.back() >= 0, "open for a slot-less close");
+        ring.release(so);
+    }
+    {
+        std::vector<std::thread> pc;
+        for (int c = 0; c < 4; ++c)
+            pc.emplace_back([&, c]() { for (int k = c; k < 40; k += 4) ring.post_close(lfds[k]); });
+        for (auto& t : pc) t.join();
+    }
     svc.stop();
-    CHECK(svc.counters().requests.load() == (uint64_t)(2 + 3600 + batch_pages + 7), "request count %llu",
+    for (int64_t f : lfds) CHECK(fcntl((int)f, F_GETFD) == -1 && errno == EBADF, "fd %lld still open", (long long)f);
+    CHECK(svc.counters().requests.load() == (uint64_t)(2 + 3600 + batch_pages + 7 + 80), "request count %llu",
           (unsigned long long)svc.counters().requests.load());
     // 256 KiB pages: reads longer than the service's 64 KiB bounce buffer,
     // at any offset, ending at or past end of file
