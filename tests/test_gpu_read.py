@@ -114,6 +114,29 @@ def test_single_read_larger_than_2gib(tmp_path):
         assert ctx.check_invariants() == 0
 
 
+def test_host_descriptors_closed_after_each_launch(tmp_path):
+    """The last gclose posts the host close without waiting for it; by the
+    time the handler returns, the host descriptor is closed: 20 open/read/
+    close launches over 5 files leave the process's descriptor count where it
+    was."""
+    say("test_host_descriptors_closed_after_each_launch")
+    paths = []
+    for i in range(5):
+        p = tmp_path / f"fd{i}.bin"
+        _rand_file(p, 70000 + i, 200 + i)
+        paths.append(str(p))
+    with make_ctx(page_size=65536, cache_bytes=4 << 20) as ctx:
+        read_file(ctx, paths[0], 0, 10)   # first launch: runtime and code objects loaded
+        before = len(os.listdir("/proc/self/fd"))
+        for k in range(20):
+            got, fsize, crc, _ = read_file(ctx, paths[k % 5], 0, 1000)
+            assert (got, crc) == (1000, 0)
+            assert len(os.listdir("/proc/self/fd")) == before, k
+        s = ctx.stats()
+        assert s["rpc_opens"] == s["rpc_closes"] == 21
+        assert ctx.check_invariants() == 0
+
+
 def test_cold_then_warm_counts_hits(tmp_path):
     say("test_cold_then_warm_counts_hits")
     p = tmp_path / "cw.bin"
