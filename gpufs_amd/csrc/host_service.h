@@ -216,20 +216,25 @@ static inline int64_t serve_slot(const RingView& rv, uint32_t s, ServiceCounters
 }
 
 // Try to take the next published request.  Returns true and sets *slot.
-static inline bool try_take(const RingView& rv, std::atomic<uint64_t>& head, uint32_t* slot, uint64_t* ticket) {
+// Takes the next published request.  For a close posted without a slot, the
+// host fd is read from closeq before the ticket is taken: while the ticket is
+// still the head, no later ticket can reuse its closeq entry (the device never
+// has more than sq_depth tickets ahead of the head), and a failed take drops
+// the value.
+static inline bool try_take(const RingView& rv, std::atomic<uint64_t>& head, uint32_t* slot, int64_t* close_fd) {
     uint64_t h = head.load(std::memory_order_relaxed);
     uint64_t e = __atomic_load_n(&rv.sq[h & (rv.sq_depth - 1)], __ATOMIC_ACQUIRE);
     if ((e >> 16) != (h & 0xFFFFFFFFFFFFull)) return false;
+    const uint32_t s = (uint32_t)(e & 0xFFFF);
+    const int64_t fd = s == GFS_SQ_NOSLOT ? __atomic_load_n(&rv.closeq[h & (rv.sq_depth - 1)], __ATOMIC_RELAXED) : -1;
     if (!head.compare_exchange_strong(h, h + 1, std::memory_order_acq_rel)) return false;
-    *slot = (uint32_t)(e & 0xFFFF);
-    *ticket = h;
+    *slot = s;
+    *close_fd = fd;
     return true;
 }
 
-// A close posted without a slot (the device does not wait for it): the host
-// fd sits in closeq under the request's ticket.
-static inline void serve_slotless_close(const RingView& rv, uint64_t ticket, ServiceCounters& ctr) {
-    const int64_t fd = __atomic_load_n(&rv.closeq[ticket & (rv.sq_depth - 1)], __ATOMIC_RELAXED);
+// A close posted without a slot (the device does not wait for it).
+static inline void serve_slotless_close(int64_t fd, ServiceCounters& ctr) {
     if (fd < 0 || ::close((int)fd) != 0) ctr.errors.fetch_add(1, std::memory_order_relaxed);
     ctr.requests.fetch_add(1, std::memory_order_relaxed);
 }
@@ -274,11 +279,11 @@ static inline void worker_loop(RingView rv, std::atomic<uint64_t>* head, std::at
     uint64_t idle = 0;
     for (;;) {
         uint32_t s;
-        uint64_t ticket;
-        if (try_take(rv, *head, &s, &ticket)) {
+        int64_t close_fd;
+        if (try_take(rv, *head, &s, &close_fd)) {
             if (wk->sleepers.load(std::memory_order_relaxed) && backlog(rv, *head)) wake_all(wk);
             const uint64_t t0 = mono_ns();
-            if (s == GFS_SQ_NOSLOT) serve_slotless_close(rv, ticket, *ctr);
+            if (s == GFS_SQ_NOSLOT) serve_slotless_close(close_fd, *ctr);
             else serve_slot(rv, s, *ctr, wb);
             ctr->busy_ns.fetch_add(mono_ns() - t0, std::memory_order_relaxed);
             idle = 0;
