@@ -84,6 +84,7 @@ struct wg_ctl {
     uint32_t ra;             // gread: extra pages fetched after the request (read-ahead)
     uint32_t gen[64];        // resolve_batch: generation of each HIT / OWN frame
     uint32_t memo_f;         // gread: frame pinned through the page memo at entry, or GFS_NOFRAME
+    uint32_t memo_f1;        // gread: the next page's frame when the read runs into it, or GFS_NOFRAME
 };
 
 // Per-workgroup read history for sequential read-ahead.  It is advisory: it
@@ -94,21 +95,25 @@ struct wg_ctl {
 // never a wrong byte.  (A tag built from the dispatch and kernel-argument
 // pointers kept four more SGPRs live through every caller's loops and pushed
 // the grep scan's window loop into scratch spills.)
-// The page the workgroup's last gread ended in: a following read inside the
-// same page pins that frame in gread's first round trip and skips the radix
-// walk (a grep-style loop reads a 256 KiB page in four 64 KiB windows).  The
-// pin only counts if the frame still carries `gen` and is VALID and the file
-// still has `version`: a frame that was evicted or reassigned has a new
-// generation, and truncation or a reopen after the file changed bumps the
-// version.  gopen clears it; a memo left in LDS by an earlier launch is checked
-// the same way, and one left by other code fails the frame-range and owner-key
-// checks.
+// The pages of the workgroup's last gread batch, read-ahead pages included:
+// their frames and generations stay in gfs_ctl.frame / gen (kind HIT or OWN)
+// until another batch reuses the arrays.  A following read inside one of
+// those pages pins that frame in gread's first round trip and skips the radix
+// walk (a grep-style loop reads a 256 KiB page in four 64 KiB windows, and the
+// pages its read-ahead brought in).  The pin only counts if the frame still
+// carries the recorded generation, is VALID and is owned by this file and
+// page, and the file still has `version`: a frame that was evicted or
+// reassigned has a new generation, and truncation or a reopen after the file
+// changed bumps the version.  Arrays overwritten by another call's batch can
+// only name a frame that fails those checks or truly holds the page.  gopen
+// clears the memo; one left in LDS by an earlier launch or by other code is
+// bounded to the arrays and checked the same way.
 struct wg_memo {
     int32_t fd;         // -1: none
-    uint32_t frame;
-    uint32_t gen;
+    uint32_t cnt;       // pages of the batch (at most 64)
     uint32_t version;
-    uint64_t page;
+    uint32_t pad;
+    uint64_t pb;        // first page of the batch
 };
 struct wg_hist {
     int32_t fd;         // -1: no history
@@ -1145,34 +1150,63 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
         // also pins that page's frame in the same round trip (wg_memo).
         int64_t r = 0;
         uint64_t fs = 0, root = 0;
-        uint32_t mf = GFS_NOFRAME;
+        uint32_t mf = GFS_NOFRAME, mf1 = GFS_NOFRAME;
         if (!fd_valid(g, fd)) r = -EBADF;
         else {
             gfs_file* e = &g->files[fd];
             const uint64_t pin = g->page_size - (offset & (g->page_size - 1));
-            // LDS is not cleared between launches: a memo this launch did not
-            // write is only used if its frame index is in range, and the
-            // frame's owner key must name this page
-            const bool try_memo = gfs_memo.fd == fd && size > 0 && size <= pin &&
-                                  (offset >> g->log_page) == gfs_memo.page && gfs_memo.frame < g->n_frames;
+            // A read of one page, or running into the next, whose pages are
+            // both in the memo batch.  LDS is not cleared between launches: a
+            // memo this launch did not write is bounded to the batch arrays,
+            // used only if its frame indexes are in range, and each frame's
+            // owner key must name its page.
+            const uint64_t p0 = offset >> g->log_page;
+            const uint64_t mi = p0 - gfs_memo.pb;   // the first page's lane in the memo batch
+            const bool two = size > pin;
+            bool try_memo = gfs_memo.fd == fd && size > 0 && (!two || size - pin <= g->page_size) && mi < 63 &&
+                            mi + (two ? 2 : 1) <= gfs_memo.cnt;
+            uint32_t mfr0 = GFS_NOFRAME, mfr1 = GFS_NOFRAME, mgen0 = 0, mgen1 = 0;
+            if (try_memo) {
+                const uint32_t k0 = gfs_ctl.kind[mi];
+                mfr0 = gfs_ctl.frame[mi];
+                mgen0 = gfs_ctl.gen[mi];
+                try_memo = (k0 == GFS_K_HIT || k0 == GFS_K_OWN) && mfr0 < g->n_frames;
+                if (two) {
+                    const uint32_t k1 = gfs_ctl.kind[mi + 1];
+                    mfr1 = gfs_ctl.frame[mi + 1];
+                    mgen1 = gfs_ctl.gen[mi + 1];
+                    try_memo = try_memo && (k1 == GFS_K_HIT || k1 == GFS_K_OWN) && mfr1 < g->n_frames;
+                }
+            }
             const uint64_t st = ld_rlx(&e->st);
             fs = e->size;
             root = ld_rlx(&e->root);
-            uint64_t mm = 0, mk = 0;
+            uint64_t mm0 = 0, mk0 = 0, mm1 = 0, mk1 = 0;
             if (try_memo) {
-                mm = fadd_rlx(&g->meta[gfs_memo.frame], 1ull);
-                mk = g->fkey[gfs_memo.frame];
+                mm0 = fadd_rlx(&g->meta[mfr0], 1ull);
+                mk0 = g->fkey[mfr0];
+                if (two) {
+                    mm1 = fadd_rlx(&g->meta[mfr1], 1ull);
+                    mk1 = g->fkey[mfr1];
+                }
             }
-            fence_acq_agent();   // the nodes under `root` (and the pinned frame) are read after it
+            fence_acq_agent();   // the nodes under `root` (and the pinned frames) are read after it
             version = fst_version(st);
             if (fst_state(st) != GFS_FS_OPEN) r = -EBADF;
             else r = offset >= fs ? 0 : (int64_t)(size < fs - offset ? size : fs - offset);
             if (try_memo) {
-                if (r > 0 && version == gfs_memo.version && meta_gen(mm) == gfs_memo.gen &&
-                    meta_state(mm) == GFS_F_VALID && mk == gfs_key((uint32_t)fd, gfs_memo.page))
-                    mf = gfs_memo.frame;
-                else
-                    fadd_rlx(&g->meta[gfs_memo.frame], ~0ull);
+                bool ok = r > 0 && version == gfs_memo.version && meta_gen(mm0) == mgen0 &&
+                          meta_state(mm0) == GFS_F_VALID && mk0 == gfs_key((uint32_t)fd, p0);
+                if (two)
+                    ok = ok && meta_gen(mm1) == mgen1 && meta_state(mm1) == GFS_F_VALID &&
+                         mk1 == gfs_key((uint32_t)fd, p0 + 1);
+                if (ok) {
+                    mf = mfr0;
+                    mf1 = two ? mfr1 : GFS_NOFRAME;
+                } else {
+                    fadd_rlx(&g->meta[mfr0], ~0ull);
+                    if (two) fadd_rlx(&g->meta[mfr1], ~0ull);
+                }
             }
         }
         // read-ahead: the third and later forward-sequential reads of a file
@@ -1208,6 +1242,7 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
         gfs_ctl.b = r > 0 ? root : 0;
         gfs_ctl.ra = ext;
         gfs_ctl.memo_f = mf;
+        gfs_ctl.memo_f1 = mf1;
     }
     __syncthreads();
     const int64_t n = uni(gfs_ctl.ret);
@@ -1219,17 +1254,30 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
     const uint32_t L = g->log_page;
     const uint64_t PS = g->page_size;
     if (memo_f != GFS_NOFRAME) {
-        // One page, pinned and checked at entry (thread 0's acquire, then the
-        // barrier): copy, meet once, and thread 0 drops the pin.  The memo
-        // already names this page.
-        copy_seg<true, true, buffer_space, SPACE_GLOBAL>(buffer, frame_ptr(g, memo_f) + (offset & (PS - 1)),
-                                                         (uint64_t)n, tid(), nthreads());
+        // One page, or two, pinned and checked at entry (thread 0's acquire,
+        // then the barrier): copy, meet once, and thread 0 drops the pins.
+        const uint32_t memo_f1 = uni(gfs_ctl.memo_f1);
+        const uint64_t in0 = PS - (offset & (PS - 1));
+        const uint64_t n0 = (uint64_t)n < in0 ? (uint64_t)n : in0;
+        copy_seg<true, true, buffer_space, SPACE_GLOBAL>(buffer, frame_ptr(g, memo_f) + (offset & (PS - 1)), n0,
+                                                         tid(), nthreads());
+        if ((uint64_t)n > n0)   // the rest lies in the next page (memo_f1 is set whenever n > n0)
+            copy_seg<true, true, buffer_space, SPACE_GLOBAL>(buffer + n0, frame_ptr(g, memo_f1), (uint64_t)n - n0,
+                                                             tid(), nthreads());
         __syncthreads();
         if (tid() == 0) {
             fadd(&g->meta[memo_f], ~0ull);
             g->fref[memo_f] = 1;
-            gfs_ctl.hits += 1;
-            count(&g->ctr.page_hits, 1);
+            uint32_t pages = 1;
+            if (memo_f1 != GFS_NOFRAME) {
+                fadd(&g->meta[memo_f1], ~0ull);
+                if ((uint64_t)n > n0) {
+                    g->fref[memo_f1] = 1;
+                    pages = 2;
+                }
+            }
+            gfs_ctl.hits += pages;
+            count(&g->ctr.page_hits, pages);
         }
         GFS_TRACE(6);
         return (size_t)n;
@@ -1280,15 +1328,12 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
             __syncthreads();
             GFS_TRACE(3);
             if (tid() == 0 && pb + creq - 1 == p_last) {
-                // the page this read ends in becomes the memo for the next one
-                const uint32_t k = gfs_ctl.kind[creq - 1];
-                if (k == GFS_K_HIT || k == GFS_K_OWN) {
-                    gfs_memo.fd = fd;
-                    gfs_memo.frame = gfs_ctl.frame[creq - 1];
-                    gfs_memo.gen = gfs_ctl.gen[creq - 1];
-                    gfs_memo.version = version;
-                    gfs_memo.page = p_last;
-                }
+                // this read's last batch (with its read-ahead pages) becomes the
+                // memo for the next reads; its lanes keep kind / frame / gen
+                gfs_memo.fd = fd;
+                gfs_memo.pb = pb;
+                gfs_memo.cnt = cnt;
+                gfs_memo.version = version;
             }
             if (wave() == 0) {
                 const uint32_t ln = lane();
