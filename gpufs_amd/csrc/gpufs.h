@@ -373,30 +373,46 @@ __device__ inline uint64_t node_alloc(gfs_dev* g) {
 // interior nodes on the way when `create`.  0 = absent (or -ENOMEM in *err).
 // `root_hint` (nonzero) is a root word read earlier by the caller: a tree only
 // grows by pushing the old root down as child 0, so an older root still holds
-// every page it covered; one too small for `leaf` fails the growth CAS below
-// and the current word is used.
+// every page it covered.  One too small for `leaf` is not grown from: the
+// current root is read first (a caller that retries with the same old hint
+// would otherwise allocate, and lose, a growth node on every retry, and the
+// node pool is a bump allocator).  A growth node lost to a racing workgroup
+// is kept for this call's next growth step.
 __device__ inline uint64_t tree_leaf(gfs_dev* g, gfs_file* f, uint64_t leaf, bool create, int* err,
                                      uint64_t root_hint = 0) {
     const uint64_t page = leaf << GFS_FANOUT_LOG;
     uint64_t rw = root_hint ? root_hint : ld_acq(&f->root);
+    bool fresh = root_hint == 0;
+    uint64_t spare = 0;
     for (int guard = 0; guard < 16; ++guard) {
         uint32_t h = (uint32_t)(rw >> 40);
         if (rw != 0 && page < tree_cap(h)) break;
+        if (!fresh) {
+            rw = ld_acq(&f->root);
+            fresh = true;
+            continue;
+        }
         if (!create) return 0;
-        uint64_t n = node_alloc(g);
+        uint64_t n = spare ? spare : node_alloc(g);
         if (!n) { *err = -ENOMEM; return 0; }
         uint64_t nw;
         if (rw == 0) {
             uint32_t nh = 1;
             while (page >= tree_cap(nh)) ++nh;
+            st_rlx(&g->nodes[n * GFS_FANOUT], 0ull);   // a spare may hold an earlier attempt's child
             nw = n | ((uint64_t)nh << 40);
         } else {
             st_rlx(&g->nodes[n * GFS_FANOUT], rw & ((1ull << 40) - 1));
             nw = n | ((uint64_t)(h + 1) << 40);
         }
         uint64_t exp = rw;
-        if (cas(&f->root, exp, nw)) rw = nw;
-        else rw = exp;   // lost the race; the unused node stays zero
+        if (cas(&f->root, exp, nw)) {
+            rw = nw;
+            spare = 0;
+        } else {
+            rw = exp;   // lost the race: keep the node for the next step, if one is needed
+            spare = n;
+        }
     }
     uint32_t h = (uint32_t)(rw >> 40);
     uint64_t node = rw & ((1ull << 40) - 1);
@@ -607,10 +623,12 @@ __device__ inline void rpc_post_close(gfs_dev* g, int32_t host_fd) {
 }
 // Wait until every bit of `want` is set in completion mask `cq`.  Returns 0
 // or -ETIMEDOUT.
+// Polls with relaxed loads and acquires once at the end: a system-scope
+// acquire load also invalidates the caches on every poll.
 __device__ inline int rpc_wait(gfs_dev* g, uint32_t cq, uint64_t want) {
     uint64_t t0 = now_ticks();
     uint32_t spins = 0;
-    while ((sys_ld_acq(&g->cq[cq]) & want) != want) {
+    while ((sys_ld(&g->cq[cq]) & want) != want) {
         if ((++spins & 255) == 0 && now_ticks() - t0 > g->wait_ticks) return -ETIMEDOUT;
         backoff();
     }
@@ -1046,7 +1064,9 @@ __device__ inline void fill_from_host(gfs_dev* g, uint32_t cnt, uint64_t pb, uin
             uint32_t spins = 0;
             uint64_t m;
             gfs_ctl.tmo = 0u;
-            while (((m = sys_ld_acq(&g->cq[cq])) & own & ~done) == 0) {
+            // relaxed polls, one acquire once reads have arrived (an acquire
+            // load at system scope would invalidate the caches on every poll)
+            while (((m = sys_ld(&g->cq[cq])) & own & ~done) == 0) {
                 if ((++spins & 255) == 0 && now_ticks() - t0 > g->wait_ticks) {
                     gfs_ctl.tmo = 1u;
                     count(&g->ctr.timeouts, 1);
@@ -1055,6 +1075,7 @@ __device__ inline void fill_from_host(gfs_dev* g, uint32_t cnt, uint64_t pb, uin
                 }
                 backoff();
             }
+            fence_acq_sys();
             gfs_ctl.a = m & own & ~done;
         }
         __syncthreads();

@@ -137,6 +137,28 @@ def test_host_descriptors_closed_after_each_launch(tmp_path):
         assert ctx.check_invariants() == 0
 
 
+def test_radix_nodes_not_lost_to_retries(tmp_path):
+    """256 workgroups read a cold 1 GiB file through 32 ring slots and a
+    64 MiB cache, so most batches wait and retry many times while the radix
+    tree grows to three levels under them.  A retry must not allocate radix
+    nodes again from the root it read at entry: with a node pool of 2,000 (the
+    tree needs 261, and lost creation races a few hundred more), every range
+    still reads in full, pass after pass."""
+    say("test_radix_nodes_not_lost_to_retries")
+    n, rb = 256, 4 << 20
+    p = tmp_path / "sparse_1g.bin"
+    with open(p, "wb") as f:
+        f.truncate(n * rb)
+    with make_ctx(page_size=65536, cache_bytes=64 << 20, ring_depth=32, tree_nodes=2000) as ctx:
+        # before the fix, two of five such passes ended in -ENOMEM
+        for k in range(6):
+            ctx.cache_drop()
+            out, dst = gread_ranges(ctx, str(p), n, rb, threads=256)
+            assert (out == rb).all(), (k, sorted(set(out.tolist())))
+            assert int(dst.count_nonzero()) == 0
+        assert ctx.check_invariants() == 0
+
+
 def test_cold_then_warm_counts_hits(tmp_path):
     say("test_cold_then_warm_counts_hits")
     p = tmp_path / "cw.bin"
