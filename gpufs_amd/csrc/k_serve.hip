@@ -60,11 +60,14 @@ __global__ __launch_bounds__(1024) void k_serve(const char* names, uint32_t stri
             uint64_t t0 = gfs::now_ticks();
             uint32_t spins = 0;
             fid = SERVE_STOP;
-            while (gfs::sys_ld_acq(&e->seq) != k + 1) {
+            // relaxed polls (a system-scope acquire load invalidates the
+            // caches each time), one acquire when the request is there
+            while (gfs::sys_ld(&e->seq) != k + 1) {
                 if (++spins > 64) __builtin_amdgcn_s_sleep(127);
                 else __builtin_amdgcn_s_sleep(8);
                 if ((spins & 63) == 0 && gfs::now_ticks() - t0 > idle_ticks) break;
             }
+            gfs::fence_acq_sys();
             if (gfs::sys_ld(&e->seq) == k + 1) {
                 fid = gfs::sys_ld(&e->file_id);
                 slot = gfs::sys_ld(&e->slot);
