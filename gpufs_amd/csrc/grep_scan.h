@@ -32,7 +32,6 @@ struct grep_lds {
     uint16_t canon[GREP_MAX_WORDS];          // 0xFFFF: word can never match
     uint8_t wlen[GREP_MAX_WORDS];
     uint32_t lenmask;                        // bit L-1: some query word has length L
-    int ov;
     int status;
     uint64_t fsize;
 };
@@ -184,8 +183,8 @@ __device__ inline uint64_t grep_seg_mask(const uint8_t* corpus, uint32_t base, u
     return am;
 }
 
-// Count tokens of window [0, dsz) of S.corpus.  corpus[dsz-1] is never a word
-// byte, so every token ends inside the window.
+// Count tokens of window [0, dsz) of S.corpus.  Bytes from dsz on count as
+// separators, so every token ends inside the window.
 // Each lane classifies one 64-byte segment and walks its tokens.  The loop is
 // wave-uniform so that lanes can trade masks: a token that runs past its
 // segment's end takes the rest of its length from the next segment's mask (the
@@ -265,20 +264,18 @@ __device__ inline int grep_file_windows(grep_lds& S, int fd, uint32_t chunk, OnW
             if (t == 0) S.status = (int64_t)got < 0 ? (int)(int64_t)got : -EIO;
             break;
         }
-        // the word that straddles the chunk end: its bytes in the look-ahead
-        // (at most 32), found by one ballot of wave 0 over the look-ahead
-        if (gfs::wave() == 0) {
+        // The word that straddles the chunk end: its bytes in the look-ahead
+        // (at most 32), found by one ballot over the look-ahead in every wave
+        // (each reads the same bytes, so no barrier is needed to share it).
+        // The scan stops before the byte after that word, which gread did not
+        // necessarily write when the look-ahead is all word bytes.
+        uint32_t ov = 0;
+        {
             const uint32_t ln = gfs::lane();
             const bool stop = last || ln >= 32 || !g_alpha(S.corpus[mis + chunk + ln]);
-            const uint64_t m = __ballot(stop) | (1ull << 32);
-            if (ln == 0) {
-                S.corpus[mis + left] = 0;
-                S.ov = (int)__builtin_ctzll(m);
-            }
+            ov = gfs::uni((uint32_t)__builtin_ctzll(__ballot(stop) | (1ull << 32)));
         }
-        __syncthreads();
-        const uint32_t ov = gfs::uni((uint32_t)S.ov);
-        const uint32_t dsz = mis + (last ? left + 1 : chunk + ov + 1);
+        const uint32_t dsz = mis + (last ? left : chunk + ov);
         cursor += (uint64_t)chunk + ov;
         [[clang::always_inline]] grep_scan_window(S, dsz, mis);
         __syncthreads();
