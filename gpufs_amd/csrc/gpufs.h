@@ -1287,11 +1287,14 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
                                                              tid(), nthreads());
         __syncthreads();
         if (tid() == 0) {
-            fadd(&g->meta[memo_f], ~0ull);
+            // every load of the frames has returned (the barrier waited for the
+            // copy's stores, which used them): a relaxed add that nothing
+            // waits for drops the pins, and thread 0 is not held up
+            fadd_rlx(&g->meta[memo_f], ~0ull);
             g->fref[memo_f] = 1;
             uint32_t pages = 1;
             if (memo_f1 != GFS_NOFRAME) {
-                fadd(&g->meta[memo_f1], ~0ull);
+                fadd_rlx(&g->meta[memo_f1], ~0ull);
                 if ((uint64_t)n > n0) {
                     g->fref[memo_f1] = 1;
                     pages = 2;
