@@ -85,6 +85,11 @@ struct wg_ctl {
     uint32_t gen[64];        // resolve_batch: generation of each HIT / OWN frame
     uint32_t memo_f;         // gread: frame pinned through the page memo at entry, or GFS_NOFRAME
     uint32_t memo_f1;        // gread: the next page's frame when the read runs into it, or GFS_NOFRAME
+    // gread's entry results (thread 0 writes them before gread's first barrier,
+    // so no other call reads them after its own last barrier)
+    int64_t gr_ret;
+    uint64_t gr_fs, gr_root;
+    uint32_t gr_ra;
 };
 
 // Per-workgroup read history for sequential read-ahead.  It is advisory: it
@@ -1161,7 +1166,10 @@ template <gfs::buf_space buffer_space = gfs::SPACE_ANY>
 __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer) {
     using namespace gfs;
     gfs_dev* g = gfs_g;
-    __syncthreads();
+    // No barrier before thread 0's round trip: it writes only its own entry
+    // fields, which no wave reads after the barrier that ends the previous
+    // call, and it overlaps the other waves' last work before this call; the
+    // barrier after it also keeps `buffer` until every wave is done with it.
     GFS_TRACE(5);
     uint32_t version = 0;   // thread 0: the file's version at entry
     if (tid() == 0) {
@@ -1258,20 +1266,25 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
             gfs_hist.last = pl;
             gfs_hist.fetched = fetched;
         }
-        gfs_ctl.ret = r;
-        gfs_ctl.a = r > 0 ? fs : 0;
-        gfs_ctl.b = r > 0 ? root : 0;
-        gfs_ctl.ra = ext;
+        gfs_ctl.gr_ret = r;
+        gfs_ctl.gr_fs = r > 0 ? fs : 0;
+        gfs_ctl.gr_root = r > 0 ? root : 0;
+        gfs_ctl.gr_ra = ext;
         gfs_ctl.memo_f = mf;
         gfs_ctl.memo_f1 = mf1;
     }
     __syncthreads();
-    const int64_t n = uni(gfs_ctl.ret);
-    const uint64_t fsize = uni(gfs_ctl.a);
-    const uint64_t root_hint = uni(gfs_ctl.b);
-    const uint32_t ra_ext = uni(gfs_ctl.ra);
+    const int64_t n = uni(gfs_ctl.gr_ret);
+    const uint64_t fsize = uni(gfs_ctl.gr_fs);
+    const uint64_t root_hint = uni(gfs_ctl.gr_root);
+    const uint32_t ra_ext = uni(gfs_ctl.gr_ra);
     const uint32_t memo_f = uni(gfs_ctl.memo_f);   // n > 0 whenever a frame was pinned
-    if (n <= 0) return (size_t)n;
+    if (n <= 0) {
+        // every wave has read the entry fields before thread 0 may rewrite them
+        // in a next gread (which writes them before its first barrier)
+        __syncthreads();
+        return (size_t)n;
+    }
     const uint32_t L = g->log_page;
     const uint64_t PS = g->page_size;
     if (memo_f != GFS_NOFRAME) {
