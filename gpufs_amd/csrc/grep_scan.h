@@ -16,6 +16,7 @@
 // the token's remaining words.  A token's bytes and a query word's bytes
 // compare equal iff the strings are equal, so the counts are exact.
 #pragma once
+#include <type_traits>
 #include "gpufs.h"
 
 #define GREP_MAX_CHUNK 65536
@@ -278,7 +279,11 @@ __device__ inline int grep_file_windows(grep_lds& S, int fd, uint32_t chunk, OnW
         const uint32_t dsz = mis + (last ? left : chunk + ov);
         cursor += (uint64_t)chunk + ov;
         [[clang::always_inline]] grep_scan_window(S, dsz, mis);
-        __syncthreads();
+        // A callback that reads the counts waits for every wave's scan.  With
+        // none, the next gread's barrier keeps the window until every wave is
+        // done with it, and thread 0's round trip for that read overlaps the
+        // other waves' scans.
+        if constexpr (!std::is_empty<typename std::remove_reference<OnWindow>::type>::value) __syncthreads();
         GFS_TRACE(7);
         on_window();
     }
