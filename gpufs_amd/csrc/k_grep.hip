@@ -42,7 +42,10 @@ __global__ __launch_bounds__(1024) void k_grep_batched(const char* names, uint32
     __shared__ __attribute__((aligned(16))) grep_lds S;
     __shared__ unsigned int cur;
     grep_build_dict(S, dict, nw);
-    for (;;) {
+    // Every take returns a distinct index, so a workgroup runs at most nfiles
+    // + 1 rounds; the bound makes that explicit, so the loop ends even if the
+    // queue word were corrupted.
+    for (uint32_t round = 0; round <= nfiles; ++round) {
         if (gfs::tid() == 0) cur = atomicAdd(queue, 1u);
         __syncthreads();
         const unsigned int f = gfs::uni(cur);

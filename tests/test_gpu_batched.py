@@ -22,10 +22,6 @@ def _corpus(tmp_path, n, seed):
     return paths, q
 
 
-@pytest.mark.skip(reason="held off the GPU: it hung twice (persistent loop lowered as a divergent lane-mask loop: "
-                         "after each file the lanes other than thread 0 re-entered the loop without it); fixed by a "
-                         "loop-end barrier and readfirstlane-broadcast loop controls, checked in the gfx950 ISA, "
-                         "not yet run on the GPU since")
 def test_batched_and_graph_match_cpu_and_plain(tmp_path, orc):
     say("test_batched_and_graph_match_cpu_and_plain")
     torch = torch_cuda()
