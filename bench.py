@@ -54,8 +54,9 @@ def parse():
     ap.add_argument("--cold-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grep", action="store_true")
-    ap.add_argument("--no-batched", action="store_true",
-                    help="skip the batched leg (10,000 uneven files, one hipGraph-captured persistent kernel)")
+    ap.add_argument("--batched", action="store_true",
+                    help="run the batched leg (10,000 uneven files, one hipGraph-captured persistent kernel); off by "
+                         "default while a GPU fault of its kernel is unexplained")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--batched-files", type=int, default=10000)
     ap.add_argument("--no-gemv", action="store_true",
@@ -285,7 +286,7 @@ def main():
     ctx.close()
     del dst
     torch.cuda.empty_cache()
-    batched_res = bench_batched(args, torch, L, _lib, dg) if not args.no_batched else None
+    batched_res = bench_batched(args, torch, L, _lib, dg) if args.batched else None
     gemv_res = bench_gemv(args, torch, L, _lib, dg) if not args.no_gemv else None
     serve_res = bench_serve(args, torch, L, _lib, dg) if not args.no_serve else None
     line = {

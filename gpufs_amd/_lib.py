@@ -32,7 +32,7 @@ STATS_FIELDS = [
     "page_hits", "page_misses", "page_waits", "evictions", "alloc_fail", "rpc_reads", "rpc_writes",
     "rpc_opens", "rpc_closes", "bytes_filled", "bytes_written", "nodes_used", "files_used",
     "host_requests", "host_bytes_read", "host_bytes_written", "frames", "frames_valid", "timeouts",
-    "host_busy_ns", "vm_errors",
+    "host_busy_ns", "vm_errors", "slots_lost",
 ]
 
 
@@ -83,6 +83,7 @@ SIGNATURES = {
     "gpufs_fnv1a64_ranges": (I, [P, U32, U64, P, P]),
     "gpufs_launch_read": (I, [P, P, I32, U64, U64, P, P, U32, P]),
     "gpufs_launch_write": (I, [P, P, I32, U64, U64, P, I32, P, U32, P]),
+    "gpufs_launch_api_script": (I, [P, P, P, U32, P, P, U32, P]),
 }
 
 _LIB = None

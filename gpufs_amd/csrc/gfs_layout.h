@@ -29,9 +29,9 @@
 #define E_FSTABLE_FULL (-1)
 #define E_IPC_OPEN_ERROR (-2)
 
-#define GFS_NAME_MAX 128
+#define GFS_NAME_MAX 128                // bytes incl. NUL (reference: FILENAME_SIZE 64)
 #define GFS_NAME_RESERVED 0xFFFFFFFFu   // name-index slot taken, entry not yet published
-#define GFS_LEAF_RESERVED (~0ull)       // radix leaf slot claimed by a workgroup, frame not yet published          // bytes incl. NUL (reference: FILENAME_SIZE 64)
+#define GFS_LEAF_RESERVED (~0ull)       // radix leaf slot claimed by a workgroup, frame not yet published
 // Ring slot bitmap words sit 128 bytes apart: every claim and release is an
 // atomic on them, and packed into one cache line they all queue at one L2
 // channel (which also carries its share of every other stream).
@@ -110,6 +110,7 @@ typedef struct gfs_counters {
     unsigned long long bytes_filled, bytes_written;
     unsigned long long timeouts;
     unsigned long long vm_errors;
+    unsigned long long slots_lost;   // ring slots quarantined after a timed-out request
 } gfs_counters;
 
 typedef struct gfs_dev {

@@ -577,6 +577,15 @@ __device__ inline int rpc_claim_wave(gfs_dev* g, bool need, uint32_t hint) {
     }
     return mine;
 }
+// A request whose wait timed out can still be served by the host later: it
+// would pread into the slot's staging page, store the slot's result and set a
+// bit in the slot's completion word.  Such a slot is never given back to the
+// ring (it stays claimed until the context is destroyed), so no later request
+// can meet those late writes, and every ticket outstanding at the host still
+// holds a slot (the submission queue's depth relies on that).
+__device__ __forceinline__ void rpc_quarantine(gfs_dev* g, uint32_t n) {
+    if (n) fadd_rlx(&g->ctr.slots_lost, (unsigned long long)n);
+}
 __device__ __forceinline__ void rpc_release(gfs_dev* g, int s) {
     __hip_atomic_fetch_or(slot_word(g, (uint32_t)s >> 6), 1ull << (s & 63), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -668,7 +677,13 @@ __device__ inline int64_t rpc_call_single(gfs_dev* g, uint32_t op, int32_t host_
     GFS_TRACE(21);
     int w = rpc_wait(g, (uint32_t)s, 1ull);
     GFS_TRACE(22);
-    int64_t res = w ? (int64_t)w : sys_ld(&g->slots[s].result);
+    if (w) {
+        // the host still owns the slot: it is not returned, and no result is read from it
+        rpc_quarantine(g, 1);
+        if (slot_out) *slot_out = -1;
+        return (int64_t)w;
+    }
+    const int64_t res = sys_ld(&g->slots[s].result);
     if (slot_out) *slot_out = s;   // caller reads extra results, then releases
     else rpc_release(g, s);
     return res;
@@ -1101,6 +1116,16 @@ __device__ inline void fill_from_host(gfs_dev* g, uint32_t cnt, uint64_t pb, uin
                     g->fbytes[gfs_ctl.frame[ln]] = (uint32_t)r;
                 }
             }
+            if (tmo) {
+                // Requests not answered yet may still be served: their slots and
+                // the slot holding the batch's completion word are quarantined
+                // (fail_own / finish_batch skip a slot index of all ones).
+                const bool lost = ((got >> ln) & 1ull) ||
+                                  (((own >> ln) & 1ull) && ln < 64 && gfs_ctl.slot[ln] == cq);
+                if (lost) gfs_ctl.slot[ln] = 0xFFFFFFFFu;
+                const uint64_t lm = __ballot(lost);
+                if (ln == 0) rpc_quarantine(g, popc(lm));
+            }
         }
         __syncthreads();
         for (uint64_t m = got; m; m &= m - 1) {
@@ -1503,90 +1528,260 @@ __device__ inline size_t gwrite(int fd, size_t offset, size_t size, uchar* buffe
 }
 
 namespace gfs {
+// Pins the frame of leaf word `v` if it holds a live, VALID, dirty page and
+// clears the page's dirty flag; returns the frame (pinned) or GFS_NOFRAME.
+// The flag is cleared before the page is copied out: a gwrite after that
+// point sets it again, so its bytes are written back by a later flush.
+__device__ inline uint32_t take_dirty(gfs_dev* g, uint64_t v) {
+    if (v == 0 || v == GFS_LEAF_RESERVED) return GFS_NOFRAME;
+    const uint32_t f = leaf_frame(v);
+    const uint64_t old = fadd_rlx(&g->meta[f], 1ull);
+    if (meta_gen(old) == leaf_gen(v) && meta_state(old) == GFS_F_VALID && (old & GFS_FL_DIRTY)) {
+        __hip_atomic_fetch_and(&g->meta[f], ~GFS_FL_DIRTY, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        return f;
+    }
+    fadd_rlx(&g->meta[f], ~0ull);
+    return GFS_NOFRAME;
+}
+// A page whose write-back did not happen is marked dirty again, so it is
+// neither evicted nor forgotten: a later gfsync / gclose retries it.
+__device__ __forceinline__ void redirty(gfs_dev* g, uint32_t f) {
+    __hip_atomic_fetch_or(&g->meta[f], GFS_FL_DIRTY, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Write back every dirty page of `fd` (workgroup-collective).  Pages of
 // O_GWRONCE files are OR-merged with the bytes on disk by the host
-// (libgpufs/host_loop.h:76-137).  Returns 0 or the first error.
+// (libgpufs/host_loop.h:76-137).  The dirty pages of one radix leaf (up to 64)
+// travel together: wave 0 pins them and claims a ring slot for each, the
+// workgroup copies them into staging, every page's WRITE is posted at once on
+// one shared completion mask, and thread 0 waits for that mask once.  A dirty
+// page that found no free slot is taken in a later round.  Returns 0 or the
+// first error; pages whose write failed stay dirty.
 __device__ inline int flush_file(gfs_dev* g, int fd) {
+    gfs_file* file = &g->files[fd];
+    const bool wronce = (file->flags & (O_CREAT | O_ACCMODE)) == O_GWRONCE;
+    const uint64_t size = file->size;
+    const int32_t host_fd = file->host_fd;
+    const uint64_t npages = (size + g->page_size - 1) >> g->log_page;
+    const uint32_t T = nthreads(), t = tid();
+    if (t == 0) gfs_memo.fd = -1;   // the batch arrays are reused below
+    int err = 0;
+    for (uint64_t leaf = 0; (leaf << GFS_FANOUT_LOG) < npages && !err; ++leaf) {
+        const uint64_t base = leaf << GFS_FANOUT_LOG;
+        const uint64_t in_leaf = npages - base < 64 ? npages - base : 64;
+        uint64_t pending = in_leaf == 64 ? ~0ull : ((1ull << in_leaf) - 1);
+        const uint64_t t0 = now_ticks();
+        uint32_t idle = 0;
+        while (pending && !err) {
+            __syncthreads();
+            if (wave() == 0) {
+                const uint32_t ln = lane();
+                int e = 0;
+                uint64_t node = 0;
+                if (ln == 0) node = tree_leaf(g, file, leaf, false, &e);
+                node = __shfl(node, 0);
+                const bool mine = node && ((pending >> ln) & 1ull);
+                uint32_t f = mine ? take_dirty(g, ld_rlx(&g->nodes[node * GFS_FANOUT + ln])) : GFS_NOFRAME;
+                const int s = rpc_claim_wave(g, f != GFS_NOFRAME, blockIdx.x * 3u);
+                const bool noslot = f != GFS_NOFRAME && s < 0;
+                if (noslot) {   // stays dirty and pending
+                    redirty(g, f);
+                    fadd_rlx(&g->meta[f], ~0ull);
+                    f = GFS_NOFRAME;
+                }
+                const bool post = f != GFS_NOFRAME;
+                const uint64_t postm = __ballot(post);
+                uint32_t bytes = 0;
+                if (post) {
+                    const uint64_t poff = (base + ln) << g->log_page;
+                    const uint64_t b = g->fbytes[f];
+                    const uint64_t lim = size > poff ? size - poff : 0;
+                    bytes = (uint32_t)(b < lim ? b : lim);
+                }
+                fence_acq_agent();
+                gfs_ctl.kind[ln] = post ? GFS_K_OWN : GFS_K_DONE;
+                gfs_ctl.frame[ln] = f;
+                gfs_ctl.slot[ln] = post ? (uint32_t)s : 0xFFFFFFFFu;
+                gfs_ctl.gen[ln] = bytes;   // bytes to write (the arrays' generation field is free here)
+                const uint32_t cq = postm ? (uint32_t)__shfl(s, (int)first_lane(postm)) : 0u;
+                if (postm && ln == first_lane(postm)) sys_st(&g->cq[cq], 0ull);
+                const uint64_t again = __ballot(noslot);
+                if (ln == 0) {
+                    gfs_ctl.own = postm;
+                    gfs_ctl.cq = cq;
+                    gfs_ctl.a = again;
+                }
+            }
+            __syncthreads();
+            const uint64_t postm = uni(gfs_ctl.own);
+            for (uint64_t m = postm; m; m &= m - 1) {
+                const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1;
+                copy_seg<false, false, SPACE_GLOBAL, SPACE_GLOBAL, 4>(staging_ptr(g, gfs_ctl.slot[i]),
+                                                                     frame_ptr(g, gfs_ctl.frame[i]), gfs_ctl.gen[i],
+                                                                     t, T);
+            }
+            __threadfence_system();   // staging bytes before the requests that point at them
+            __syncthreads();
+            if (wave() == 0) {
+                const uint32_t ln = lane();
+                int perr = 0;
+                if (postm) {
+                    const uint32_t cq = uni(gfs_ctl.cq);
+                    const bool post = (postm >> ln) & 1ull;
+                    if (post)
+                        rpc_post(g, (int)gfs_ctl.slot[ln], wronce ? GFS_OP_DIFF : GFS_OP_WRITE, host_fd, 0,
+                                 (base + ln) << g->log_page, gfs_ctl.gen[ln], cq | (ln << 16));
+                    wave_count(&g->ctr.rpc_writes, post);
+                    int w = 0;
+                    if (ln == 0) w = rpc_wait(g, cq, postm);
+                    w = uni(w);
+                    // after a timeout: the requests answered before the limit
+                    const uint64_t answered = w ? uni((uint64_t)sys_ld(&g->cq[cq])) & postm : postm;
+                    fence_acq_sys();
+                    if (post) {
+                        const uint32_t f = gfs_ctl.frame[ln];
+                        const bool got = (answered >> ln) & 1ull;
+                        const int64_t r = got ? sys_ld(&g->slots[gfs_ctl.slot[ln]].result) : (int64_t)-ETIMEDOUT;
+                        if (r < 0 || (uint64_t)r != gfs_ctl.gen[ln]) {
+                            perr = r < 0 ? (int)r : -EIO;
+                            redirty(g, f);
+                        } else {
+                            count(&g->ctr.bytes_written, (unsigned long long)r);
+                        }
+                        fadd(&g->meta[f], ~0ull);   // unpin
+                    }
+                    // Slots go back to the ring, except after a timeout those the
+                    // host may still answer: the unanswered ones and the one
+                    // holding the completion word.
+                    const bool lost = post && w && (!((answered >> ln) & 1ull) || gfs_ctl.slot[ln] == cq);
+                    rpc_release_wave(g, post && !lost, post ? (int)gfs_ctl.slot[ln] : 0);
+                    const uint64_t lm = __ballot(lost);
+                    if (ln == 0) rpc_quarantine(g, popc(lm));
+                }
+                const uint64_t em = __ballot(perr != 0);
+                const int first = em ? __shfl(perr, (int)first_lane(em)) : 0;
+                if (ln == 0) gfs_ctl.ret = first;
+            }
+            __syncthreads();
+            err = (int)uni(gfs_ctl.ret);
+            const uint64_t nxt = uni(gfs_ctl.a);
+            if (!err && nxt) {
+                // dirty pages found no free slot: wait for slots, bounded
+                if (postm == 0) {
+                    if (++idle > 4) backoff();
+                    if ((idle & 63) == 0 && wg_timed_out(g, t0)) err = -ETIMEDOUT;
+                } else {
+                    idle = 0;
+                }
+            }
+            pending = nxt;
+        }
+    }
+    __syncthreads();
+    return err;
+}
+
+// Single-thread write-back of every dirty page of `fd` (one thread, the rest
+// of its wave may be inactive): page by page through one ring slot.
+__device__ inline int flush_file_single(gfs_dev* g, int fd) {
     gfs_file* file = &g->files[fd];
     const bool wronce = (file->flags & (O_CREAT | O_ACCMODE)) == O_GWRONCE;
     const uint64_t size = file->size;
     const uint64_t npages = (size + g->page_size - 1) >> g->log_page;
     int err = 0;
-    for (uint64_t leaf = 0; (leaf << GFS_FANOUT_LOG) < npages && !err; ++leaf) {
-        __syncthreads();
-        if (wave() == 0) {
-            const uint32_t ln = lane();
-            int e = 0;
-            uint64_t node = 0;
-            if (ln == 0) node = tree_leaf(g, file, leaf, false, &e);
-            node = __shfl(node, 0);
-            uint64_t page = (leaf << GFS_FANOUT_LOG) + ln;
-            bool dirty = false;
-            uint32_t f = GFS_NOFRAME;
-            if (node && page < npages) {
-                uint64_t v = ld_rlx(&g->nodes[node * GFS_FANOUT + ln]);
-                if (v) {
-                    f = leaf_frame(v);
-                    uint64_t old = fadd_rlx(&g->meta[f], 1ull);
-                    if (meta_gen(old) == leaf_gen(v) && meta_state(old) == GFS_F_VALID && (old & GFS_FL_DIRTY)) {
-                        dirty = true;
-                        __hip_atomic_fetch_and(&g->meta[f], ~GFS_FL_DIRTY, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-                    } else {
-                        fadd_rlx(&g->meta[f], ~0ull);
-                    }
-                }
-            }
-            fence_acq_agent();
-            gfs_ctl.kind[ln] = dirty ? GFS_K_OWN : GFS_K_DONE;
-            gfs_ctl.frame[ln] = f;
-        }
-        __syncthreads();
-        for (uint32_t i = 0; i < 64; ++i) {
-            if (uni((uint32_t)gfs_ctl.kind[i]) != GFS_K_OWN) continue;
-            const uint32_t f = gfs_ctl.frame[i];
-            const uint64_t page = (leaf << GFS_FANOUT_LOG) + i;
-            uint64_t poff = page << g->log_page;
-            uint64_t bytes = g->fbytes[f];
-            uint64_t lim = size > poff ? size - poff : 0;
-            if (bytes > lim) bytes = lim;
-            if (tid() == 0) {
-                int s = -1;
-                uint64_t t0 = now_ticks();
-                while ((s = rpc_claim(g, true, blockIdx.x)) < 0) {
-                    if (now_ticks() - t0 > g->wait_ticks) break;
-                    backoff();
-                }
-                gfs_ctl.slot[i] = s < 0 ? 0xFFFFFFFFu : (uint32_t)s;
-            }
-            __syncthreads();
-            const uint32_t s = uni(gfs_ctl.slot[i]);
-            if (s != 0xFFFFFFFFu) {
-                wg_copy(staging_ptr(g, s), frame_ptr(g, f), bytes);
+    for (uint64_t leaf = 0; (leaf << GFS_FANOUT_LOG) < npages; ++leaf) {
+        int e = 0;
+        const uint64_t node = tree_leaf(g, file, leaf, false, &e);
+        if (!node) continue;
+        for (uint32_t i = 0; i < GFS_FANOUT && (leaf << GFS_FANOUT_LOG) + i < npages; ++i) {
+            const uint32_t f = take_dirty(g, ld_rlx(&g->nodes[node * GFS_FANOUT + i]));
+            if (f == GFS_NOFRAME) continue;
+            const uint64_t poff = ((leaf << GFS_FANOUT_LOG) + i) << g->log_page;
+            const uint64_t b = g->fbytes[f], lim = size > poff ? size - poff : 0;
+            const uint64_t bytes = b < lim ? b : lim;
+            int s = -1;
+            const uint64_t t0 = now_ticks();
+            while ((s = rpc_claim(g, true, blockIdx.x)) < 0 && now_ticks() - t0 <= g->wait_ticks) backoff();
+            int64_t r = -ETIMEDOUT;
+            if (s >= 0) {
+                fence_acq_agent();
+                copy_seg<false, false, SPACE_GLOBAL, SPACE_GLOBAL, 4>(staging_ptr(g, (uint32_t)s), frame_ptr(g, f),
+                                                                     bytes, 0, 1);
                 __threadfence_system();
-            }
-            __syncthreads();
-            if (tid() == 0) {
-                if (s == 0xFFFFFFFFu) {
-                    err = -ETIMEDOUT;
+                sys_st(&g->cq[s], 0ull);
+                rpc_post(g, s, wronce ? GFS_OP_DIFF : GFS_OP_WRITE, file->host_fd, 0, poff, bytes, (uint32_t)s);
+                count(&g->ctr.rpc_writes, 1);
+                const int w = rpc_wait(g, (uint32_t)s, 1ull);
+                if (w) {
+                    rpc_quarantine(g, 1);
+                    r = w;
                 } else {
-                    sys_st(&g->cq[s], 0u);
-                    rpc_post(g, (int)s, wronce ? GFS_OP_DIFF : GFS_OP_WRITE, file->host_fd, 0, poff, bytes, s);
-                    count(&g->ctr.rpc_writes, 1);
-                    count(&g->ctr.bytes_written, bytes);
-                    int w = rpc_wait(g, s, 1ull);
-                    int64_t r = w ? (int64_t)w : sys_ld(&g->slots[s].result);
-                    if (r < 0 || (uint64_t)r != bytes) err = r < 0 ? (int)r : -EIO;
-                    rpc_release(g, (int)s);
+                    r = sys_ld(&g->slots[s].result);
+                    rpc_release(g, s);
                 }
-                fadd(&g->meta[f], ~0ull);
-                gfs_ctl.ret = err;
             }
-            __syncthreads();
-            err = (int)uni(gfs_ctl.ret);
+            if (r < 0 || (uint64_t)r != bytes) {
+                redirty(g, f);
+                if (!err) err = r < 0 ? (int)r : -EIO;
+            } else {
+                count(&g->ctr.bytes_written, bytes);
+            }
+            fadd(&g->meta[f], ~0ull);
         }
     }
-    __syncthreads();
     return err;
+}
+
+// Forgets every cached page of `fd` (one thread): dirty flags are cleared, the
+// radix tree is dropped and the file gets a new version, so no reader can meet
+// the old pages (their frames are reclaimed by the CLOCK sweep).  Used by
+// truncation, and by a last close whose write-back failed (the bytes that
+// could not be written are not kept as cached data the disk does not hold).
+__device__ inline void discard_pages(gfs_dev* g, int fd) {
+    gfs_file* e = &g->files[fd];
+    const uint64_t npages = (e->size + g->page_size - 1) >> g->log_page;
+    for (uint64_t leaf = 0; (leaf << GFS_FANOUT_LOG) < npages; ++leaf) {
+        int er = 0;
+        const uint64_t node = tree_leaf(g, e, leaf, false, &er);
+        if (!node) continue;
+        for (uint32_t i = 0; i < GFS_FANOUT; ++i) {
+            const uint32_t f = take_dirty(g, ld_rlx(&g->nodes[node * GFS_FANOUT + i]));
+            if (f != GFS_NOFRAME) fadd(&g->meta[f], ~0ull);
+        }
+    }
+    st_rlx(&e->root, 0ull);
+    fadd(&e->st, 1ull << 32);   // version + 1 (fstate.h: every change is an added delta)
+}
+
+// Host truncation of open file `fd` to `size` (one thread), then its cached
+// pages are dropped as in the reference (libgpufs/fs_calls.cu:127-140).  The
+// caller writes dirty pages below `size` back first.
+__device__ inline int truncate_single(gfs_dev* g, int fd, int64_t size) {
+    gfs_file* e = &g->files[fd];
+    const int64_t c = rpc_call_single(g, GFS_OP_TRUNC, e->host_fd, 0, (uint64_t)size, 0, nullptr, nullptr);
+    if (c < 0) return (int)c;
+    discard_pages(g, fd);
+    e->size = (uint64_t)size;
+    return 0;
+}
+
+// The last close of `fd` after its write-back (result `r`): the host close is
+// posted, not waited for; on a failed write-back the pages are discarded.
+__device__ inline void close_last(gfs_dev* g, int fd, int r) {
+    gfs_file* e = &g->files[fd];
+    if (r) discard_pages(g, fd);
+    // the host closes its fd after the caller has moved on: nothing is
+    // written through it any more (the write-back waited for its writes)
+    rpc_post_close(g, e->host_fd);
+    count(&g->ctr.rpc_closes, 1);
+    e->host_fd = -1;
+    e->dirty = 0;
+    fst_dev P{0, 0};
+    fst_close_done(&e->st, P);
+}
+
+__device__ __forceinline__ bool fd_open(gfs_dev* g, int fd) {
+    return fd_valid(g, fd) && fst_state(ld_acq(&g->files[fd].st)) == GFS_FS_OPEN;
 }
 } // namespace gfs
 
@@ -1595,7 +1790,7 @@ __device__ inline int gfsync(int fd) {
     using namespace gfs;
     gfs_dev* g = gfs_g;
     __syncthreads();
-    if (tid() == 0) gfs_ctl.ret = (fd_valid(g, fd) && fst_state(ld_acq(&g->files[fd].st)) == GFS_FS_OPEN) ? 0 : -EBADF;
+    if (tid() == 0) gfs_ctl.ret = fd_open(g, fd) ? 0 : -EBADF;
     __syncthreads();
     if (uni(gfs_ctl.ret)) return (int)uni(gfs_ctl.ret);
     int r = flush_file(g, fd);
@@ -1632,14 +1827,7 @@ __device__ inline int gclose(int fd) {
     int r = uni(gfs_ctl.flag) ? flush_file(g, fd) : 0;
     __syncthreads();
     if (tid() == 0) {
-        // the host closes its fd after the workgroup has moved on: nothing is
-        // written through it any more (the flush above waited for its writes)
-        rpc_post_close(g, e->host_fd);
-        count(&g->ctr.rpc_closes, 1);
-        e->host_fd = -1;
-        e->dirty = 0;
-        fst_dev P{0, 0};
-        fst_close_done(&e->st, P);
+        close_last(g, fd, r);
         gfs_ctl.ret = r;
     }
     __syncthreads();
@@ -1647,32 +1835,80 @@ __device__ inline int gclose(int fd) {
 }
 
 // libgpufs/fs_calls.cu:127-149.  The reference supports only size 0; any size
-// is accepted here.  Cached pages are dropped (new version) as in the reference.
+// is accepted here.  Dirty pages are written back first when the new size
+// keeps some of them, then the file is truncated on the host and its cached
+// pages are dropped (new version) as in the reference.
 __device__ inline int gftruncate(int fd, int size) {
     using namespace gfs;
     gfs_dev* g = gfs_g;
     __syncthreads();
-    if (tid() == 0) {
-        int r = 0;
-        if (!fd_valid(g, fd) || fst_state(ld_acq(&g->files[fd].st)) != GFS_FS_OPEN || size < 0) r = -EBADF;
-        else {
-            gfs_file* e = &g->files[fd];
-            int64_t c = rpc_call_single(g, GFS_OP_TRUNC, e->host_fd, 0, (uint64_t)size, 0, nullptr, nullptr);
-            if (c < 0) r = (int)c;
-            else {
-                e->size = (uint64_t)size;
-                st_rlx(&e->root, 0ull);
-                for (;;) {
-                    uint64_t st = ld_acq(&e->st);
-                    uint64_t exp = st;
-                    if (cas(&e->st, exp, fst_make(fst_state(st), fst_opens(st), fst_version(st) + 1))) break;
-                }
-            }
-        }
-        gfs_ctl.ret = r;
-    }
+    if (tid() == 0) gfs_ctl.ret = (fd_open(g, fd) && size >= 0) ? 0 : -EBADF;
+    __syncthreads();
+    int r = (int)uni(gfs_ctl.ret);
+    if (r) return r;
+    if (size > 0) r = flush_file(g, fd);
+    __syncthreads();
+    if (tid() == 0) gfs_ctl.ret = r ? r : truncate_single(g, fd, size);
     __syncthreads();
     return (int)uni(gfs_ctl.ret);
+}
+
+// ---- single-thread calls (libgpufs/fs_calls.cu.h:34-36): callable by one
+// thread while the rest of its wave and workgroup do something else (the
+// reference's samples call them inside their BEGIN_SINGLE_THREAD sections,
+// samples/grep_text/grep_text_gpu.cu:258-268).  They touch no LDS state of the
+// workgroup-collective calls, and move pages with that thread alone.
+
+// libgpufs/fs_calls.cu:152-225.  Same results as gopen.
+__device__ inline int single_thread_open(const char* filename, int flags) {
+    return gfs::open_single(filename, flags);
+}
+
+// libgpufs/fs_calls.cu:127-140.  0 on success or a negative code (the
+// reference returns truncate_cpu()==0, 1 on success, and only then skips its
+// cache drop).  Any size is accepted; dirty pages below it are written first.
+__device__ inline int single_thread_ftruncate(int fd, int size) {
+    using namespace gfs;
+    gfs_dev* g = gfs_g;
+    if (!fd_open(g, fd) || size < 0) return -EBADF;
+    if (size > 0) {
+        const int r = flush_file_single(g, fd);
+        if (r) return r;
+    }
+    return truncate_single(g, fd, size);
+}
+
+// Declared at libgpufs/fs_calls.cu.h:34 (and called by
+// samples/grep_text/grep_text_gpu_mmap.cu:427) but never defined there: writes
+// back the file's dirty pages, like gfsync, from one thread.
+__device__ inline int single_thread_fsync(int fd) {
+    using namespace gfs;
+    gfs_dev* g = gfs_g;
+    if (!fd_open(g, fd)) return -EBADF;
+    const int r = flush_file_single(g, fd);
+    if (r == 0) g->files[fd].dirty = 0;
+    return r;
+}
+
+// Called by samples/grep_text/grep_text_gpu_mmap.cu:259-260,413-429 (not
+// declared in the reference's headers): gclose from one thread.
+__device__ inline int single_thread_close(int fd) {
+    using namespace gfs;
+    gfs_dev* g = gfs_g;
+    if (!fd_valid(g, fd)) return -EBADF;
+    fst_dev P{0, 0};
+    const int c = fst_close(&g->files[fd].st, P);
+    if (c <= 0) return c;
+    const int r = g->files[fd].dirty ? flush_file_single(g, fd) : 0;
+    close_last(g, fd, r);
+    return r;
+}
+
+// libgpufs/fs_calls.cu:845-850: not implemented in the reference either (it
+// asserts and returns 0); here it reports -ENOSYS.
+__device__ inline uint gunlink(char* filename) {
+    (void)filename;
+    return (uint)(-ENOSYS);
 }
 
 // libgpufs/fs_calls.cu:422-481.  Pins the page holding `offset` and returns a
@@ -1876,18 +2112,23 @@ __device__ inline volatile void* gmmap_warp(void* addr, size_t size, int prot, i
                         const uint64_t poff = page << L;
                         uint64_t want = fsize > poff ? fsize - poff : 0;
                         if (want > g->page_size) want = g->page_size;
+                        int w = 0;
                         if (ln == 0) {
                             sys_st(&g->cq[s], 0ull);
                             rpc_post(g, s, GFS_OP_READ, host_fd, 0, poff, want, (uint32_t)s);
                             count(&g->ctr.rpc_reads, 1);
-                            const int w = rpc_wait(g, (uint32_t)s, 1ull);
+                            w = rpc_wait(g, (uint32_t)s, 1ull);
                             r = w ? (int64_t)w : sys_ld(&g->slots[s].result);
                             if (r >= 0 && (uint64_t)r != want) r = -EIO;
                         }
                         r = uni(r);
                         if (r >= 0) copy_seg<true, false>(frame_ptr(g, nf), staging_ptr(g, (uint32_t)s), (uint64_t)r, ln, 64);
-                        // the wave's staging loads have all returned (their data was stored)
-                        if (ln == 0) rpc_release(g, s);
+                        // the wave's staging loads have all returned (their data was stored);
+                        // a request that timed out keeps its slot (rpc_quarantine)
+                        if (ln == 0) {
+                            if (w) rpc_quarantine(g, 1);
+                            else rpc_release(g, s);
+                        }
                     } else {
                         uint8_t* fp = frame_ptr(g, nf);
                         for (uint64_t k = ln; k < (g->page_size >> 4); k += 64) *(v4u*)(fp + 16 * k) = v4u{0, 0, 0, 0};
@@ -1984,11 +2225,22 @@ __device__ inline int gmsync(volatile void* addr, size_t length, int flags) {
         sys_st(&g->cq[s], 0u);
         rpc_post(g, (int)s, wronce ? GFS_OP_DIFF : GFS_OP_WRITE, file->host_fd, 0, poff, bytes, s);
         count(&g->ctr.rpc_writes, 1);
-        count(&g->ctr.bytes_written, bytes);
-        int w = rpc_wait(g, s, 1ull);
-        int64_t r = w ? (int64_t)w : sys_ld(&g->slots[s].result);
-        rpc_release(g, (int)s);
-        gfs_ctl.ret = (r < 0 || (uint64_t)r != bytes) ? (r < 0 ? r : -EIO) : 0;
+        const int w = rpc_wait(g, s, 1ull);
+        int64_t r;
+        if (w) {   // the host may still answer: the slot is not returned
+            rpc_quarantine(g, 1);
+            r = w;
+        } else {
+            r = sys_ld(&g->slots[s].result);
+            rpc_release(g, (int)s);
+        }
+        if (r < 0 || (uint64_t)r != bytes) {
+            redirty(g, (uint32_t)f);   // not on disk: the page stays dirty
+            gfs_ctl.ret = r < 0 ? r : -EIO;
+        } else {
+            count(&g->ctr.bytes_written, bytes);
+            gfs_ctl.ret = 0;
+        }
     }
     __syncthreads();
     return (int)uni(gfs_ctl.ret);

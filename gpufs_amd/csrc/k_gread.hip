@@ -132,3 +132,92 @@ extern "C" int gpufs_fnv1a64_ranges(const void* data_dev, uint32_t n_ranges, uin
                        (const uint8_t*)data_dev, n_ranges, range_bytes, out_dev);
     return hipGetLastError() == hipSuccess ? 0 : -EIO;
 }
+
+// A script of device API calls run by one workgroup on one file, for the API
+// tests: record k is ops[4k .. 4k+3] = {op, a, b, c}, its result goes to
+// results[k].  Collective calls are made by every thread; the single-thread
+// calls by thread 0 alone while the others wait at a barrier, the way the
+// reference's samples call them (samples/grep_text/grep_text_gpu.cu:258-268).
+// `buf` is the device buffer GREAD / GWRITE use at byte offset c.
+enum api_op : int64_t {
+    API_GOPEN = 1,        // a = flags -> fd (kept as the current descriptor)
+    API_GREAD = 2,        // a = offset, b = length, c = buffer offset -> bytes
+    API_GWRITE = 3,       // same -> bytes
+    API_GFSYNC = 4,
+    API_GFTRUNCATE = 5,   // a = size
+    API_GCLOSE = 6,
+    API_FSTAT = 7,
+    API_ST_OPEN = 8,      // a = flags -> fd
+    API_ST_FTRUNCATE = 9, // a = size
+    API_ST_FSYNC = 10,
+    API_ST_CLOSE = 11,
+    API_GMMAP_FILL = 12,  // a = offset, b = length (inside one page), c = byte: gmmap, fill, gmsync, gmunmap -> gmsync's result
+    API_GUNLINK = 13,
+};
+
+__global__ __launch_bounds__(1024) void k_api_script(const char* path, const int64_t* ops, uint32_t n_ops,
+                                                     uint8_t* buf, int64_t* results) {
+    __shared__ int64_t st_ret;
+    int fd = -1;
+    const uint32_t t = gfs::tid();
+    for (uint32_t k = 0; k < n_ops; ++k) {
+        // every thread reads the same record; readfirstlane makes the branches scalar
+        const int64_t op = gfs::uni(ops[4 * k]), a = gfs::uni(ops[4 * k + 1]), b = gfs::uni(ops[4 * k + 2]),
+                      c = gfs::uni(ops[4 * k + 3]);
+        int64_t r = 0;
+        switch (op) {
+        case API_GOPEN: r = fd = gopen(path, (int)a); break;
+        case API_GREAD: r = (int64_t)gread(fd, (size_t)a, (size_t)b, buf + c); break;
+        case API_GWRITE: r = (int64_t)gwrite(fd, (size_t)a, (size_t)b, buf + c); break;
+        case API_GFSYNC: r = gfsync(fd); break;
+        case API_GFTRUNCATE: r = gftruncate(fd, (int)a); break;
+        case API_GCLOSE: r = gclose(fd); break;
+        case API_FSTAT: r = (int64_t)fstat(fd); break;
+        case API_GMMAP_FILL: {
+            volatile uint8_t* p = (volatile uint8_t*)gmmap(nullptr, (size_t)b, 0, 0, fd, (off_t)a);
+            if (p == (volatile uint8_t*)GMAP_FAILED) {
+                r = -EIO;
+                break;
+            }
+            for (int64_t i = t; i < b; i += gfs::nthreads()) p[i] = (uint8_t)c;
+            __syncthreads();
+            r = gmsync(p, (size_t)b, 0);
+            const int u = gmunmap(p, (size_t)b);
+            if (r == 0) r = u;
+            break;
+        }
+        default: {
+            // single-thread calls: thread 0 only
+            __syncthreads();
+            if (t == 0) {
+                int64_t v = -EINVAL;
+                if (op == API_ST_OPEN) v = single_thread_open(path, (int)a);
+                else if (op == API_ST_FTRUNCATE) v = single_thread_ftruncate(fd, (int)a);
+                else if (op == API_ST_FSYNC) v = single_thread_fsync(fd);
+                else if (op == API_ST_CLOSE) v = single_thread_close(fd);
+                else if (op == API_GUNLINK) v = (int32_t)gunlink((char*)path);
+                st_ret = v;
+            }
+            __syncthreads();
+            r = gfs::uni((uint64_t)st_ret);
+            if (op == API_ST_OPEN) fd = (int)r;
+            __syncthreads();
+        }
+        }
+        if (t == 0) results[k] = r;
+        // keeps this thread-0 region apart from the next record's (see k_grep_batched)
+        __syncthreads();
+    }
+}
+
+extern "C" int gpufs_launch_api_script(gpufs_ctx* ctx, const char* path_dev, const int64_t* ops_dev,
+                                       uint32_t n_ops, void* buf_dev, int64_t* results_dev, uint32_t threads,
+                                       void* stream) {
+    if (!ctx || !path_dev || !ops_dev || !results_dev || n_ops == 0) return -EINVAL;
+    int v = check_threads(threads);
+    if (v) return v;
+    if ((v = gpufs_bind(ctx))) return v;
+    hipLaunchKernelGGL(k_api_script, dim3(1), dim3(threads), 0, pick_stream(ctx, stream), path_dev, ops_dev, n_ops,
+                       (uint8_t*)buf_dev, results_dev);
+    return hipGetLastError() == hipSuccess ? 0 : -EIO;
+}

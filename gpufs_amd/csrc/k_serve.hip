@@ -45,8 +45,9 @@ __global__ __launch_bounds__(1024) void k_serve(const char* names, uint32_t stri
                                                 const char* dict, uint32_t nw, uint32_t chunk,
                                                 serve_req* ring, uint32_t ring_mask, uint32_t* results,
                                                 int32_t* res_status, uint32_t* res_pages, uint64_t* done,
-                                                unsigned long long* ticket, uint64_t idle_ticks, uint32_t* held,
-                                                unsigned int* held_n, uint32_t held_cap) {
+                                                unsigned long long* ticket, const uint32_t* halt,
+                                                uint64_t claim_ticks, uint32_t* held, unsigned int* held_n,
+                                                uint32_t held_cap) {
     __shared__ __attribute__((aligned(16))) grep_lds S;
     __shared__ uint32_t fid, slot;
     __shared__ uint64_t myseq;
@@ -57,15 +58,18 @@ __global__ __launch_bounds__(1024) void k_serve(const char* names, uint32_t stri
         if (t == 0) {
             unsigned long long k = atomicAdd(ticket, 1ull);
             serve_req* e = &ring[k & ring_mask];
-            uint64_t t0 = gfs::now_ticks();
             uint32_t spins = 0;
             fid = SERVE_STOP;
-            // relaxed polls (a system-scope acquire load invalidates the
-            // caches each time), one acquire when the request is there
+            // Waits as long as the server runs, however long it idles: a ticket
+            // once taken is served.  The wait ends when the request is published
+            // or when the host sets the halt word (gpufs_serve_stop, after it has
+            // published a stop request for every workgroup).  Relaxed polls (a
+            // system-scope acquire load invalidates the caches each time), one
+            // acquire when the request is there.
             while (gfs::sys_ld(&e->seq) != k + 1) {
                 if (++spins > 64) __builtin_amdgcn_s_sleep(127);
                 else __builtin_amdgcn_s_sleep(8);
-                if ((spins & 63) == 0 && gfs::now_ticks() - t0 > idle_ticks) break;
+                if ((spins & 63) == 0 && gfs::sys_ld(halt) != 0u) break;
             }
             gfs::fence_acq_sys();
             if (gfs::sys_ld(&e->seq) == k + 1) {
@@ -89,7 +93,7 @@ __global__ __launch_bounds__(1024) void k_serve(const char* names, uint32_t stri
                     uint32_t hv;
                     while ((hv = __hip_atomic_load(&held[x], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) ==
                                SERVE_HOLD_CLAIM &&
-                           gfs::now_ticks() - c0 < idle_ticks)
+                           gfs::now_ticks() - c0 < claim_ticks)
                         __builtin_amdgcn_s_sleep(64);
                     hold = hv;
                 }
@@ -158,6 +162,7 @@ struct gpufs_server {
     int32_t* rstatus;    // host coherent: slots
     uint32_t* rpages;    // host coherent: slots x 2 (page hits, misses of the request)
     uint64_t* done;      // host coherent: slots
+    uint32_t* halt;      // host coherent: set by gpufs_serve_stop, ends every workgroup's request wait
     uint64_t hits, misses;   // summed over completed requests
     unsigned long long* ticket;   // device
     uint32_t* held;               // device, per file: 0, SERVE_HOLD_CLAIM, or the server's open fd + 1
@@ -214,6 +219,19 @@ static void serve_publish(gpufs_server* S, uint32_t file_id, uint32_t slot) {
     __atomic_store_n(&e->seq, k + 1, __ATOMIC_RELEASE);
 }
 
+static void serve_free(gpufs_server* S) {
+    (void)hipHostFree(S->ring);
+    (void)hipHostFree(S->results);
+    (void)hipHostFree(S->rstatus);
+    (void)hipHostFree(S->rpages);
+    (void)hipHostFree(S->done);
+    (void)hipHostFree(S->halt);
+    (void)hipFree(S->ticket);
+    (void)hipFree(S->held);
+    (void)hipFree(S->held_n);
+    delete S;
+}
+
 extern "C" int gpufs_serve_start(gpufs_ctx* ctx, const gpufs_serve_config* cfg, const char* names_dev,
                                  uint32_t name_stride, uint32_t n_files, const uint64_t* file_sizes,
                                  const char* dict_dev, gpufs_server** out) {
@@ -248,34 +266,40 @@ extern "C" int gpufs_serve_start(gpufs_ctx* ctx, const gpufs_serve_config* cfg, 
               hipHostMalloc((void**)&S->rstatus, sizeof(int32_t) * cfg->slots, hf) == hipSuccess &&
               hipHostMalloc((void**)&S->rpages, sizeof(uint32_t) * 2 * (size_t)cfg->slots, hf) == hipSuccess &&
               hipHostMalloc((void**)&S->done, sizeof(uint64_t) * cfg->slots, hf) == hipSuccess &&
+              hipHostMalloc((void**)&S->halt, sizeof(uint32_t), hf) == hipSuccess &&
               hipMalloc((void**)&S->ticket, sizeof(unsigned long long)) == hipSuccess &&
               hipMalloc((void**)&S->held, sizeof(uint32_t) * (size_t)n_files) == hipSuccess &&
               hipMalloc((void**)&S->held_n, sizeof(unsigned int)) == hipSuccess;
     if (!ok) {
-        (void)hipHostFree(S->ring); (void)hipHostFree(S->results); (void)hipHostFree(S->rstatus);
-        (void)hipHostFree(S->rpages); (void)hipHostFree(S->done); (void)hipFree(S->ticket);
-        (void)hipFree(S->held); (void)hipFree(S->held_n);
-        delete S;
+        serve_free(S);
         return -ENOMEM;
     }
     memset(S->ring, 0, sizeof(serve_req) * rs);
     memset(S->done, 0, sizeof(uint64_t) * cfg->slots);
+    *S->halt = 0;
+    gfs_dev hd;
     if (hipMemsetAsync(S->ticket, 0, sizeof(unsigned long long), S->stream) != hipSuccess ||
         hipMemsetAsync(S->held, 0, sizeof(uint32_t) * (size_t)n_files, S->stream) != hipSuccess ||
-        hipMemsetAsync(S->held_n, 0, sizeof(unsigned int), S->stream) != hipSuccess)
+        hipMemsetAsync(S->held_n, 0, sizeof(unsigned int), S->stream) != hipSuccess ||
+        hipMemcpyAsync(&hd, gpufs_device_state(ctx), sizeof(hd), hipMemcpyDeviceToHost, S->stream) != hipSuccess ||
+        hipStreamSynchronize(S->stream) != hipSuccess) {
+        serve_free(S);
         return -EIO;
+    }
     S->next_ticket = 0;
     gpufs_host_start(ctx);
-    // idle workgroups give up after twice the RPC wait limit without a request
-    gfs_dev hd;
-    if (hipMemcpy(&hd, gpufs_device_state(ctx), sizeof(hd), hipMemcpyDeviceToHost) != hipSuccess) return -EIO;
-    const uint64_t idle = hd.wait_ticks * 2;
     const uint32_t hold_cap = serve_hold_cap();
     serve_reserve_fds(hold_cap);
+    // at stop, a workgroup waits at most the RPC wait limit for another one's
+    // open of a held file to resolve
     hipLaunchKernelGGL(k_serve, dim3(S->workgroups), dim3(S->threads), 0, S->stream, names_dev, name_stride,
                        n_files, dict_dev, cfg->n_words, cfg->chunk_bytes, S->ring, S->ring_mask, S->results,
-                       S->rstatus, S->rpages, S->done, S->ticket, idle, S->held, S->held_n, hold_cap);
-    if (hipGetLastError() != hipSuccess) return -EIO;
+                       S->rstatus, S->rpages, S->done, S->ticket, (const uint32_t*)S->halt, hd.wait_ticks, S->held,
+                       S->held_n, hold_cap);
+    if (hipGetLastError() != hipSuccess) {
+        serve_free(S);
+        return -EIO;
+    }
     S->running = true;
     gpufs_resident_add(ctx, 1);
     *out = S;
@@ -306,6 +330,7 @@ extern "C" int gpufs_serve_run(gpufs_server* S, const uint32_t* file_ids, const 
     };
     uint64_t last_done = t0;
     uint64_t guard_t = t0;
+    uint64_t polls = 0;
     while (completed < n) {
         const uint64_t now = serve_now_ns();
         while (next_arrival < n && (double)(now - t0) * 1e-9 >= arrival_s[next_arrival]) pending.push_back(next_arrival++);
@@ -344,6 +369,9 @@ extern "C" int gpufs_serve_run(gpufs_server* S, const uint32_t* file_ids, const 
             ++inflight;
         }
         if (inflight > 0 && serve_now_ns() - guard_t > 120ull * 1000000000ull) return -ETIMEDOUT;
+        // the server's workgroups never leave while it runs: a finished (or
+        // failed) kernel means no request can be served any more
+        if (inflight > 0 && (++polls & 1023) == 0 && hipStreamQuery(S->stream) != hipErrorNotReady) return -EIO;
         if (inflight == 0) guard_t = serve_now_ns();
     }
     if (elapsed_s_out) *elapsed_s_out = (double)(last_done - t0) * 1e-9 - (n ? arrival_s[0] : 0.0);
@@ -355,6 +383,9 @@ extern "C" int gpufs_serve_stop(gpufs_server* S) {
     int rc = 0;
     if (S->running) {
         for (uint32_t w = 0; w < S->workgroups; ++w) serve_publish(S, SERVE_STOP, 0);
+        // every waiting workgroup also sees the halt word (it ends the wait of a
+        // ticket no stop request was published for)
+        __atomic_store_n(S->halt, 1u, __ATOMIC_RELEASE);
         // bounded wait for the workgroups to leave
         const uint64_t t0 = serve_now_ns();
         hipError_t e;
@@ -366,15 +397,7 @@ extern "C" int gpufs_serve_stop(gpufs_server* S) {
         S->running = false;
         gpufs_resident_add(S->ctx, -1);
     }
-    (void)hipHostFree(S->ring);
-    (void)hipHostFree(S->results);
-    (void)hipHostFree(S->rstatus);
-    (void)hipHostFree(S->rpages);
-    (void)hipHostFree(S->done);
-    (void)hipFree(S->ticket);
-    (void)hipFree(S->held);
-    (void)hipFree(S->held_n);
-    delete S;
+    serve_free(S);
     return rc;
 }
 

@@ -230,6 +230,8 @@ extern "C" int gpufs_init(const gpufs_config* cfg_in, gpufs_ctx** out) {
 
 extern "C" int gpufs_destroy(gpufs_ctx* c) {
     if (!c) return -EINVAL;
+    // a resident server's workgroups wait for requests until gpufs_serve_stop
+    REFUSE_IF_RESIDENT(c);
     c->service.stop();
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
@@ -353,6 +355,7 @@ extern "C" int gpufs_stats_get(gpufs_ctx* c, gpufs_stats* o) {
     o->timeouts = cur.ctr.timeouts;
     o->host_busy_ns = hc.busy_ns.load();
     o->vm_errors = cur.ctr.vm_errors;
+    o->slots_lost = cur.ctr.slots_lost;
     return 0;
 }
 
@@ -385,13 +388,13 @@ extern "C" int gpufs_check_invariants(gpufs_ctx* c) {
     std::vector<gfs_file> files(nf);
     if (nf) HIP_TRY(hipMemcpy(files.data(), h.files, sizeof(gfs_file) * nf, hipMemcpyDeviceToHost));
     {
-        // every ring slot is free (a request that timed out may keep its slot:
-        // the host can still write its staging page)
+        // every ring slot is free but those quarantined after a timed-out
+        // request (the host can still write their staging pages)
         std::vector<uint64_t> bits((size_t)h.slot_words * GFS_SLOT_WORD_STRIDE);
         HIP_TRY(hipMemcpy(bits.data(), h.slot_free, bits.size() * 8, hipMemcpyDeviceToHost));
         uint64_t nfree = 0;
         for (uint32_t w = 0; w < h.slot_words; ++w) nfree += (uint64_t)__builtin_popcountll(bits[(size_t)w * GFS_SLOT_WORD_STRIDE]);
-        if (cur.ctr.timeouts == 0 && nfree != h.ring_depth) return -1010;
+        if (nfree + cur.ctr.slots_lost != h.ring_depth) return -1010;
     }
     for (uint32_t f = 0; f < h.n_frames; ++f)
         if (meta[f] & GFS_REF_MASK) return -1001;   // a frame is still pinned

@@ -71,13 +71,15 @@ typedef struct gpufs_stats {
     uint64_t timeouts;       /* device waits that hit the wait limit (GPUFS_WAIT_TIMEOUT_S) */
     uint64_t host_busy_ns;   /* time the host workers spent serving requests, summed over workers */
     uint64_t vm_errors;      /* FatPointer dereferences whose page could not be mapped */
+    uint64_t slots_lost;     /* ring slots kept out of the ring after a request timed out (the host
+                                may still answer it); the ring is that much smaller until destroy */
 } gpufs_stats;
 
 /* ---- lifecycle: replaces initializer(GPUGlobals_ptr*) (libgpufs/fs_initializer.cu:116)
  * and `delete GPUGlobals` (libgpufs/fs_initializer.cu.h:180-198). */
 int   gpufs_config_default(gpufs_config* cfg);
 int   gpufs_init(const gpufs_config* cfg, gpufs_ctx** out);
-int   gpufs_destroy(gpufs_ctx* ctx);
+int   gpufs_destroy(gpufs_ctx* ctx);   /* -EBUSY while a server (gpufs_serve_start) runs on ctx */
 /* The stream kernels that call the device API are launched on
  * (GPUGlobals::streamMgr->kernelStream, libgpufs/fs_initializer.cu.h:56-75). */
 void* gpufs_stream(gpufs_ctx* ctx);
@@ -161,7 +163,9 @@ int gpufs_graph_destroy(void* graph_exec);
  * admitted (FIFO) when a request slot is free and its pages fit the frame
  * budget; results land in counts_out[i*n_words ...], latency_s_out[i] is
  * arrival -> completion seen by the host.  Page cache and file table stay
- * resident across requests and runs.  gpufs_serve_stop ends the workgroups. */
+ * resident across requests and runs; the workgroups wait for requests however
+ * long the server idles, until gpufs_serve_stop ends them.  gpufs_serve_run
+ * returns -EIO if the server kernel has ended (it cannot serve any more). */
 typedef struct gpufs_server gpufs_server;
 typedef struct gpufs_serve_config {
     uint32_t workgroups;    /* persistent server workgroups */
@@ -247,6 +251,18 @@ int gpufs_launch_read(gpufs_ctx* ctx, const char* path_dev, int32_t open_flags,
 int gpufs_launch_write(gpufs_ctx* ctx, const char* path_dev, int32_t open_flags,
                        uint64_t offset, uint64_t length, const void* src_dev,
                        int32_t do_fsync, int64_t* result_dev, uint32_t threads, void* stream);
+/* A script of device API calls made by one workgroup on one file, for the API
+ * tests: record k is ops_dev[4k..4k+3] = {op, a, b, c} and its result goes to
+ * results_dev[k].  op: 1 gopen(flags=a), 2 gread(off=a, len=b, buf_dev+c),
+ * 3 gwrite(same), 4 gfsync, 5 gftruncate(size=a), 6 gclose, 7 fstat,
+ * 8 single_thread_open(flags=a), 9 single_thread_ftruncate(size=a),
+ * 10 single_thread_fsync, 11 single_thread_close, 12 gmmap of [a, a+b) filled
+ * with byte c, gmsync, gmunmap (result: gmsync's), 13 gunlink.  Calls 8-11 and
+ * 13 are made by thread 0 alone (reference device API,
+ * libgpufs/fs_calls.cu.h:34-65). */
+int gpufs_launch_api_script(gpufs_ctx* ctx, const char* path_dev, const int64_t* ops_dev,
+                            uint32_t n_ops, void* buf_dev, int64_t* results_dev, uint32_t threads,
+                            void* stream);
 
 #ifdef __cplusplus
 }

@@ -4,9 +4,13 @@
 // read the result) against the real host service (host_service.h).  Batched
 // clients post up to 64 reads sharing one completion word, take each page as
 // its bit appears and return its slot at once, keeping the word's slot to the
-// end, as a workgroup's page fill does.  Checks every result, the data in
-// staging, slot conservation, and runs under ThreadSanitizer.
+// end, as a workgroup's page fill does.  Also models a wait that timed out:
+// the client keeps (quarantines) the slots of its unanswered requests, the
+// host answers them late, and fresh requests through the other slots stay
+// exact.  Checks every result, the data in staging, slot conservation, and
+// runs under ThreadSanitizer.
 #include <atomic>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -244,6 +248,67 @@ int main(int argc, char** argv) {
         for (auto& t : bc) t.join();
         bsvc.stop();
         CHECK(big.free_slots.size() == big_depth, "big slots conserved");
+    }
+    // Late completions: a client publishes a batch of three reads while no
+    // host service runs, its bounded wait ends, and it keeps the three slots
+    // (the device quarantines a slot whose request timed out: the host still
+    // holds its ticket).  When the service starts it answers those requests
+    // late, while fresh clients read through the other slots: every fresh read
+    // is exact, and the late answers land only in the quarantined slots'
+    // staging and completion word.
+    {
+        Ring lr(8, log_page);
+        const int dfd = open(path.c_str(), O_RDONLY);
+        CHECK(dfd >= 0, "open for late reads");
+        std::vector<int> q = {lr.claim(), lr.claim(), lr.claim()};
+        const uint32_t cqi = (uint32_t)q[0];
+        __atomic_store_n(&lr.cq[cqi], 0ull, __ATOMIC_RELAXED);
+        const uint64_t loff[3] = {4096, 40960, 8192 + 7};
+        for (int i = 0; i < 3; ++i) {
+            gfs_rpc_slot* r = &lr.slots[q[i]];
+            r->op = GFS_OP_READ; r->host_fd = dfd; r->flags = 0; r->cq = cqi | ((uint32_t)i << 16);
+            r->offset = loff[i]; r->length = 4096; r->result = INT64_MIN;
+            lr.publish(q[i]);
+        }
+        const auto t0 = std::chrono::steady_clock::now();
+        while ((__atomic_load_n(&lr.cq[cqi], __ATOMIC_ACQUIRE) & 7ull) != 7ull &&
+               std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(20))
+            std::this_thread::yield();
+        CHECK(__atomic_load_n(&lr.cq[cqi], __ATOMIC_ACQUIRE) == 0, "no service yet: nothing answered");
+        // timed out: q's slots are not released
+        gfs_host::Service lsvc;
+        lsvc.start(lr.rv, 2);
+        std::vector<std::thread> fc;
+        std::atomic<int> fresh{0};
+        for (int c = 0; c < 3; ++c) {
+            fc.emplace_back([&, c]() {
+                std::mt19937_64 r(700 + c);
+                for (int k = 0; k < 200; ++k) {
+                    int s2;
+                    while ((s2 = lr.claim()) < 0) std::this_thread::yield();
+                    CHECK(s2 != q[0] && s2 != q[1] && s2 != q[2], "a quarantined slot was handed out");
+                    const uint64_t off = r() % data.size();
+                    const uint64_t len = 1 + r() % 4096;
+                    const int64_t got = lr.call(s2, GFS_OP_READ, dfd, 0, off, len);
+                    const uint64_t exp = std::min<uint64_t>(len, data.size() - off);
+                    CHECK(got == (int64_t)exp, "fresh read beside late answers");
+                    if (got > 0) CHECK(memcmp(&lr.staging[(size_t)s2 << log_page], &data[off], (size_t)got) == 0,
+                                       "fresh bytes beside late answers");
+                    lr.release(s2);
+                    fresh++;
+                }
+            });
+        }
+        for (auto& t : fc) t.join();
+        while ((__atomic_load_n(&lr.cq[cqi], __ATOMIC_ACQUIRE) & 7ull) != 7ull) std::this_thread::yield();
+        lsvc.stop();
+        CHECK(fresh == 600, "fresh reads");
+        CHECK(lr.free_slots.size() == 5, "5 slots in the ring, 3 quarantined");
+        for (int i = 0; i < 3; ++i) {
+            CHECK(lr.slots[q[i]].result == 4096, "late answer %d", i);
+            CHECK(memcmp(&lr.staging[(size_t)q[i] << log_page], &data[loff[i]], 4096) == 0, "late bytes %d", i);
+        }
+        close(dfd);
     }
     unlink(path.c_str()); unlink(wpath.c_str());
     if (failures) { fprintf(stderr, "%d failures\n", failures); return 1; }

@@ -62,3 +62,33 @@ def test_serve_alone_equals_mix_equals_cpu(tmp_path, orc):
             assert s["rpc_opens"] == distinct, (s, distinct)
         assert s["rpc_opens"] == s["rpc_closes"], s
         assert ctx.check_invariants() == 0
+
+
+def test_serve_survives_idle(tmp_path, orc, monkeypatch):
+    """A server left idle for longer than the device wait limit (1 s here, so
+    3 s of idling is three times it) still serves: its workgroups never give
+    up a request ticket they have taken.  Both runs are bit-exact."""
+    import time
+    say("test_serve_survives_idle")
+    monkeypatch.setenv("GPUFS_WAIT_TIMEOUT_S", "1")
+    n = 24
+    voc = dg.vocab(2000, seed=81)
+    q = np.ascontiguousarray(voc[dg.query(2000, 256, seed=82)])
+    sizes = dg.lognormal_sizes(n, seed=83, median=8192.0, sigma=1.0, lo=1, hi=1 << 18)
+    paths = [str(tmp_path / f"i{i:03d}.txt") for i in range(n)]
+    dg.text_files(paths, sizes, 84, voc, threads=4)
+    cpu, cst = orc.scan_files(paths, q, 65536, threads=4)
+    assert (cst == 0).all()
+    with make_ctx(page_size=64 << 10, cache_bytes=32 << 20, ring_depth=64) as ctx:
+        srv = Server(ctx, paths, sizes, q, workgroups=4, slots=8)
+        try:
+            ids = np.arange(n, dtype=np.uint32)
+            c, st, _, _ = srv.run(ids[:12], np.zeros(12))
+            assert (st == 0).all() and (c == cpu[:12]).all()
+            time.sleep(3.0)
+            c, st, _, _ = srv.run(ids, np.linspace(0.0, 0.01, n))
+            assert (st == 0).all(), st
+            assert (c == cpu).all()
+        finally:
+            srv.stop()
+        assert ctx.check_invariants() == 0
