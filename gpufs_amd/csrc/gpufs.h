@@ -90,6 +90,7 @@ struct wg_ctl {
     int64_t gr_ret;
     uint64_t gr_fs, gr_root;
     uint32_t gr_ra;
+    uint32_t pinf[64];       // pin_pages: frame of each pinned page
 };
 
 // Per-workgroup read history for sequential read-ahead.  It is advisory: it
@@ -1427,6 +1428,92 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
     GFS_TRACE(6);
     return err ? (size_t)(int64_t)err : (size_t)n;
 }
+
+namespace gfs {
+// Pins pages [pb, pb+cnt) of open file `fd` (cnt <= 64, all below the end of
+// the file) for the caller, reading the missing ones from the host as gread
+// does, without copying anything out (workgroup-collective).  On success
+// gfs_ctl.pinf[i] holds page pb+i's frame, VALID and pinned, until
+// unpin_pages; the caller may read the frames' bytes directly.  Returns 0 or a
+// negative code (nothing stays pinned then).
+__device__ inline int pin_pages(gfs_dev* g, int fd, uint64_t pb, uint32_t cnt) {
+    const uint64_t all = cnt >= 64 ? ~0ull : ((1ull << cnt) - 1);
+    uint64_t pending = all, pinned = 0;
+    __syncthreads();
+    if (tid() == 0) {
+        gfs_memo.fd = -1;   // the batch arrays are reused below
+        gfs_ctl.gr_fs = g->files[fd].size;
+        gfs_ctl.gr_root = ld_acq(&g->files[fd].root);
+    }
+    __syncthreads();
+    const uint64_t fsize = uni(gfs_ctl.gr_fs);
+    const uint64_t root_hint = uni(gfs_ctl.gr_root);
+    const uint64_t t0 = now_ticks();
+    uint32_t idle = 0;
+    int err = 0;
+    while (pending && !err) {
+        __syncthreads();
+        if (wave() == 0) {
+            if (lane() == 0) gfs_ctl.err = 0;
+            resolve_batch(g, fd, pb, cnt, pending, 0ull, false, root_hint);
+        }
+        __syncthreads();
+        if (uni(gfs_ctl.err)) { err = uni(gfs_ctl.err); break; }
+        fill_from_host(g, cnt, pb, fsize);
+        if (wave() == 0) {
+            const uint32_t ln = lane();
+            if (gfs_ctl.err) fail_own(g, fd, pb, cnt);
+            const uint32_t k = ln < cnt ? gfs_ctl.kind[ln] : GFS_K_DONE;
+            const uint32_t f = gfs_ctl.frame[ln];
+            if (k == GFS_K_OWN) {
+                // published VALID, the allocation's pin is kept
+                fadd(&g->meta[f], (GFS_F_VALID - GFS_F_FILLING) * (1ull << GFS_ST_SHIFT));
+                if (gfs_ctl.slot[ln] != 0xFFFFFFFFu && !((gfs_ctl.rel >> ln) & 1ull)) rpc_release(g, (int)gfs_ctl.slot[ln]);
+            }
+            if (k == GFS_K_HIT || k == GFS_K_OWN) gfs_ctl.pinf[ln] = f;
+            const uint64_t got = __ballot(k == GFS_K_HIT || k == GFS_K_OWN);
+            const uint64_t waitm = __ballot(k == GFS_K_WAIT);
+            if (ln == 0) {
+                gfs_ctl.a = waitm;
+                gfs_ctl.b = got;
+            }
+        }
+        __syncthreads();
+        pinned |= uni(gfs_ctl.b);
+        if (uni(gfs_ctl.err)) { err = uni(gfs_ctl.err); break; }
+        const uint64_t nxt = uni(gfs_ctl.a);
+        if (nxt == pending) {
+            if (++idle > 4) backoff();
+            if ((idle & 63) == 0 && wg_timed_out(g, t0)) { err = -ETIMEDOUT; break; }
+        } else {
+            idle = 0;
+        }
+        pending = nxt;
+    }
+    if (err) {
+        if (wave() == 0) {
+            const uint32_t ln = lane();
+            if ((pinned >> ln) & 1ull) fadd(&g->meta[gfs_ctl.pinf[ln]], ~0ull);
+        }
+        __syncthreads();
+        return err;
+    }
+    fence_acq_agent();   // the frames' bytes are read after their pins
+    __syncthreads();
+    return 0;
+}
+// Drops the pins of pin_pages (workgroup-collective); the caller's reads of
+// the frames have all returned.
+__device__ inline void unpin_pages(gfs_dev* g, uint32_t cnt) {
+    __syncthreads();
+    if (wave() == 0 && lane() < cnt) {
+        const uint32_t f = gfs_ctl.pinf[lane()];
+        g->fref[f] = 1;
+        fadd_rlx(&g->meta[f], ~0ull);
+    }
+    __syncthreads();
+}
+} // namespace gfs
 
 // Writes [offset, offset+size) of `buffer` through the page cache
 // (libgpufs/fs_calls.cu:739-802).  Pages are fetched from the host first

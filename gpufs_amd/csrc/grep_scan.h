@@ -314,6 +314,318 @@ __device__ inline int grep_one_file(grep_lds& S, const char* name, uint32_t nw, 
     return st;
 }
 
+// ---------------------------------------------------------------------------
+// One file scanned by several workgroups (parts).  The reference's chunk loop
+// (grep_text_gpu.cu:298-318) moves its window by chunk + ov, where ov is the
+// run of word bytes (at most 32) at the chunk's end, so every window but the
+// last ends right before a non-word byte whenever ov < 32: the windows then
+// split no token, and the count of a non-last window is the count of the
+// tokens that start in it.  So, unless some ov is 32,
+//     counts(file) = tokens of the whole file
+//                    + tokens of [c_L + i*chunk, size) for every i >= 1 with
+//                      c_L + i*chunk < size,
+// where c_L is the start of the reference's last window (that window is
+// scanned to the end of the file and the cursor then moves by chunk only, so
+// each later window re-scans the file's last bytes, from a word start).
+// Each part counts the tokens that start in its byte range, with window
+// boundaries of its own that also fall before non-word bytes; the last part
+// to finish computes the reference's window chain from the cached pages,
+// adds the re-scanned tail, and when a boundary (the parts' or the
+// reference's) meets a run of 32 or more word bytes, rescans the whole file
+// with the reference's own loop.  The result equals grep_file_windows.
+
+// Run of word bytes at LDS offset x (at most 64), bytes from `avail` on
+// counting as non-word; every wave computes it from the same bytes.
+__device__ __forceinline__ uint32_t lds_word_run(const uint8_t* corpus, uint32_t x, uint32_t avail) {
+    const uint32_t ln = gfs::lane();
+    const bool stop = ln >= avail || !g_alpha(corpus[x + (ln < avail ? ln : 0)]);
+    const uint64_t m = __ballot(stop);
+    return m ? (uint32_t)__builtin_ctzll(m) : 64u;
+}
+
+// The tokens that start in [b, e) of the file, where b is the first non-word
+// byte at or after p_begin (p_begin itself when it is 0) and e likewise for
+// p_end (the file's end when p_end == size), counted into S.counts.  Windows of
+// `chunk` bytes end at the first non-word byte after the chunk (the
+// reference's look-ahead rule).  *longrun is set when a run of 32 or more word
+// bytes meets one of these boundaries (a token might be split there).
+// Returns 0 or a negative code (workgroup-uniform).
+__device__ inline int grep_part_windows(grep_lds& S, int fd, uint32_t chunk, uint64_t size, uint64_t p_begin,
+                                        uint64_t p_end, uint32_t* longrun) {
+    const uint32_t t = gfs::tid();
+    if (t == 0) S.status = 0;
+    __syncthreads();
+    uint64_t cursor = p_begin;
+    bool first = p_begin > 0;
+    uint32_t lr = 0;
+    for (;;) {
+        const bool final = p_end - cursor <= chunk;
+        const uint64_t want = final ? p_end + 64 : cursor + chunk + 32;
+        const uint64_t rend = want < size ? want : size;
+        const uint32_t mis = (uint32_t)(cursor & 15);
+        const uint32_t len = (uint32_t)(rend - cursor) + mis;
+        size_t got;
+        [[clang::always_inline]] got = gread<gfs::SPACE_LDS>(fd, cursor - mis, len, S.corpus);
+        if (got != len) {
+            if (t == 0) S.status = (int64_t)got < 0 ? (int)(int64_t)got : -EIO;
+            break;
+        }
+        uint32_t skip = mis;
+        if (first) {
+            // the token that runs into p_begin belongs to the previous part
+            const uint32_t r = lds_word_run(S.corpus, mis, len - mis < 64 ? len - mis : 64);
+            if (r >= 32) lr = 1;
+            else skip = mis + r;
+            first = false;
+        }
+        uint32_t dsz, ov = 0;
+        if (final) {
+            uint64_t e = size;
+            if (p_end < size) {
+                const uint32_t xo = mis + (uint32_t)(p_end - cursor);
+                const uint32_t r = lds_word_run(S.corpus, xo, len - xo < 64 ? len - xo : 64);
+                if (r >= 32) lr = 1;
+                else e = p_end + r;
+            }
+            dsz = mis + (uint32_t)(e - cursor);
+        } else {
+            const uint32_t xo = mis + chunk;
+            ov = lds_word_run(S.corpus, xo, len - xo < 32 ? len - xo : 32);
+            if (ov >= 32) lr = 1;
+            dsz = mis + chunk + ov;
+        }
+        [[clang::always_inline]] grep_scan_window(S, dsz, skip);
+        if (final) break;
+        cursor += (uint64_t)chunk + ov;
+        // a window whose look-ahead run reached p_end ended where this part
+        // ends (the run at p_end is part of the same run, shorter than 32)
+        if (cursor >= p_end) break;
+    }
+    __syncthreads();
+    const int st = gfs::uni(S.status);
+    __syncthreads();
+    *longrun = lr;
+    return st;
+}
+
+#define GREP_CHAIN_UNITS 4096   // 16-byte units whose word-byte masks one chain round keeps in LDS
+
+// Word-byte mask of 16 bytes, bit i = byte i.
+__device__ __forceinline__ uint32_t alpha16(gfs::v4u v) {
+    return alpha4(v.x) | (alpha4(v.y) << 4) | (alpha4(v.z) << 8) | (alpha4(v.w) << 12);
+}
+// Units of chain step j of a round: step j's look-ahead starts at most 31*j
+// bytes after the 16-byte unit holding the round's j-th chunk end, and needs
+// 32 bytes plus the unit offset.
+__device__ __forceinline__ uint32_t chain_units(uint32_t j) { return ((31u * j + 15u) >> 4) + 3u; }
+
+// The reference's window chain over a file of `size` bytes
+// (grep_text_gpu.cu:298-318): c_0 = 0, c_{k+1} = c_k + chunk + ov_k, while at
+// least chunk + 32 bytes remain after c_k.  *c_last = the last window's start;
+// *cut = 1 when some ov_k is 32 (that window cuts a token).  Read from the
+// cached pages, pinned a batch at a time: a round loads, for each of its next
+// steps, every 16-byte unit the step's look-ahead can fall in (all loads
+// issued at once), then thread 0 walks the steps over their word-byte masks
+// in LDS.  Workgroup-collective; returns 0 or a negative code.
+__device__ inline int grep_chain_last(grep_lds& S, int fd, uint64_t size, uint32_t chunk, uint64_t* c_last,
+                                      uint32_t* cut) {
+    using namespace gfs;
+    gfs_dev* g = gfs_g;
+    const uint32_t L = g->log_page;
+    const uint64_t PS = g->page_size;
+    const uint64_t npages = (size + PS - 1) >> L;
+    uint32_t maxpin = g->n_frames / 8;
+    maxpin = maxpin > 64 ? 64 : (maxpin < 2 ? 2 : maxpin);
+    const uint32_t T = nthreads(), t = tid();
+    uint16_t* m = (uint16_t*)S.corpus;
+    uint64_t c = 0;
+    uint32_t sp = 0;
+    while (size - c >= (uint64_t)chunk + 32 && !sp) {
+        const uint64_t P = (c + chunk) >> L;
+        const uint32_t cnt = (uint32_t)(npages - P < maxpin ? npages - P : maxpin);
+        const int r = pin_pages(g, fd, P, cnt);
+        if (r) return r;
+        const uint64_t hi0 = (P + cnt) << L;
+        const uint64_t hi = hi0 < size ? hi0 : size;
+        // this round's steps: as many as fit the unit budget and the pinned pages
+        uint32_t R = 0, U = 0;
+        while (R < 512) {
+            const uint32_t n = chain_units(R);
+            const uint64_t b = (c + (uint64_t)(R + 1) * chunk) & ~15ull;
+            if (U + n > GREP_CHAIN_UNITS || b >= hi) break;
+            U += n;
+            ++R;
+        }
+        for (uint32_t u = t; u < U; u += T) {
+            uint32_t j = 0, s0 = 0;
+            while (s0 + chain_units(j) <= u) s0 += chain_units(j++);
+            const uint64_t addr = ((c + (uint64_t)(j + 1) * chunk) & ~15ull) + 16ull * (u - s0);
+            uint32_t mk = 0;
+            if (addr < hi) {
+                const uint32_t f = gfs_ctl.pinf[(addr >> L) - P];
+                const v4u v = *(const gv4u*)(frame_ptr(g, f) + (addr & (PS - 1)));
+                mk = alpha16(v);
+                if (addr + 16 > size) mk &= (1u << (uint32_t)(size - addr)) - 1;
+            }
+            m[u] = (uint16_t)mk;
+        }
+        __syncthreads();
+        if (t == 0) {
+            const uint64_t c0 = c;
+            uint32_t s0 = 0;
+            for (uint32_t j = 0; j < R; s0 += chain_units(j++)) {
+                if (size - c < (uint64_t)chunk + 32) break;
+                const uint64_t q = c + chunk;
+                if (q + 32 > hi) break;
+                const uint64_t off = q - ((c0 + (uint64_t)(j + 1) * chunk) & ~15ull);
+                const uint32_t idx = s0 + (uint32_t)(off >> 4);
+                const uint64_t w = (uint64_t)m[idx] | ((uint64_t)m[idx + 1] << 16) | ((uint64_t)m[idx + 2] << 32);
+                const uint32_t bits = (uint32_t)(w >> (off & 15));
+                const uint32_t run = bits == 0xFFFFFFFFu ? 32u : (uint32_t)__builtin_ctz(~bits);
+                if (run == 32) {
+                    sp = 1;
+                    break;
+                }
+                c = q + run;
+            }
+            S.fsize = c;
+            S.status = (int)sp;
+        }
+        __syncthreads();
+        c = uni((uint64_t)S.fsize);
+        sp = (uint32_t)uni(S.status);
+        unpin_pages(g, cnt);
+    }
+    *c_last = c;
+    *cut = sp;
+    return 0;
+}
+
+// The reference's windows after its last full one: the window at c_L is
+// scanned to the end of the file and the cursor moves by `chunk` only, so
+// every x = c_L + i*chunk below the end starts one more window [x, size), each
+// scanned from a word start (several when chunk < 32).  [from, size) with
+// from = c_L + chunk is shorter than 32 bytes; its windows' tokens are added
+// into S.counts.  Returns 0 or a negative code.
+__device__ inline int grep_tail(grep_lds& S, int fd, uint64_t from, uint64_t size, uint32_t chunk) {
+    const uint32_t mis = (uint32_t)(from & 15);
+    const uint32_t len = (uint32_t)(size - from) + mis;
+    size_t got;
+    [[clang::always_inline]] got = gread<gfs::SPACE_LDS>(fd, from - mis, len, S.corpus);
+    if (got != len) return (int64_t)got < 0 ? (int)(int64_t)got : -EIO;
+    for (uint64_t x = from; x < size; x += chunk)
+        [[clang::always_inline]] grep_scan_window(S, len, mis + (uint32_t)(x - from));
+    __syncthreads();
+    return 0;
+}
+
+// Per-file state of a split scan, zeroed before the launch.
+struct grep_split_state {
+    uint32_t arrived;   // parts done
+    uint32_t longrun;   // some part met a run of 32 or more word bytes at a boundary
+    int32_t err;        // first error of a part
+    uint32_t pad;
+};
+
+// Completes a split file (the workgroup whose part finished last; cold path,
+// kept out of the part loop's registers): the reference's window chain, then
+// either the re-scanned tail added into the row, or, when a boundary met a
+// run of 32 or more word bytes, the reference's own loop over the whole file
+// with the row overwritten.  Returns 0 or a negative code.
+__device__ __noinline__ int grep_split_finish(grep_lds& S, int fd, uint32_t nw, uint32_t chunk, uint64_t size,
+                                              uint32_t* row, uint32_t redo) {
+    const uint32_t T = gfs::nthreads(), t = gfs::tid();
+    int st = 0;
+    uint64_t cl = 0;
+    if (!redo) {
+        uint32_t cut = 0;
+        st = grep_chain_last(S, fd, size, chunk, &cl, &cut);
+        redo = cut;
+    }
+    if (st) return st;
+    if (redo) {
+        for (uint32_t q = t; q < nw; q += T) S.counts[q] = 0;
+        __syncthreads();
+        st = grep_file_windows(S, fd, chunk, [] {});
+        for (uint32_t q = t; q < nw; q += T) {
+            const uint32_t k = S.canon[q];
+            row[q] = k == 0xFFFF ? 0u : S.counts[k];
+        }
+    } else if (size - cl > chunk) {
+        for (uint32_t q = t; q < nw; q += T) S.counts[q] = 0;
+        __syncthreads();
+        st = grep_tail(S, fd, cl + chunk, size, chunk);
+        for (uint32_t q = t; q < nw; q += T) {
+            const uint32_t k = S.canon[q];
+            const uint32_t v = k == 0xFFFF ? 0u : S.counts[k];
+            if (v) atomicAdd(&row[q], v);
+        }
+    }
+    __syncthreads();
+    return st;
+}
+
+// Part j of the file `name` (workgroup-collective).  The file is split into
+// K_f = min(parts, size / min_part) parts (at least 1; min_part >= 64, so no
+// part is empty); workgroups j >= K_f scan nothing.  Every one of the `parts`
+// workgroups of the file arrives exactly once, after adding its counts into
+// the caller-zeroed row (a workgroup whose gopen failed records the error);
+// the last to arrive completes the row (grep_split_finish) and writes
+// status[0].
+__device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t nw, uint32_t chunk, uint32_t* row,
+                                       int32_t* status, uint32_t j, uint32_t parts, uint64_t min_part,
+                                       grep_split_state* fs) {
+    const uint32_t T = gfs::nthreads(), t = gfs::tid();
+    __shared__ uint32_t fin_flag;
+    for (uint32_t q = t; q < nw; q += T) S.counts[q] = 0;
+    const int fd = gopen(name, O_GRDONLY);
+    uint64_t size = 0;
+    int st = fd < 0 ? fd : 0;
+    uint32_t lr = 0;
+    if (fd >= 0) {
+        if (t == 0) S.fsize = fstat(fd);
+        __syncthreads();
+        size = gfs::uni((uint64_t)S.fsize);
+        uint64_t kf = size / min_part;
+        kf = kf < 1 ? 1 : (kf > parts ? parts : kf);
+        if (j < kf && size) {
+            const uint64_t pb = j == 0 ? 0 : ((size * j / kf) & ~63ull);
+            const uint64_t pe = j + 1 == kf ? size : ((size * (j + 1) / kf) & ~63ull);
+            st = grep_part_windows(S, fd, chunk, size, pb, pe, &lr);
+            for (uint32_t q = t; q < nw; q += T) {
+                const uint32_t k = S.canon[q];
+                const uint32_t v = k == 0xFFFF ? 0u : S.counts[k];
+                if (v) atomicAdd(&row[q], v);
+            }
+            __threadfence();
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        if (lr) atomicOr(&fs->longrun, 1u);
+        if (st) atomicCAS(&fs->err, 0, st);
+        const uint32_t old = __hip_atomic_fetch_add(&fs->arrived, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        fin_flag = old + 1 == parts ? 1u : 0u;
+    }
+    __syncthreads();
+    if (gfs::uni(fin_flag)) {
+        // every part's counts are in the row: their adds precede their arrival
+        if (t == 0) {
+            S.status = __hip_atomic_load(&fs->err, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            fin_flag = __hip_atomic_load(&fs->longrun, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        st = gfs::uni(S.status);
+        const uint32_t redo = gfs::uni(fin_flag);
+        if (!st && size) st = grep_split_finish(S, fd, nw, chunk, size, row, redo);
+        const int c = fd >= 0 ? gclose(fd) : 0;
+        if (t == 0) *status = st ? st : c;
+        __syncthreads();
+        return;
+    }
+    if (fd >= 0) gclose(fd);
+}
+
 static int grep_args_ok(uint32_t n_files, uint32_t n_words, uint32_t chunk_bytes, uint32_t threads) {
     if (n_files == 0 || n_words == 0 || n_words > GREP_MAX_WORDS) return -EINVAL;
     if (chunk_bytes == 0 || chunk_bytes > GREP_MAX_CHUNK) return -EINVAL;

@@ -23,14 +23,16 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+// Workgroup b scans part b % parts of file b / parts (grep_file_split): a
+// launch over fewer files than compute units still fills the device.
 __global__ __launch_bounds__(1024) void k_grep(const char* names, uint32_t stride, const char* dict, uint32_t nw,
-                                               uint32_t chunk, uint32_t* counts_out, int32_t* status) {
+                                               uint32_t chunk, uint32_t* counts_out, int32_t* status, uint32_t parts,
+                                               uint64_t min_part, grep_split_state* fstate) {
     __shared__ __attribute__((aligned(16))) grep_lds S;
-    const uint32_t f = blockIdx.x;
+    const uint32_t f = blockIdx.x / parts, j = blockIdx.x % parts;
     grep_build_dict(S, dict, nw);
-    int st;
-    [[clang::always_inline]] st = grep_one_file(S, names + (size_t)f * stride, nw, chunk, counts_out + (size_t)f * nw);
-    if (gfs::tid() == 0) status[f] = st;
+    [[clang::always_inline]] grep_file_split(S, names + (size_t)f * stride, nw, chunk, counts_out + (size_t)f * nw,
+                                             status + f, j, parts, min_part, fstate + f);
 }
 
 // Persistent form: each workgroup builds the dictionary table once, then pulls
@@ -75,18 +77,66 @@ __global__ void k_zero_u32(unsigned int* p, uint32_t n) {
     if (i < n) p[i] = 0;
 }
 
-extern "C" int gpufs_launch_grep(gpufs_ctx* ctx, const char* names_dev, uint32_t name_stride, uint32_t n_files,
-                                 const char* dict_dev, uint32_t n_words, uint32_t chunk_bytes,
-                                 uint32_t* counts_dev, int32_t* status_dev, uint32_t threads, void* stream) {
+// Zeroes the count rows and the per-file split state before a split scan
+// (a fill kernel, so the launch can also be captured in a graph).
+__global__ void k_grep_clear(uint32_t* counts, uint64_t n_counts, grep_split_state* fs, uint32_t n_files) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_counts) counts[i] = 0;
+    if (i < n_files) fs[i] = grep_split_state{0, 0, 0, 0};
+}
+
+// Parts per file when the caller leaves it to the launcher: enough workgroups
+// to give every compute unit one (each holds a whole compute unit's LDS), at
+// most 16 per file.
+static uint32_t grep_auto_parts(uint32_t n_files) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    uint32_t k = ((uint32_t)cus + n_files - 1) / n_files;
+    return k < 1 ? 1 : (k > 16 ? 16 : k);
+}
+
+static int launch_grep_split(gpufs_ctx* ctx, const char* names_dev, uint32_t name_stride, uint32_t n_files,
+                             const char* dict_dev, uint32_t n_words, uint32_t chunk_bytes, uint32_t* counts_dev,
+                             int32_t* status_dev, uint32_t parts, uint64_t min_part, grep_split_state* fs,
+                             uint32_t threads, hipStream_t s) {
+    const uint64_t n_counts = (uint64_t)n_files * n_words;
+    const uint64_t n_clear = n_counts > n_files ? n_counts : n_files;
+    hipLaunchKernelGGL(k_grep_clear, dim3((uint32_t)((n_clear + 255) / 256)), dim3(256), 0, s, counts_dev, n_counts,
+                       fs, n_files);
+    hipLaunchKernelGGL(k_grep, dim3(n_files * parts), dim3(threads), 0, s, names_dev, name_stride, dict_dev, n_words,
+                       chunk_bytes, counts_dev, status_dev, parts, min_part, fs);
+    return hipGetLastError() == hipSuccess ? 0 : -EIO;
+}
+
+extern "C" int gpufs_launch_grep_split(gpufs_ctx* ctx, const char* names_dev, uint32_t name_stride,
+                                       uint32_t n_files, const char* dict_dev, uint32_t n_words,
+                                       uint32_t chunk_bytes, uint32_t* counts_dev, int32_t* status_dev,
+                                       uint32_t parts_per_file, uint64_t min_part_bytes, uint32_t threads,
+                                       void* stream) {
     if (!ctx || !names_dev || !dict_dev || !counts_dev || !status_dev) return -EINVAL;
     int v = grep_args_ok(n_files, n_words, chunk_bytes, threads);
     if (v) return v;
-    v = gpufs_bind(ctx);
-    if (v) return v;
+    if (parts_per_file == 0) parts_per_file = grep_auto_parts(n_files);
+    if (parts_per_file > 64 || (uint64_t)n_files * parts_per_file > 0x7FFFFFFFull) return -EINVAL;
+    if (min_part_bytes == 0) min_part_bytes = 4ull * chunk_bytes > (256u << 10) ? 4ull * chunk_bytes : (256u << 10);
+    if (min_part_bytes < 64) return -EINVAL;
+    if ((v = gpufs_bind(ctx))) return v;
     hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)gpufs_stream(ctx);
-    hipLaunchKernelGGL(k_grep, dim3(n_files), dim3(threads), 0, s, names_dev, name_stride, dict_dev, n_words,
-                       chunk_bytes, counts_dev, status_dev);
-    return hipGetLastError() == hipSuccess ? 0 : -EIO;
+    grep_split_state* fs = nullptr;
+    if (hipMallocAsync((void**)&fs, sizeof(grep_split_state) * (size_t)n_files, s) != hipSuccess) return -ENOMEM;
+    v = launch_grep_split(ctx, names_dev, name_stride, n_files, dict_dev, n_words, chunk_bytes, counts_dev,
+                          status_dev, parts_per_file, min_part_bytes, fs, threads, s);
+    (void)hipFreeAsync(fs, s);
+    return v;
+}
+
+extern "C" int gpufs_launch_grep(gpufs_ctx* ctx, const char* names_dev, uint32_t name_stride, uint32_t n_files,
+                                 const char* dict_dev, uint32_t n_words, uint32_t chunk_bytes,
+                                 uint32_t* counts_dev, int32_t* status_dev, uint32_t threads, void* stream) {
+    return gpufs_launch_grep_split(ctx, names_dev, name_stride, n_files, dict_dev, n_words, chunk_bytes, counts_dev,
+                                   status_dev, 0, 0, threads, stream);
 }
 
 static void launch_batched(hipStream_t s, const char* names_dev, uint32_t name_stride, uint32_t n_files,

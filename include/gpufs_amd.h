@@ -120,16 +120,30 @@ int gpufs_launch_gread_ranges(gpufs_ctx* ctx, const char* path_dev, uint32_t n_r
                               uint32_t threads, void* stream);
 
 /* grep_text-style scan (samples/grep_text/grep_text_gpu.cu:193-387, scan
- * semantics :79,139-160, stitching :298-318): workgroup f gopens file
- * names_dev + f*name_stride, greads it in chunk_bytes windows with a 32-byte
- * look-ahead and adds whole-word matches of the n_words dictionary words
- * (dict_dev: n_words records of 32 bytes, NUL-padded) into
- * counts_dev[f*n_words + q] (uint32, must be zeroed by the caller).
- * status_dev[f] = 0 or a negative error code. */
+ * semantics :79,139-160, stitching :298-318): file f is names_dev +
+ * f*name_stride; its whole-word match counts of the n_words dictionary words
+ * (dict_dev: n_words records of 32 bytes, NUL-padded) are written to
+ * counts_dev[f*n_words + q] (uint32), exactly as the reference's chunk loop of
+ * chunk_bytes windows with a 32-byte look-ahead counts them.  Each file is
+ * gopen()ed and gread() into LDS by several workgroups (parts, chosen so the
+ * launch fills the device); status_dev[f] = 0 or a negative error code. */
 int gpufs_launch_grep(gpufs_ctx* ctx, const char* names_dev, uint32_t name_stride,
                       uint32_t n_files, const char* dict_dev, uint32_t n_words,
                       uint32_t chunk_bytes, uint32_t* counts_dev, int32_t* status_dev,
                       uint32_t threads, void* stream);
+/* The same scan with the split chosen by the caller: a file of S bytes is
+ * scanned by min(parts_per_file, max(1, S / min_part_bytes)) workgroups, each
+ * counting the tokens that start in its byte range; the workgroup that
+ * finishes last adds what the reference's last-window rule counts again and,
+ * when a boundary meets a run of 32 or more word bytes, recounts the file with
+ * the reference's loop.  parts_per_file 0 = as gpufs_launch_grep (<= 64);
+ * min_part_bytes 0 = max(256 KiB, 4 * chunk_bytes), else >= 64.  No reference
+ * counterpart (the reference gives each file one thread block). */
+int gpufs_launch_grep_split(gpufs_ctx* ctx, const char* names_dev, uint32_t name_stride,
+                            uint32_t n_files, const char* dict_dev, uint32_t n_words,
+                            uint32_t chunk_bytes, uint32_t* counts_dev, int32_t* status_dev,
+                            uint32_t parts_per_file, uint64_t min_part_bytes, uint32_t threads,
+                            void* stream);
 
 /* Batched form of the grep scan over many files: `workgroups` persistent
  * workgroups build the dictionary table once, then pull file indices from the

@@ -115,17 +115,23 @@ def gread_ranges(ctx, path, n_ranges, range_bytes, threads=1024, dst=None):
     return out.cpu().numpy(), dst
 
 
-def grep(ctx, paths, dict_words, chunk, threads=1024):
+def grep(ctx, paths, dict_words, chunk, threads=1024, parts=None, min_part=0):
+    """gpufs_launch_grep, or gpufs_launch_grep_split when `parts` is given.
+    The count rows start as garbage: the call writes them entirely."""
     torch = torch_cuda()
     names = dev_names(paths)
     d = dev_bytes(np.ascontiguousarray(dict_words, dtype=np.uint8).reshape(-1))
     nw = int(np.asarray(dict_words).shape[0])
-    counts = dev_zeros(len(paths) * nw, torch.int32)
-    status = dev_zeros(len(paths), torch.int32)
-    rc = _lib.lib().gpufs_launch_grep(ctx.h, ctypes.c_void_p(names.data_ptr()), STRIDE, len(paths),
-                                      ctypes.c_void_p(d.data_ptr()), nw, chunk,
-                                      ctypes.c_void_p(counts.data_ptr()), ctypes.c_void_p(status.data_ptr()),
-                                      threads, None)
+    counts = torch.full((len(paths) * nw,), 0x5A5A5A5A, dtype=torch.int32, device="cuda")
+    status = torch.full((len(paths),), -12345, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    L = _lib.lib()
+    common = (ctx.h, ctypes.c_void_p(names.data_ptr()), STRIDE, len(paths), ctypes.c_void_p(d.data_ptr()), nw, chunk,
+              ctypes.c_void_p(counts.data_ptr()), ctypes.c_void_p(status.data_ptr()))
+    if parts is None:
+        rc = L.gpufs_launch_grep(*common, threads, None)
+    else:
+        rc = L.gpufs_launch_grep_split(*common, parts, min_part, threads, None)
     _lib.check("gpufs_launch_grep", rc)
     finish(ctx)
     return counts.cpu().numpy().view(np.uint32).reshape(len(paths), nw), status.cpu().numpy()

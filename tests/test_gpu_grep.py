@@ -134,8 +134,73 @@ def test_grep_page_memo_under_eviction(tmp_path, orc):
     dg.text_files(paths, [size] * 8, 63, voc)
     cpu, _ = orc.scan_files(paths, q, 4096)
     with make_ctx(page_size=16 << 10, cache_bytes=2 << 20, ring_depth=128) as ctx:
-        counts, status = grep(ctx, paths, q, 4096)
+        counts, status = grep(ctx, paths, q, 4096, parts=1)
         assert (status == 0).all(), status
+        assert (counts == cpu).all()
+        s = ctx.stats()
+        assert s["evictions"] > 0, s
+        assert ctx.check_invariants() == 0
+
+
+@pytest.mark.parametrize("chunk", [8, 64, 100, 65536])
+def test_grep_split_parts_match_reference(tmp_path, orc, chunk):
+    """Several workgroups per file (2, 5 and 16 parts, parts of at least 64
+    bytes): counts equal the reference loop's, including files whose last
+    window repeats (tail re-scans), empty and one-byte files, and files with
+    runs of 32 or more word bytes that make a part boundary or the reference's
+    look-ahead cut a token (the recount path)."""
+    say(f"test_grep_split_parts_match_reference chunk={chunk}")
+    rng = np.random.default_rng(100 + chunk)
+    letters = np.frombuffer(b"aab_Zz", dtype=np.uint8)
+    seps = np.frombuffer(b" \n,.0\x00\xff", dtype=np.uint8)
+
+    def text(n, long_runs):
+        out, tot = [], 0
+        while tot < n:
+            k = int(rng.integers(1, 45 if long_runs else 9))
+            out.append(letters[rng.integers(0, len(letters), k)])
+            out.append(seps[rng.integers(0, len(seps), int(rng.integers(1, 3)))])
+            tot += len(out[-1]) + len(out[-2])
+        return np.concatenate(out)[:n].copy() if out else np.zeros(0, dtype=np.uint8)
+
+    sizes = [0, 1, 63, 64, 65, 5000, 20000, 200003, 300000] if chunk == 65536 else [0, 1, 64, 700, 5000, 20000, 60001]
+    paths, datas = [], []
+    for i, n in enumerate(sizes):
+        t = text(n, long_runs=(i % 3 == 2))
+        p = str(tmp_path / f"s{i}.bin")
+        t.tofile(p)
+        paths.append(p)
+        datas.append(t)
+    words = [b"a", b"aa", b"aab", b"ab", b"b", b"Zz", b"_", b"a_b", b"zZ", b"ab_a", b"", b"a" * 32]
+    for _ in range(30):
+        words.append(bytes(rng.choice(letters, size=int(rng.integers(1, 7)))))
+    d = np.zeros((len(words), 32), dtype=np.uint8)
+    for i, w in enumerate(words):
+        d[i, :len(w)] = np.frombuffer(w, dtype=np.uint8)
+    ref = [orc.scan_fast(t.tobytes(), d, chunk) for t in datas]
+    with make_ctx(page_size=4096 if chunk < 4096 else 65536, cache_bytes=16 << 20) as ctx:
+        for parts in (2, 5, 16):
+            counts, status = grep(ctx, paths, d, chunk, threads=512, parts=parts, min_part=64)
+            assert (status == 0).all(), (parts, status)
+            for i in range(len(paths)):
+                assert (counts[i] == ref[i]).all(), (parts, i, sizes[i])
+        assert ctx.check_invariants() == 0
+
+
+def test_grep_split_under_eviction(tmp_path, orc):
+    """A split scan whose files do not fit the cache: parts and the finishing
+    workgroup's chain (which pins pages a batch at a time) evict each other's
+    pages; counts still equal the reference's."""
+    say("test_grep_split_under_eviction")
+    voc = dg.vocab(3000, seed=41)
+    q = voc[dg.query(3000, 1024, seed=42)]
+    paths = [str(tmp_path / f"e{i}.txt") for i in range(6)]
+    dg.text_files(paths, [3 << 20] * 6, 43, voc, threads=4)
+    cpu, st = orc.scan_files(paths, q, 65536, threads=4)
+    assert (st == 0).all()
+    with make_ctx(page_size=65536, cache_bytes=12 << 20, ring_depth=128) as ctx:
+        counts, status = grep(ctx, paths, q, 65536, parts=4, min_part=65536)
+        assert (status == 0).all(), (status, ctx.stats())
         assert (counts == cpu).all()
         s = ctx.stats()
         assert s["evictions"] > 0, s
