@@ -1436,7 +1436,7 @@ namespace gfs {
 // gfs_ctl.pinf[i] holds page pb+i's frame, VALID and pinned, until
 // unpin_pages; the caller may read the frames' bytes directly.  Returns 0 or a
 // negative code (nothing stays pinned then).
-__device__ inline int pin_pages(gfs_dev* g, int fd, uint64_t pb, uint32_t cnt) {
+__device__ __noinline__ int pin_pages(gfs_dev* g, int fd, uint64_t pb, uint32_t cnt) {
     const uint64_t all = cnt >= 64 ? ~0ull : ((1ull << cnt) - 1);
     uint64_t pending = all, pinned = 0;
     __syncthreads();

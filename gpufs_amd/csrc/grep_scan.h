@@ -408,7 +408,8 @@ __device__ inline int grep_part_windows(grep_lds& S, int fd, uint32_t chunk, uin
     return st;
 }
 
-#define GREP_CHAIN_UNITS 4096   // 16-byte units whose word-byte masks one chain round keeps in LDS
+#define GREP_CHAIN_UNITS 4608   // 16-byte units whose word-byte masks one chain round keeps in LDS (9 KiB)
+#define GREP_CHAIN_STEPS 64     // chain steps per round: 16 waves x 4
 
 // Word-byte mask of 16 bytes, bit i = byte i.
 __device__ __forceinline__ uint32_t alpha16(gfs::v4u v) {
@@ -416,28 +417,36 @@ __device__ __forceinline__ uint32_t alpha16(gfs::v4u v) {
 }
 // Units of chain step j of a round: step j's look-ahead starts at most 31*j
 // bytes after the 16-byte unit holding the round's j-th chunk end, and needs
-// 32 bytes plus the unit offset.
+// 32 bytes plus the unit offset (at most 126 units for j < 64).
 __device__ __forceinline__ uint32_t chain_units(uint32_t j) { return ((31u * j + 15u) >> 4) + 3u; }
+
+typedef __attribute__((address_space(3))) uint16_t lds_u16;
 
 // The reference's window chain over a file of `size` bytes
 // (grep_text_gpu.cu:298-318): c_0 = 0, c_{k+1} = c_k + chunk + ov_k, while at
 // least chunk + 32 bytes remain after c_k.  *c_last = the last window's start;
 // *cut = 1 when some ov_k is 32 (that window cuts a token).  Read from the
-// cached pages, pinned a batch at a time: a round loads, for each of its next
-// steps, every 16-byte unit the step's look-ahead can fall in (all loads
-// issued at once), then thread 0 walks the steps over their word-byte masks
-// in LDS.  Workgroup-collective; returns 0 or a negative code.
-__device__ inline int grep_chain_last(grep_lds& S, int fd, uint64_t size, uint32_t chunk, uint64_t* c_last,
-                                      uint32_t* cut) {
+// cached pages, pinned a batch at a time.  A round covers up to 4 steps per
+// wave (64 with 16 waves): wave w loads, for steps w + k * waves (k < 4), every 16-byte unit the step's
+// look-ahead can fall in (all its loads issued together) and stores their
+// word-byte masks in LDS; then wave 0 walks the steps over the masks.
+// Workgroup-collective; returns 0 or a negative code.
+__device__ __noinline__ int grep_chain_last(grep_lds& S, int fd, uint64_t size, uint32_t chunk, uint64_t* c_last,
+                                            uint32_t* cut) {
     using namespace gfs;
     gfs_dev* g = gfs_g;
-    const uint32_t L = g->log_page;
-    const uint64_t PS = g->page_size;
+    // read once into scalars: reloaded inside the load loop, they made every
+    // unit load wait for the one before it
+    const uint32_t L = uni(g->log_page);
+    const uint64_t PS = uni((uint64_t)g->page_size);
+    uint8_t* const pool = (uint8_t*)uni((uint64_t)(uintptr_t)g->pool);
     const uint64_t npages = (size + PS - 1) >> L;
-    uint32_t maxpin = g->n_frames / 8;
+    uint32_t maxpin = uni(g->n_frames) / 8;
     maxpin = maxpin > 64 ? 64 : (maxpin < 2 ? 2 : maxpin);
-    const uint32_t T = nthreads(), t = tid();
-    uint16_t* m = (uint16_t*)S.corpus;
+    const uint32_t ln = lane(), w = wave(), nwaves = nthreads() >> 6;
+    // the masks live in the window buffer, addressed as LDS (this code may be
+    // compiled outside the kernel, where S is a generic reference)
+    lds_u16* m = (lds_u16*)(S.corpus);
     uint64_t c = 0;
     uint32_t sp = 0;
     while (size - c >= (uint64_t)chunk + 32 && !sp) {
@@ -449,38 +458,64 @@ __device__ inline int grep_chain_last(grep_lds& S, int fd, uint64_t size, uint32
         const uint64_t hi = hi0 < size ? hi0 : size;
         // this round's steps: as many as fit the unit budget and the pinned pages
         uint32_t R = 0, U = 0;
-        while (R < 512) {
+        const uint32_t rmax = 4 * nwaves < GREP_CHAIN_STEPS ? 4 * nwaves : GREP_CHAIN_STEPS;
+        while (R < rmax) {
             const uint32_t n = chain_units(R);
             const uint64_t b = (c + (uint64_t)(R + 1) * chunk) & ~15ull;
             if (U + n > GREP_CHAIN_UNITS || b >= hi) break;
             U += n;
             ++R;
         }
-        for (uint32_t u = t; u < U; u += T) {
-            uint32_t j = 0, s0 = 0;
-            while (s0 + chain_units(j) <= u) s0 += chain_units(j++);
-            const uint64_t addr = ((c + (uint64_t)(j + 1) * chunk) & ~15ull) + 16ull * (u - s0);
-            uint32_t mk = 0;
-            if (addr < hi) {
-                const uint32_t f = gfs_ctl.pinf[(addr >> L) - P];
-                const v4u v = *(const gv4u*)(frame_ptr(g, f) + (addr & (PS - 1)));
-                mk = alpha16(v);
-                if (addr + 16 > size) mk &= (1u << (uint32_t)(size - addr)) - 1;
+        // wave w: steps w + 16k (k < 4), two units per lane per step
+        v4u val[8];
+        uint64_t ad[8];
+        uint32_t dst[8];
+        uint32_t s0 = 0, jn = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t j = w + (uint32_t)k * nwaves;
+            while (jn < j) s0 += chain_units(jn++);   // s0 = first unit of step j
+            const uint32_t n = chain_units(j);
+            const uint64_t b = (c + (uint64_t)(j + 1) * chunk) & ~15ull;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint32_t l = ln + 64u * (uint32_t)h;
+                const bool on = j < R && l < n;
+                ad[2 * k + h] = b + 16ull * l;
+                dst[2 * k + h] = on ? s0 + l : 0xFFFFFFFFu;
+                val[2 * k + h] = v4u{0, 0, 0, 0};
+                if (on && ad[2 * k + h] < hi) {
+                    const uint64_t a = ad[2 * k + h];
+                    const uint32_t f = gfs_ctl.pinf[(a >> L) - P];
+                    val[2 * k + h] = *(const gv4u*)(pool + ((uint64_t)f << L) + (a & (PS - 1)));
+                }
             }
-            m[u] = (uint16_t)mk;
+        }
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+            if (dst[x] == 0xFFFFFFFFu) continue;
+            uint32_t mk = 0;
+            if (ad[x] < hi) {
+                mk = alpha16(val[x]);
+                if (ad[x] + 16 > size) mk &= (1u << (uint32_t)(size - ad[x])) - 1;
+            }
+            m[dst[x]] = (uint16_t)mk;
         }
         __syncthreads();
-        if (t == 0) {
+        if (w == 0) {
+            // the whole wave walks with the same values (they stay scalar); the
+            // three masks of a step are one broadcast read each
             const uint64_t c0 = c;
-            uint32_t s0 = 0;
-            for (uint32_t j = 0; j < R; s0 += chain_units(j++)) {
+            uint32_t su = 0;
+            for (uint32_t j = 0; j < R; su += chain_units(j++)) {
                 if (size - c < (uint64_t)chunk + 32) break;
                 const uint64_t q = c + chunk;
                 if (q + 32 > hi) break;
                 const uint64_t off = q - ((c0 + (uint64_t)(j + 1) * chunk) & ~15ull);
-                const uint32_t idx = s0 + (uint32_t)(off >> 4);
-                const uint64_t w = (uint64_t)m[idx] | ((uint64_t)m[idx + 1] << 16) | ((uint64_t)m[idx + 2] << 32);
-                const uint32_t bits = (uint32_t)(w >> (off & 15));
+                const uint32_t idx = su + (uint32_t)(off >> 4);
+                const uint32_t m0 = m[idx], m1 = m[idx + 1], m2 = m[idx + 2];
+                const uint64_t wd = (uint64_t)uni(m0) | ((uint64_t)uni(m1) << 16) | ((uint64_t)uni(m2) << 32);
+                const uint32_t bits = (uint32_t)(wd >> (off & 15));
                 const uint32_t run = bits == 0xFFFFFFFFu ? 32u : (uint32_t)__builtin_ctz(~bits);
                 if (run == 32) {
                     sp = 1;
@@ -488,8 +523,10 @@ __device__ inline int grep_chain_last(grep_lds& S, int fd, uint64_t size, uint32
                 }
                 c = q + run;
             }
-            S.fsize = c;
-            S.status = (int)sp;
+            if (ln == 0) {
+                S.fsize = c;
+                S.status = (int)sp;
+            }
         }
         __syncthreads();
         c = uni((uint64_t)S.fsize);
@@ -507,7 +544,7 @@ __device__ inline int grep_chain_last(grep_lds& S, int fd, uint64_t size, uint32
 // scanned from a word start (several when chunk < 32).  [from, size) with
 // from = c_L + chunk is shorter than 32 bytes; its windows' tokens are added
 // into S.counts.  Returns 0 or a negative code.
-__device__ inline int grep_tail(grep_lds& S, int fd, uint64_t from, uint64_t size, uint32_t chunk) {
+__device__ __noinline__ int grep_tail(grep_lds& S, int fd, uint64_t from, uint64_t size, uint32_t chunk) {
     const uint32_t mis = (uint32_t)(from & 15);
     const uint32_t len = (uint32_t)(size - from) + mis;
     size_t got;
@@ -524,33 +561,47 @@ struct grep_split_state {
     uint32_t arrived;   // parts done
     uint32_t longrun;   // some part met a run of 32 or more word bytes at a boundary
     int32_t err;        // first error of a part
-    uint32_t pad;
+    uint32_t cut;       // part 0's chain: some reference window cuts a token
+    uint64_t c_last;    // part 0's chain: start of the reference's last window
 };
 
+// Part boundaries: part 0 also walks the reference's window chain before its
+// scan, so it takes 4/5 of the bytes of each other part.  Multiples of 64, and
+// consecutive boundaries are at least min_part * 4/5 - 63 > 64 bytes apart
+// (min_part >= 256).
+__device__ __forceinline__ uint64_t split_point(uint64_t size, uint64_t j, uint64_t kf) {
+    if (j == 0) return 0;
+    if (j >= kf) return size;
+    return (size * (5 * j - 1) / (5 * kf - 1)) & ~63ull;
+}
+
+// The reference's own loop over the whole file, the row overwritten with its
+// counts (a split whose boundary met a run of 32 or more word bytes).
+__device__ __noinline__ int grep_recount(grep_lds& S, int fd, uint32_t nw, uint32_t chunk, uint32_t* row) {
+    const uint32_t T = gfs::nthreads(), t = gfs::tid();
+    for (uint32_t q = t; q < nw; q += T) S.counts[q] = 0;
+    __syncthreads();
+    const int st = grep_file_windows(S, fd, chunk, [] {});
+    for (uint32_t q = t; q < nw; q += T) {
+        const uint32_t k = S.canon[q];
+        row[q] = k == 0xFFFF ? 0u : S.counts[k];
+    }
+    __syncthreads();
+    return st;
+}
+
 // Completes a split file (the workgroup whose part finished last; cold path,
-// kept out of the part loop's registers): the reference's window chain, then
-// either the re-scanned tail added into the row, or, when a boundary met a
-// run of 32 or more word bytes, the reference's own loop over the whole file
-// with the row overwritten.  Returns 0 or a negative code.
+// kept out of the part loop's registers): given the start c_last of the
+// reference's last window, either the re-scanned tail added into the row, or,
+// when a boundary met a run of 32 or more word bytes (redo), the reference's
+// own loop over the whole file with the row overwritten.  Returns 0 or a
+// negative code.
 __device__ __noinline__ int grep_split_finish(grep_lds& S, int fd, uint32_t nw, uint32_t chunk, uint64_t size,
-                                              uint32_t* row, uint32_t redo) {
+                                              uint32_t* row, uint32_t redo, uint64_t cl) {
     const uint32_t T = gfs::nthreads(), t = gfs::tid();
     int st = 0;
-    uint64_t cl = 0;
-    if (!redo) {
-        uint32_t cut = 0;
-        st = grep_chain_last(S, fd, size, chunk, &cl, &cut);
-        redo = cut;
-    }
-    if (st) return st;
     if (redo) {
-        for (uint32_t q = t; q < nw; q += T) S.counts[q] = 0;
-        __syncthreads();
-        st = grep_file_windows(S, fd, chunk, [] {});
-        for (uint32_t q = t; q < nw; q += T) {
-            const uint32_t k = S.canon[q];
-            row[q] = k == 0xFFFF ? 0u : S.counts[k];
-        }
+        st = grep_recount(S, fd, nw, chunk, row);
     } else if (size - cl > chunk) {
         for (uint32_t q = t; q < nw; q += T) S.counts[q] = 0;
         __syncthreads();
@@ -566,12 +617,13 @@ __device__ __noinline__ int grep_split_finish(grep_lds& S, int fd, uint32_t nw, 
 }
 
 // Part j of the file `name` (workgroup-collective).  The file is split into
-// K_f = min(parts, size / min_part) parts (at least 1; min_part >= 64, so no
-// part is empty); workgroups j >= K_f scan nothing.  Every one of the `parts`
-// workgroups of the file arrives exactly once, after adding its counts into
-// the caller-zeroed row (a workgroup whose gopen failed records the error);
-// the last to arrive completes the row (grep_split_finish) and writes
-// status[0].
+// K_f = min(parts, size / min_part) parts (at least 1; min_part >= 256, so no
+// part is empty); workgroups j >= K_f scan nothing.  Workgroup 0 first walks
+// the reference's window chain (grep_chain_last), overlapped with the other
+// parts' scans.  Every one of the `parts` workgroups of the file arrives
+// exactly once, after adding its counts into the caller-zeroed row (a
+// workgroup whose gopen failed records the error); the last to arrive
+// completes the row (grep_split_finish) and writes status[0].
 __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t nw, uint32_t chunk, uint32_t* row,
                                        int32_t* status, uint32_t j, uint32_t parts, uint64_t min_part,
                                        grep_split_state* fs) {
@@ -588,9 +640,18 @@ __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t n
         size = gfs::uni((uint64_t)S.fsize);
         uint64_t kf = size / min_part;
         kf = kf < 1 ? 1 : (kf > parts ? parts : kf);
-        if (j < kf && size) {
-            const uint64_t pb = j == 0 ? 0 : ((size * j / kf) & ~63ull);
-            const uint64_t pe = j + 1 == kf ? size : ((size * (j + 1) / kf) & ~63ull);
+        if (j == 0 && size) {
+            // the reference's window chain, while the other parts scan
+            uint64_t cl = 0;
+            uint32_t cut = 0;
+            st = grep_chain_last(S, fd, size, chunk, &cl, &cut);
+            if (t == 0) {
+                fs->c_last = cl;
+                fs->cut = cut;
+            }
+        }
+        if (j < kf && size && !st) {
+            const uint64_t pb = split_point(size, j, kf), pe = split_point(size, j + 1, kf);
             st = grep_part_windows(S, fd, chunk, size, pb, pe, &lr);
             for (uint32_t q = t; q < nw; q += T) {
                 const uint32_t k = S.canon[q];
@@ -612,12 +673,14 @@ __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t n
         // every part's counts are in the row: their adds precede their arrival
         if (t == 0) {
             S.status = __hip_atomic_load(&fs->err, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-            fin_flag = __hip_atomic_load(&fs->longrun, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            fin_flag = __hip_atomic_load(&fs->longrun, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) | fs->cut;
+            S.fsize = fs->c_last;
         }
         __syncthreads();
         st = gfs::uni(S.status);
         const uint32_t redo = gfs::uni(fin_flag);
-        if (!st && size) st = grep_split_finish(S, fd, nw, chunk, size, row, redo);
+        const uint64_t cl = gfs::uni((uint64_t)S.fsize);
+        if (!st && size) st = grep_split_finish(S, fd, nw, chunk, size, row, redo, cl);
         const int c = fd >= 0 ? gclose(fd) : 0;
         if (t == 0) *status = st ? st : c;
         __syncthreads();

@@ -82,7 +82,7 @@ __global__ void k_zero_u32(unsigned int* p, uint32_t n) {
 __global__ void k_grep_clear(uint32_t* counts, uint64_t n_counts, grep_split_state* fs, uint32_t n_files) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_counts) counts[i] = 0;
-    if (i < n_files) fs[i] = grep_split_state{0, 0, 0, 0};
+    if (i < n_files) fs[i] = grep_split_state{0, 0, 0, 0, 0};
 }
 
 // Parts per file when the caller leaves it to the launcher: enough workgroups
@@ -121,7 +121,7 @@ extern "C" int gpufs_launch_grep_split(gpufs_ctx* ctx, const char* names_dev, ui
     if (parts_per_file == 0) parts_per_file = grep_auto_parts(n_files);
     if (parts_per_file > 64 || (uint64_t)n_files * parts_per_file > 0x7FFFFFFFull) return -EINVAL;
     if (min_part_bytes == 0) min_part_bytes = 4ull * chunk_bytes > (256u << 10) ? 4ull * chunk_bytes : (256u << 10);
-    if (min_part_bytes < 64) return -EINVAL;
+    if (min_part_bytes < 256) return -EINVAL;
     if ((v = gpufs_bind(ctx))) return v;
     hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)gpufs_stream(ctx);
     grep_split_state* fs = nullptr;

@@ -137,7 +137,7 @@ int gpufs_launch_grep(gpufs_ctx* ctx, const char* names_dev, uint32_t name_strid
  * finishes last adds what the reference's last-window rule counts again and,
  * when a boundary meets a run of 32 or more word bytes, recounts the file with
  * the reference's loop.  parts_per_file 0 = as gpufs_launch_grep (<= 64);
- * min_part_bytes 0 = max(256 KiB, 4 * chunk_bytes), else >= 64.  No reference
+ * min_part_bytes 0 = max(256 KiB, 4 * chunk_bytes), else >= 256.  No reference
  * counterpart (the reference gives each file one thread block). */
 int gpufs_launch_grep_split(gpufs_ctx* ctx, const char* names_dev, uint32_t name_stride,
                             uint32_t n_files, const char* dict_dev, uint32_t n_words,

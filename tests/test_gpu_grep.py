@@ -144,7 +144,7 @@ def test_grep_page_memo_under_eviction(tmp_path, orc):
 
 @pytest.mark.parametrize("chunk", [8, 64, 100, 65536])
 def test_grep_split_parts_match_reference(tmp_path, orc, chunk):
-    """Several workgroups per file (2, 5 and 16 parts, parts of at least 64
+    """Several workgroups per file (2, 5 and 16 parts, parts of at least 256
     bytes): counts equal the reference loop's, including files whose last
     window repeats (tail re-scans), empty and one-byte files, and files with
     runs of 32 or more word bytes that make a part boundary or the reference's
@@ -180,7 +180,7 @@ def test_grep_split_parts_match_reference(tmp_path, orc, chunk):
     ref = [orc.scan_fast(t.tobytes(), d, chunk) for t in datas]
     with make_ctx(page_size=4096 if chunk < 4096 else 65536, cache_bytes=16 << 20) as ctx:
         for parts in (2, 5, 16):
-            counts, status = grep(ctx, paths, d, chunk, threads=512, parts=parts, min_part=64)
+            counts, status = grep(ctx, paths, d, chunk, threads=512, parts=parts, min_part=256)
             assert (status == 0).all(), (parts, status)
             for i in range(len(paths)):
                 assert (counts[i] == ref[i]).all(), (parts, i, sizes[i])

@@ -87,16 +87,18 @@ def chain_last(data, chunk):
 def split_counts(data, words, chunk, parts, min_part):
     """Returns (counts per word, took_fallback)."""
     size = len(data)
-    assert min_part >= 64   # the launcher's bound: parts are then never empty
+    assert min_part >= 256   # the launcher's bound: parts are then never empty
     kf = max(1, min(parts, size // min_part))
     cstr = [w.split(b"\0")[0] for w in words]   # a record is a C string (grep_text_gpu.cu:139-160)
     qset = {w for w in cstr if w and all(ALPHA[b] for b in w)}
     if kf == 1:
         return None, True   # one part: the reference loop itself
     counts, longrun = {}, False
+    def split_point(j):   # part 0 takes 4/5 of a share (it also walks the chain)
+        return 0 if j == 0 else (size if j >= kf else (size * (5 * j - 1) // (5 * kf - 1)) & ~63)
     for j in range(kf):
-        pb = 0 if j == 0 else (size * j // kf) & ~63
-        pe = size if j + 1 == kf else (size * (j + 1) // kf) & ~63
+        pb, pe = split_point(j), split_point(j + 1)
+        assert pe - pb >= 64 or kf == 1
         wins, lr = part_windows(data, chunk, pb, pe)
         longrun |= lr
         for a, e in wins:
@@ -142,7 +144,7 @@ def test_split_equals_reference(orc, long_runs):
         words = [w for w in words if w]
         chunk = int(rng.choice([8, 16, 64, 100, 256]))
         parts = int(rng.integers(2, 9))
-        got, fell_back = split_counts(data, words, chunk, parts, 64)
+        got, fell_back = split_counts(data, words, chunk, parts, 256)
         ref = list(orc.scan_fast(data.tobytes(), dict_rec(words), chunk))
         if fell_back:
             continue   # the device then runs the reference loop itself
@@ -158,19 +160,20 @@ def test_split_tail_rescan_counts_twice(orc):
     """A file whose reference last window is longer than the chunk: its final
     bytes are counted again by the reference, and the split path adds them.
     With a chunk below 32 bytes the last window repeats several times."""
-    data = np.frombuffer(b"ab " * 49 + b"cab", dtype=np.uint8).copy()   # 150 bytes
+    n = next(n for n in range(200, 400) if len(b"ab " * n) - chain_last(np.frombuffer(b"ab " * n, dtype=np.uint8), 64)[0] > 64)
+    data = np.frombuffer(b"ab " * n + b"cab", dtype=np.uint8).copy()
     words = [b"ab", b"cab"]
     chunk = 64
     cl, cut = chain_last(data, chunk)
-    assert cl == 65 and not cut and len(data) - cl > chunk   # byte 64 is the 'b' of an "ab"
-    got, fb = split_counts(data, words, chunk, 2, 64)
+    assert not cut and len(data) - cl > chunk
+    got, fb = split_counts(data, words, chunk, 2, 256)
     ref = list(orc.scan_fast(data.tobytes(), dict_rec(words), chunk))
     assert not fb and got == ref
-    assert ref != [49, 1]   # the re-scanned tail is counted again
-    data = np.frombuffer(b"ab " * 60, dtype=np.uint8).copy()
-    got, fb = split_counts(data, [b"ab"], 8, 2, 64)
+    assert ref != [n, 1]   # the re-scanned tail is counted again
+    data = np.frombuffer(b"ab " * 200, dtype=np.uint8).copy()
+    got, fb = split_counts(data, [b"ab"], 8, 2, 256)
     ref = list(orc.scan_fast(data.tobytes(), dict_rec([b"ab"]), 8))
-    assert not fb and got == ref and ref[0] > 61
+    assert not fb and got == ref and ref[0] > 201
 
 
 def test_model_planted_mistake_is_caught(orc):
@@ -181,7 +184,7 @@ def test_model_planted_mistake_is_caught(orc):
         data = adversarial(rng, int(rng.integers(200, 3000)), False)
         words = [b"ab", b"abc", b"cab", b"a", b"Zz"]
         chunk = 64
-        got, fb = split_counts(data, words, chunk, 3, 64)
+        got, fb = split_counts(data, words, chunk, 3, 256)
         if fb:
             continue
         cl, _ = chain_last(data, chunk)
