@@ -69,6 +69,7 @@ SIGNATURES = {
     "gpufs_launch_gread_ranges": (I, [P, P, U32, U64, P, P, U32, P]),
     "gpufs_launch_grep": (I, [P, P, U32, U32, P, U32, U32, P, P, U32, P]),
     "gpufs_launch_grep_split": (I, [P, P, U32, U32, P, U32, U32, P, P, U32, U64, U32, P]),
+    "gpufs_launch_grep_noalloc": (I, [P, P, U32, U32, P, U32, U32, P, P, U32, U64, U32, P]),
     "gpufs_launch_grep_batched": (I, [P, P, U32, U32, P, U32, U32, P, P, P, U32, U32, P, P]),
     "gpufs_grep_batched_graph": (I, [P, P, U32, U32, P, U32, U32, P, P, P, U32, U32, P, ctypes.POINTER(P)]),
     "gpufs_graph_launch": (I, [P, P]),

@@ -132,6 +132,55 @@ extern "C" int gpufs_launch_grep_split(gpufs_ctx* ctx, const char* names_dev, ui
     return v;
 }
 
+// ---------------------------------------------------------------------------
+// The split scan without any allocation (for callers that cannot allocate:
+// code under graph capture, serving loops).  Its per-file state lives in
+// `scratch`, a file-scope device array; every GPU holds its own copy at its
+// own address, which gpufs_init looks up once per device
+// (grep_scratch_setup).  Calls on one GPU share the array: each call clears
+// the entries it uses with a fill kernel first, and calls that overlap in
+// time on one GPU race.
+#define GREP_NOALLOC_MAX_FILES 65536u
+#define GREP_MAX_DEVICES 64
+static __device__ __align__(256) grep_split_state scratch[GREP_NOALLOC_MAX_FILES];
+static grep_split_state* scratch_dev[GREP_MAX_DEVICES];   // device address of `scratch` per device
+
+// Called by gpufs_init with `device` current.  In host code the array's name
+// is only a host-side stand-in, so its device address is taken here once and
+// used everywhere the host passes the array on.
+__attribute__((visibility("hidden"))) int grep_scratch_setup(int device) {
+    if (device < 0 || device >= GREP_MAX_DEVICES) return -ENODEV;
+    if (__atomic_load_n(&scratch_dev[device], __ATOMIC_ACQUIRE)) return 0;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -EIO;
+    __atomic_store_n(&scratch_dev[device], (grep_split_state*)p, __ATOMIC_RELEASE);
+    return 0;
+}
+
+extern "C" int gpufs_launch_grep_noalloc(gpufs_ctx* ctx, const char* names_dev, uint32_t name_stride,
+                                         uint32_t n_files, const char* dict_dev, uint32_t n_words,
+                                         uint32_t chunk_bytes, uint32_t* counts_dev, int32_t* status_dev,
+                                         uint32_t parts_per_file, uint64_t min_part_bytes, uint32_t threads,
+                                         void* stream) {
+    if (!ctx || !names_dev || !dict_dev || !counts_dev || !status_dev) return -EINVAL;
+    int v = grep_args_ok(n_files, n_words, chunk_bytes, threads);
+    if (v) return v;
+    if (n_files > GREP_NOALLOC_MAX_FILES) return -E2BIG;
+    if (parts_per_file == 0) parts_per_file = grep_auto_parts(n_files);
+    if (parts_per_file > 64) return -EINVAL;
+    if (min_part_bytes == 0) min_part_bytes = 4ull * chunk_bytes > (256u << 10) ? 4ull * chunk_bytes : (256u << 10);
+    if (min_part_bytes < 256) return -EINVAL;
+    gpufs_config cc;
+    gpufs_get_config(ctx, &cc);
+    if (cc.device < 0 || cc.device >= GREP_MAX_DEVICES) return -ENODEV;
+    grep_split_state* fs = __atomic_load_n(&scratch_dev[cc.device], __ATOMIC_ACQUIRE);
+    if (!fs) return -ENODEV;   // gpufs_init did not set up this device
+    if ((v = gpufs_bind(ctx))) return v;
+    hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)gpufs_stream(ctx);
+    return launch_grep_split(ctx, names_dev, name_stride, n_files, dict_dev, n_words, chunk_bytes, counts_dev,
+                             status_dev, parts_per_file, min_part_bytes, fs, threads, s);
+}
+
 extern "C" int gpufs_launch_grep(gpufs_ctx* ctx, const char* names_dev, uint32_t name_stride, uint32_t n_files,
                                  const char* dict_dev, uint32_t n_words, uint32_t chunk_bytes,
                                  uint32_t* counts_dev, int32_t* status_dev, uint32_t threads, void* stream) {

@@ -37,6 +37,9 @@ struct gpufs_ctx {
 // Resident kernels hold every compute unit, so a device copy issued beside
 // them (done by a blit kernel) would wait until they exit.
 __attribute__((visibility("hidden"))) void gpufs_resident_add(gpufs_ctx* c, int d) { c->resident += d; }
+// Device address of the no-allocation grep scan's state array on `device`
+// (k_grep.hip), looked up once per device.
+int grep_scratch_setup(int device);
 #define REFUSE_IF_RESIDENT(c) do { if ((c)->resident.load() > 0) return -EBUSY; } while (0)
 
 #define HIP_TRY(x)                                                        \
@@ -145,6 +148,8 @@ extern "C" int gpufs_init(const gpufs_config* cfg_in, gpufs_ctx** out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return -ENODEV;
     if (cfg.device < 0 || cfg.device >= ndev) return -ENODEV;
     HIP_TRY(hipSetDevice(cfg.device));
+    v = grep_scratch_setup(cfg.device);
+    if (v) return v;
 
     gpufs_ctx* c = new (std::nothrow) gpufs_ctx();
     if (!c) return -ENOMEM;

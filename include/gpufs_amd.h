@@ -144,6 +144,18 @@ int gpufs_launch_grep_split(gpufs_ctx* ctx, const char* names_dev, uint32_t name
                             uint32_t chunk_bytes, uint32_t* counts_dev, int32_t* status_dev,
                             uint32_t parts_per_file, uint64_t min_part_bytes, uint32_t threads,
                             void* stream);
+/* gpufs_launch_grep_split without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * hipGraph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * per-file state lives in a fixed device array of the library (24 B x 65,536
+ * files per GPU, held until the process ends); n_files above 65,536 returns
+ * -E2BIG.  Calls on one GPU share that array (each call clears what it uses
+ * first), so calls that overlap in time on one GPU race. */
+int gpufs_launch_grep_noalloc(gpufs_ctx* ctx, const char* names_dev, uint32_t name_stride,
+                              uint32_t n_files, const char* dict_dev, uint32_t n_words,
+                              uint32_t chunk_bytes, uint32_t* counts_dev, int32_t* status_dev,
+                              uint32_t parts_per_file, uint64_t min_part_bytes, uint32_t threads,
+                              void* stream);
 
 /* Batched form of the grep scan over many files: `workgroups` persistent
  * workgroups build the dictionary table once, then pull file indices from the
