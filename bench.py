@@ -54,9 +54,8 @@ def parse():
     ap.add_argument("--cold-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grep", action="store_true")
-    ap.add_argument("--batched", action="store_true",
-                    help="run the batched leg (10,000 uneven files, one hipGraph-captured persistent kernel); off by "
-                         "default while a GPU fault of its kernel is unexplained")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched leg (10,000 uneven files, one hipGraph-captured persistent kernel)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--batched-files", type=int, default=10000)
     ap.add_argument("--no-gemv", action="store_true",
@@ -286,7 +285,7 @@ def main():
     ctx.close()
     del dst
     torch.cuda.empty_cache()
-    batched_res = bench_batched(args, torch, L, _lib, dg) if args.batched else None
+    batched_res = bench_batched(args, torch, L, _lib, dg) if not args.no_batched else None
     gemv_res = bench_gemv(args, torch, L, _lib, dg) if not args.no_gemv else None
     serve_res = bench_serve(args, torch, L, _lib, dg) if not args.no_serve else None
     line = {
@@ -435,7 +434,9 @@ def bench_batched(args, torch, L, _lib, dg):
                        host_threads=args.host_threads, max_files=2 * n)
     counts = torch.zeros(n * 1024, dtype=torch.int32, device="cuda")
     status = torch.zeros(n, dtype=torch.int32, device="cuda")
-    queue = torch.zeros(1, dtype=torch.int32, device="cuda")
+    parts = best["parts_per_file"]
+    wsb = int(L.gpufs_grep_batched_workspace_bytes(n, parts))
+    ws = torch.zeros((wsb + 7) // 8, dtype=torch.int64, device="cuda")
     times = torch.zeros(2 * n, dtype=torch.int64, device="cuda")
     torch.cuda.synchronize()
     ctx.host_start()
@@ -443,8 +444,8 @@ def bench_batched(args, torch, L, _lib, dg):
     g = ctypes.c_void_p()
     _lib.check("graph", L.gpufs_grep_batched_graph(
         ctx.h, ctypes.c_void_p(names_d.data_ptr()), 256, n, ctypes.c_void_p(dict_d.data_ptr()), 1024, 65536,
-        ctypes.c_void_p(counts.data_ptr()), ctypes.c_void_p(status.data_ptr()), ctypes.c_void_p(queue.data_ptr()),
-        best["workgroups"], best["threads"], ctypes.c_void_p(times.data_ptr()), ctypes.byref(g)))
+        ctypes.c_void_p(counts.data_ptr()), ctypes.c_void_p(status.data_ptr()), ctypes.c_void_p(ws.data_ptr()),
+        best["workgroups"], best["threads"], parts, 0, ctypes.c_void_p(times.data_ptr()), ctypes.byref(g)))
 
     def replay():
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -472,10 +473,11 @@ def bench_batched(args, torch, L, _lib, dg):
     torch.cuda.synchronize()
     t = time.perf_counter()
     for i in range(n_plain):
-        _lib.check("plain", L.gpufs_launch_grep(ctx.h, ctypes.c_void_p(names_d[i].data_ptr()), 256, 1,
-                                                ctypes.c_void_p(dict_d.data_ptr()), 1024, 65536,
-                                                ctypes.c_void_p(one.data_ptr()), ctypes.c_void_p(st1.data_ptr()),
-                                                1024, None))
+        # the reference's form: one workgroup per file, one launch per file
+        _lib.check("plain", L.gpufs_launch_grep_split(ctx.h, ctypes.c_void_p(names_d[i].data_ptr()), 256, 1,
+                                                      ctypes.c_void_p(dict_d.data_ptr()), 1024, 65536,
+                                                      ctypes.c_void_p(one.data_ptr()), ctypes.c_void_p(st1.data_ptr()),
+                                                      1, 0, 1024, None))
     ev = torch.cuda.Event()
     ev.record(stream)
     ev.synchronize()
@@ -627,10 +629,11 @@ def bench_serve(args, torch, L, _lib, dg):
         torch.cuda.synchronize()
         t = time.perf_counter()
         for i in range(npl):
-            _lib.check("plain", L.gpufs_launch_grep(ctx.h, ctypes.c_void_p(srv.names[int(ids[i])].data_ptr()), 256,
-                                                    1, ctypes.c_void_p(srv.dict.data_ptr()), q.shape[0], 65536,
-                                                    ctypes.c_void_p(outs[i].data_ptr()),
-                                                    ctypes.c_void_p(sts[i:].data_ptr()), 1024, None))
+            # the reference's form: one workgroup per file, one launch per request
+            _lib.check("plain", L.gpufs_launch_grep_split(ctx.h, ctypes.c_void_p(srv.names[int(ids[i])].data_ptr()),
+                                                          256, 1, ctypes.c_void_p(srv.dict.data_ptr()), q.shape[0],
+                                                          65536, ctypes.c_void_p(outs[i].data_ptr()),
+                                                          ctypes.c_void_p(sts[i:].data_ptr()), 1, 0, 1024, None))
         ev = torch.cuda.Event()
         ev.record(stream)
         ev.synchronize()
@@ -638,7 +641,7 @@ def bench_serve(args, torch, L, _lib, dg):
         assert (sts.cpu().numpy() == 0).all()
         assert (outs.cpu().numpy().view(np.uint32) == c[:npl]).all(), "plain and served counts differ"
         plain = {"requests": npl, "files_per_s": round(npl / t_plain, 1),
-                 "form": "one gpufs_launch_grep launch per request on the context stream, same cache"}
+                 "form": "one launch of one workgroup per request on the context stream, same cache"}
         out["plain_per_file_launch"] = plain
         out["served_over_plain"] = round(sat["files_per_s"] / plain["files_per_s"], 2)
         log(f"plain per-file launches: {plain}; served/plain = {out['served_over_plain']}x")

@@ -70,8 +70,9 @@ SIGNATURES = {
     "gpufs_launch_grep": (I, [P, P, U32, U32, P, U32, U32, P, P, U32, P]),
     "gpufs_launch_grep_split": (I, [P, P, U32, U32, P, U32, U32, P, P, U32, U64, U32, P]),
     "gpufs_launch_grep_noalloc": (I, [P, P, U32, U32, P, U32, U32, P, P, U32, U64, U32, P]),
-    "gpufs_launch_grep_batched": (I, [P, P, U32, U32, P, U32, U32, P, P, P, U32, U32, P, P]),
-    "gpufs_grep_batched_graph": (I, [P, P, U32, U32, P, U32, U32, P, P, P, U32, U32, P, ctypes.POINTER(P)]),
+    "gpufs_grep_batched_workspace_bytes": (U64, [U32, U32]),
+    "gpufs_launch_grep_batched": (I, [P, P, U32, U32, P, U32, U32, P, P, P, U32, U32, U32, U64, P, P]),
+    "gpufs_grep_batched_graph": (I, [P, P, U32, U32, P, U32, U32, P, P, P, U32, U32, U32, U64, P, ctypes.POINTER(P)]),
     "gpufs_graph_launch": (I, [P, P]),
     "gpufs_graph_destroy": (I, [P]),
     "gpufs_serve_start": (I, [P, ctypes.POINTER(ServeConfig), P, U32, U32, P, P, ctypes.POINTER(P)]),

@@ -1,11 +1,13 @@
 """Autotuner for the batched form: page size, workgroup count / size and RPC
 ring depth are chosen by measuring the real workload on a sample of its files.
 
-Search: an exhaustive sweep of a small grid (page size x threads per
-workgroup x workgroups x ring depth), each point run cold (page cache
+Search: a sweep of a small grid (page size x threads per workgroup x
+workgroups x ring depth x parts per file), each point run cold (page cache
 emptied, host page cache warm) on the same file sample; the point with the
-highest files/s wins.  The grid is kept small on purpose: every point builds
-a fresh context (the page size is fixed per context).
+highest files/s wins.  Parts per file only change the run when the sample
+holds files large enough to split, so that axis is swept on the best point of
+the others.  Every point builds a fresh context (the page size is fixed per
+context).
 """
 import ctypes
 import itertools
@@ -18,8 +20,9 @@ from . import _lib
 DEFAULT_SPACE = {
     "page_size": [64 << 10, 256 << 10, 1 << 20],
     "threads": [512, 1024],
-    "workgroups_per_cu": [1],
+    "workgroups_per_cu": [1],          # the scan's LDS holds one workgroup per compute unit
     "ring_depth": [256, 1024],
+    "parts_per_file": [1, 4, 16],      # workgroups a large file may be split over
 }
 
 
@@ -36,7 +39,9 @@ def batched_trial(cfg, names_d, n_files, dict_d, n_words, chunk, cache_bytes, ho
     try:
         counts = torch.zeros(n_files * n_words, dtype=torch.int32, device="cuda")
         status = torch.zeros(n_files, dtype=torch.int32, device="cuda")
-        queue = torch.zeros(1, dtype=torch.int32, device="cuda")
+        parts = cfg.get("parts_per_file", 8)
+        wsb = int(L.gpufs_grep_batched_workspace_bytes(n_files, parts))
+        ws = torch.zeros((wsb + 7) // 8, dtype=torch.int64, device="cuda")
         torch.cuda.synchronize()
         ctx.host_start()
         stream = torch.cuda.ExternalStream(ctx.stream)
@@ -45,8 +50,8 @@ def batched_trial(cfg, names_d, n_files, dict_d, n_words, chunk, cache_bytes, ho
         rc = L.gpufs_launch_grep_batched(ctx.h, ctypes.c_void_p(names_d.data_ptr()), 256, n_files,
                                          ctypes.c_void_p(dict_d.data_ptr()), n_words, chunk,
                                          ctypes.c_void_p(counts.data_ptr()), ctypes.c_void_p(status.data_ptr()),
-                                         ctypes.c_void_p(queue.data_ptr()), cfg["workgroups"], cfg["threads"], None,
-                                         None)
+                                         ctypes.c_void_p(ws.data_ptr()), cfg["workgroups"], cfg["threads"], parts, 0,
+                                         None, None)
         _lib.check("gpufs_launch_grep_batched", rc)
         e1.record(stream)
         e1.synchronize()
@@ -66,17 +71,30 @@ def tune_batched(names_d, n_files, dict_d, n_words, chunk, cache_bytes, host_thr
     cus = _device_cus()
     table = []
     best = None
-    for page, threads, wpc, ring in itertools.product(space["page_size"], space["threads"],
-                                                      space["workgroups_per_cu"], space["ring_depth"]):
-        if page * ring > (1 << 30):   # pinned staging kept at or below 1 GiB
-            continue
-        cfg = {"page_size": page, "threads": threads, "workgroups": int(wpc * cus), "ring_depth": ring}
+    def trial(cfg):
         t = time.time()
         fps, dt = batched_trial(cfg, names_d, n_files, dict_d, n_words, chunk, cache_bytes, host_threads, torch)
         row = dict(cfg, files_per_s=round(fps, 1), kernel_s=round(dt, 5))
         table.append(row)
         if log:
             log(f"autotune {row} ({time.time() - t:.1f}s)")
+        return fps
+
+    p0 = space["parts_per_file"][len(space["parts_per_file"]) // 2]
+    for page, threads, wpc, ring in itertools.product(space["page_size"], space["threads"],
+                                                      space["workgroups_per_cu"], space["ring_depth"]):
+        if page * ring > (1 << 30):   # pinned staging kept at or below 1 GiB
+            continue
+        cfg = {"page_size": page, "threads": threads, "workgroups": int(wpc * cus), "ring_depth": ring,
+               "parts_per_file": p0}
+        fps = trial(cfg)
         if best is None or fps > best[1]:
+            best = (cfg, fps)
+    for parts in space["parts_per_file"]:
+        if parts == p0:
+            continue
+        cfg = dict(best[0], parts_per_file=parts)
+        fps = trial(cfg)
+        if fps > best[1]:
             best = (cfg, fps)
     return best[0], table

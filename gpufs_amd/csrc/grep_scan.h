@@ -624,28 +624,43 @@ __device__ __noinline__ int grep_split_finish(grep_lds& S, int fd, uint32_t nw, 
 // exactly once, after adding its counts into the caller-zeroed row (a
 // workgroup whose gopen failed records the error); the last to arrive
 // completes the row (grep_split_finish) and writes status[0].
+//
+// `target` is the number of arrivals that completes the file: `parts` when a
+// workgroup is launched for every part slot (gpufs_launch_grep), or 0 for
+// "the file's own part count" (K_f, 1 when gopen failed) when only the parts
+// that exist are run (the batched form, whose part 0 learns K_f and hands the
+// other parts out: on_size(K_f) is called by every thread once the size is
+// known, or with 1 when gopen failed).
+//
+// With fs == nullptr the file is scanned by this workgroup alone (the
+// reference's form, one workgroup per file): its counts stay in LDS, the tail
+// is added there and the row is stored at the end, so nothing needs clearing
+// before the launch and no state is shared.
+template <class OnSize>
 __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t nw, uint32_t chunk, uint32_t* row,
                                        int32_t* status, uint32_t j, uint32_t parts, uint64_t min_part,
-                                       grep_split_state* fs) {
+                                       grep_split_state* fs, uint32_t target, OnSize&& on_size) {
     const uint32_t T = gfs::nthreads(), t = gfs::tid();
     __shared__ uint32_t fin_flag;
+    const bool single = fs == nullptr;
     for (uint32_t q = t; q < nw; q += T) S.counts[q] = 0;
     const int fd = gopen(name, O_GRDONLY);
-    uint64_t size = 0;
+    uint64_t size = 0, kf = 1, cl = 0;
     int st = fd < 0 ? fd : 0;
-    uint32_t lr = 0;
+    uint32_t lr = 0, cut = 0;
     if (fd >= 0) {
         if (t == 0) S.fsize = fstat(fd);
         __syncthreads();
         size = gfs::uni((uint64_t)S.fsize);
-        uint64_t kf = size / min_part;
+        kf = single ? 1 : size / min_part;
         kf = kf < 1 ? 1 : (kf > parts ? parts : kf);
+    }
+    on_size((uint32_t)kf);
+    if (fd >= 0) {
         if (j == 0 && size) {
             // the reference's window chain, while the other parts scan
-            uint64_t cl = 0;
-            uint32_t cut = 0;
             st = grep_chain_last(S, fd, size, chunk, &cl, &cut);
-            if (t == 0) {
+            if (t == 0 && !single) {
                 fs->c_last = cl;
                 fs->cut = cut;
             }
@@ -653,20 +668,38 @@ __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t n
         if (j < kf && size && !st) {
             const uint64_t pb = split_point(size, j, kf), pe = split_point(size, j + 1, kf);
             st = grep_part_windows(S, fd, chunk, size, pb, pe, &lr);
+            if (!single) {
+                for (uint32_t q = t; q < nw; q += T) {
+                    const uint32_t k = S.canon[q];
+                    const uint32_t v = k == 0xFFFF ? 0u : S.counts[k];
+                    if (v) atomicAdd(&row[q], v);
+                }
+                __threadfence();
+            }
+        }
+    }
+    if (single) {
+        // alone on the file: the counts stay in LDS until the row is stored
+        if (fd >= 0 && size && !st && (lr || cut)) {
+            st = grep_recount(S, fd, nw, chunk, row);
+        } else {
+            if (fd >= 0 && size && !st && size - cl > chunk) st = grep_tail(S, fd, cl + chunk, size, chunk);
             for (uint32_t q = t; q < nw; q += T) {
                 const uint32_t k = S.canon[q];
-                const uint32_t v = k == 0xFFFF ? 0u : S.counts[k];
-                if (v) atomicAdd(&row[q], v);
+                row[q] = k == 0xFFFF || fd < 0 ? 0u : S.counts[k];
             }
-            __threadfence();
         }
+        const int c = fd >= 0 ? gclose(fd) : 0;
+        if (t == 0) *status = st ? st : c;
+        __syncthreads();
+        return;
     }
     __syncthreads();
     if (t == 0) {
         if (lr) atomicOr(&fs->longrun, 1u);
         if (st) atomicCAS(&fs->err, 0, st);
         const uint32_t old = __hip_atomic_fetch_add(&fs->arrived, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        fin_flag = old + 1 == parts ? 1u : 0u;
+        fin_flag = old + 1 == (target ? target : (uint32_t)kf) ? 1u : 0u;
     }
     __syncthreads();
     if (gfs::uni(fin_flag)) {

@@ -32,42 +32,117 @@ __global__ __launch_bounds__(1024) void k_grep(const char* names, uint32_t strid
     const uint32_t f = blockIdx.x / parts, j = blockIdx.x % parts;
     grep_build_dict(S, dict, nw);
     [[clang::always_inline]] grep_file_split(S, names + (size_t)f * stride, nw, chunk, counts_out + (size_t)f * nw,
-                                             status + f, j, parts, min_part, fstate + f);
+                                             status + f, j, parts, min_part, fstate ? fstate + f : nullptr, parts,
+                                             [](uint32_t) {});
 }
 
-// Persistent form: each workgroup builds the dictionary table once, then pulls
-// file indices from a device counter until the list is exhausted.
+// ---------------------------------------------------------------------------
+// Batched form: one persistent kernel over many files of uneven size.  Each
+// workgroup builds the dictionary table once and then takes work items until
+// none is left: a new file from the file queue (a device counter), or an extra
+// part of a file another workgroup has opened.  The workgroup that takes a
+// file is its part 0: it opens it, learns its part count K_f (the split of
+// grep_file_split) and publishes parts 1..K_f-1 in the part ring, where other
+// workgroups take them, so a large file is scanned by several workgroups while
+// small files take one each.  Work is over when the file queue is exhausted,
+// no file taken is still publishing its parts, and the ring is empty.
+struct grep_batch_ctl {
+    uint32_t next_file;   // file queue: next file index
+    uint32_t expanding;   // files taken whose extra parts are not published yet
+    uint32_t head, tail;  // part ring: items [head, tail) reserved or published
+    uint32_t pad[60];     // to 256 bytes
+};
+// Workspace: ctl | per-file split state | part ring (8-byte items:
+// ready << 63 | file << 24 | K_f << 8 | part).
+static uint64_t batch_ws_bytes(uint32_t n_files, uint32_t parts) {
+    return sizeof(grep_batch_ctl) + sizeof(grep_split_state) * (uint64_t)n_files +
+           8ull * (uint64_t)n_files * (parts > 1 ? parts - 1 : 1);
+}
+
+#define BATCH_STOP 0xFFFFFFFFFFFFFFFFull
+
 __global__ __launch_bounds__(1024) void k_grep_batched(const char* names, uint32_t stride, uint32_t nfiles,
                                                        const char* dict, uint32_t nw, uint32_t chunk,
-                                                       uint32_t* counts_out, int32_t* status,
-                                                       unsigned int* queue, uint64_t* times) {
+                                                       uint32_t* counts_out, int32_t* status, uint8_t* ws,
+                                                       uint32_t parts, uint64_t min_part, uint64_t* times) {
     __shared__ __attribute__((aligned(16))) grep_lds S;
-    __shared__ unsigned int cur;
+    __shared__ uint64_t item;
+    grep_batch_ctl* q = (grep_batch_ctl*)ws;
+    grep_split_state* fs = (grep_split_state*)(ws + sizeof(grep_batch_ctl));
+    uint64_t* ring = (uint64_t*)(fs + nfiles);
     grep_build_dict(S, dict, nw);
-    // Every take returns a distinct index, so a workgroup runs at most nfiles
-    // + 1 rounds; the bound makes that explicit, so the loop ends even if the
-    // queue word were corrupted.
-    for (uint32_t round = 0; round <= nfiles; ++round) {
-        if (gfs::tid() == 0) cur = atomicAdd(queue, 1u);
-        __syncthreads();
-        const unsigned int f = gfs::uni(cur);
-        __syncthreads();
-        if (f >= nfiles) break;
-        const uint64_t t0 = gfs::now_ticks();
-        int st;
-        [[clang::always_inline]] st = grep_one_file(S, names + (size_t)f * stride, nw, chunk,
-                                                    counts_out + (size_t)f * nw);
-        if (gfs::tid() == 0) {
-            status[f] = st;
-            if (times) {
-                times[2 * (size_t)f] = t0;
-                times[2 * (size_t)f + 1] = gfs::now_ticks();
+    const uint32_t t = gfs::tid();
+    gfs_dev* g = gfs_g;
+    // Every item is taken once and a workgroup handles one per round: at most
+    // nfiles * parts + 1 rounds.
+    const uint64_t rounds = (uint64_t)nfiles * parts + 1;
+    for (uint64_t round = 0; round < rounds; ++round) {
+        if (t == 0) {
+            uint64_t it = BATCH_STOP;
+            const uint64_t t0 = gfs::now_ticks();
+            for (uint32_t spins = 0;; ++spins) {
+                // an extra part first: its file is open already
+                uint32_t h = __hip_atomic_load(&q->head, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t tl = __hip_atomic_load(&q->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                if (h < tl) {
+                    if (__hip_atomic_compare_exchange_strong(&q->head, &h, h + 1, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT)) {
+                        // reserved by a running part 0, published right after: bounded wait
+                        uint64_t v;
+                        while (!((v = __hip_atomic_load(&ring[h], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) >> 63) &&
+                               gfs::now_ticks() - t0 < g->wait_ticks)
+                            __builtin_amdgcn_s_sleep(2);
+                        it = v;   // not ready after the wait limit: stop this workgroup
+                        break;
+                    }
+                    continue;
+                }
+                if (__hip_atomic_load(&q->next_file, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < nfiles) {
+                    // count the file as expanding before taking it, so no workgroup
+                    // stops while its parts may still be published
+                    atomicAdd(&q->expanding, 1u);
+                    const uint32_t f = atomicAdd(&q->next_file, 1u);
+                    if (f < nfiles) {
+                        it = (1ull << 63) | ((uint64_t)f << 24);   // part 0, K_f not known yet
+                        break;
+                    }
+                    atomicSub(&q->expanding, 1u);
+                    continue;
+                }
+                if (__hip_atomic_load(&q->expanding, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
+                    __hip_atomic_load(&q->head, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >=
+                        __hip_atomic_load(&q->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT))
+                    break;   // nothing left
+                if (gfs::now_ticks() - t0 > g->wait_ticks) break;
+                if (spins < 64) __builtin_amdgcn_s_sleep(2);
+                else __builtin_amdgcn_s_sleep(32);
             }
+            item = it;
         }
-        // A barrier between this thread-0 region and the next iteration's keeps
-        // the compiler from threading the loop-invariant `tid == 0` test across
-        // the back-edge, which would send the other lanes round the loop
-        // without thread 0.
+        __syncthreads();
+        const uint64_t it = gfs::uni((uint64_t)item);
+        __syncthreads();
+        if (it == BATCH_STOP || !(it >> 63)) break;
+        const uint32_t f = (uint32_t)((it >> 24) & 0x7FFFFFFFFFull);
+        const uint32_t j = (uint32_t)(it & 0xFF), kfi = (uint32_t)((it >> 8) & 0xFFFF);
+        if (f >= nfiles) break;
+        if (j == 0 && t == 0 && times) times[2 * (size_t)f] = gfs::now_ticks();
+        [[clang::always_inline]] grep_file_split(
+            S, names + (size_t)f * stride, nw, chunk, counts_out + (size_t)f * nw, status + f, j, parts, min_part,
+            fs + f, j == 0 ? 0u : kfi, [&](uint32_t kf) {
+                if (j != 0 || t != 0) return;
+                if (kf > 1) {
+                    const uint32_t r0 = atomicAdd(&q->tail, kf - 1);
+                    for (uint32_t x = 1; x < kf; ++x)
+                        __hip_atomic_store(&ring[r0 + x - 1],
+                                           (1ull << 63) | ((uint64_t)f << 24) | ((uint64_t)kf << 8) | x,
+                                           __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                __hip_atomic_fetch_sub(&q->expanding, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            });
+        if (t == 0 && times) times[2 * (size_t)f + 1] = gfs::now_ticks();   // the last part's end is kept
+        // A barrier between this thread-0 region and the next round's keeps the
+        // compiler from threading the `tid == 0` test across the back-edge.
         __syncthreads();
     }
 }
@@ -101,6 +176,12 @@ static int launch_grep_split(gpufs_ctx* ctx, const char* names_dev, uint32_t nam
                              const char* dict_dev, uint32_t n_words, uint32_t chunk_bytes, uint32_t* counts_dev,
                              int32_t* status_dev, uint32_t parts, uint64_t min_part, grep_split_state* fs,
                              uint32_t threads, hipStream_t s) {
+    if (parts == 1) {
+        // one workgroup per file: no shared state, nothing to clear first
+        hipLaunchKernelGGL(k_grep, dim3(n_files), dim3(threads), 0, s, names_dev, name_stride, dict_dev, n_words,
+                           chunk_bytes, counts_dev, status_dev, 1u, min_part, (grep_split_state*)nullptr);
+        return hipGetLastError() == hipSuccess ? 0 : -EIO;
+    }
     const uint64_t n_counts = (uint64_t)n_files * n_words;
     const uint64_t n_clear = n_counts > n_files ? n_counts : n_files;
     hipLaunchKernelGGL(k_grep_clear, dim3((uint32_t)((n_clear + 255) / 256)), dim3(256), 0, s, counts_dev, n_counts,
@@ -124,6 +205,9 @@ extern "C" int gpufs_launch_grep_split(gpufs_ctx* ctx, const char* names_dev, ui
     if (min_part_bytes < 256) return -EINVAL;
     if ((v = gpufs_bind(ctx))) return v;
     hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)gpufs_stream(ctx);
+    if (parts_per_file == 1)
+        return launch_grep_split(ctx, names_dev, name_stride, n_files, dict_dev, n_words, chunk_bytes, counts_dev,
+                                 status_dev, 1, min_part_bytes, nullptr, threads, s);
     grep_split_state* fs = nullptr;
     if (hipMallocAsync((void**)&fs, sizeof(grep_split_state) * (size_t)n_files, s) != hipSuccess) return -ENOMEM;
     v = launch_grep_split(ctx, names_dev, name_stride, n_files, dict_dev, n_words, chunk_bytes, counts_dev,
@@ -188,51 +272,84 @@ extern "C" int gpufs_launch_grep(gpufs_ctx* ctx, const char* names_dev, uint32_t
                                    status_dev, 0, 0, threads, stream);
 }
 
-static void launch_batched(hipStream_t s, const char* names_dev, uint32_t name_stride, uint32_t n_files,
-                           const char* dict_dev, uint32_t n_words, uint32_t chunk_bytes, uint32_t* counts_dev,
-                           int32_t* status_dev, uint32_t* queue_dev, uint32_t workgroups, uint32_t threads,
-                           uint64_t* times_dev) {
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, (unsigned int*)queue_dev, 1u);
+// Zeroes the batched form's workspace (control words, split state, part ring)
+// and the count rows: a fill kernel, so the launch can be captured in a graph.
+__global__ void k_batch_clear(uint64_t* ws, uint64_t ws_words, uint32_t* counts, uint64_t n_counts) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ws_words) ws[i] = 0;
+    if (i < n_counts) counts[i] = 0;
+}
+
+static int launch_batched(hipStream_t s, const char* names_dev, uint32_t name_stride, uint32_t n_files,
+                          const char* dict_dev, uint32_t n_words, uint32_t chunk_bytes, uint32_t* counts_dev,
+                          int32_t* status_dev, void* ws_dev, uint32_t workgroups, uint32_t threads,
+                          uint32_t parts, uint64_t min_part, uint64_t* times_dev) {
+    const uint64_t words = batch_ws_bytes(n_files, parts) / 8, nc = (uint64_t)n_files * n_words;
+    const uint64_t n = words > nc ? words : nc;
+    hipLaunchKernelGGL(k_batch_clear, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, (uint64_t*)ws_dev, words,
+                       counts_dev, nc);
     hipLaunchKernelGGL(k_grep_batched, dim3(workgroups), dim3(threads), 0, s, names_dev, name_stride, n_files,
-                       dict_dev, n_words, chunk_bytes, counts_dev, status_dev, (unsigned int*)queue_dev, times_dev);
+                       dict_dev, n_words, chunk_bytes, counts_dev, status_dev, (uint8_t*)ws_dev, parts, min_part,
+                       times_dev);
+    return hipGetLastError() == hipSuccess ? 0 : -EIO;
+}
+
+static int batched_args(uint32_t n_files, uint32_t n_words, uint32_t chunk_bytes, uint32_t threads,
+                        uint32_t workgroups, uint32_t* parts, uint64_t* min_part) {
+    int v = grep_args_ok(n_files, n_words, chunk_bytes, threads);
+    if (v) return v;
+    if (workgroups == 0 || n_files >= (1u << 31)) return -EINVAL;
+    if (*parts == 0) *parts = 8;
+    if (*parts > 255) return -EINVAL;
+    if (*min_part == 0) *min_part = 4ull * chunk_bytes > (1u << 20) ? 4ull * chunk_bytes : (1u << 20);
+    if (*min_part < 256) return -EINVAL;
+    return 0;
+}
+
+extern "C" uint64_t gpufs_grep_batched_workspace_bytes(uint32_t n_files, uint32_t parts_per_file) {
+    if (parts_per_file == 0) parts_per_file = 8;
+    return batch_ws_bytes(n_files, parts_per_file);
 }
 
 extern "C" int gpufs_launch_grep_batched(gpufs_ctx* ctx, const char* names_dev, uint32_t name_stride,
                                          uint32_t n_files, const char* dict_dev, uint32_t n_words,
                                          uint32_t chunk_bytes, uint32_t* counts_dev, int32_t* status_dev,
-                                         uint32_t* queue_dev, uint32_t workgroups, uint32_t threads,
-                                         uint64_t* file_times_dev, void* stream) {
-    if (!ctx || !names_dev || !dict_dev || !counts_dev || !status_dev || !queue_dev || workgroups == 0) return -EINVAL;
-    int v = grep_args_ok(n_files, n_words, chunk_bytes, threads);
+                                         void* workspace_dev, uint32_t workgroups, uint32_t threads,
+                                         uint32_t parts_per_file, uint64_t min_part_bytes, uint64_t* file_times_dev,
+                                         void* stream) {
+    if (!ctx || !names_dev || !dict_dev || !counts_dev || !status_dev || !workspace_dev) return -EINVAL;
+    int v = batched_args(n_files, n_words, chunk_bytes, threads, workgroups, &parts_per_file, &min_part_bytes);
     if (v) return v;
     if ((v = gpufs_bind(ctx))) return v;
     hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)gpufs_stream(ctx);
-    launch_batched(s, names_dev, name_stride, n_files, dict_dev, n_words, chunk_bytes, counts_dev, status_dev,
-                   queue_dev, workgroups, threads, file_times_dev);
-    return hipGetLastError() == hipSuccess ? 0 : -EIO;
+    return launch_batched(s, names_dev, name_stride, n_files, dict_dev, n_words, chunk_bytes, counts_dev, status_dev,
+                          workspace_dev, workgroups, threads, parts_per_file, min_part_bytes, file_times_dev);
 }
 
-// Capture (queue reset + persistent kernel) into an executable hipGraph.
+// Capture (workspace clear + persistent kernel) into an executable hipGraph.
 extern "C" int gpufs_grep_batched_graph(gpufs_ctx* ctx, const char* names_dev, uint32_t name_stride,
                                         uint32_t n_files, const char* dict_dev, uint32_t n_words,
                                         uint32_t chunk_bytes, uint32_t* counts_dev, int32_t* status_dev,
-                                        uint32_t* queue_dev, uint32_t workgroups, uint32_t threads,
-                                        uint64_t* file_times_dev, void** graph_exec_out) {
-    if (!ctx || !names_dev || !dict_dev || !counts_dev || !status_dev || !queue_dev || !graph_exec_out ||
-        workgroups == 0)
+                                        void* workspace_dev, uint32_t workgroups, uint32_t threads,
+                                        uint32_t parts_per_file, uint64_t min_part_bytes, uint64_t* file_times_dev,
+                                        void** graph_exec_out) {
+    if (!ctx || !names_dev || !dict_dev || !counts_dev || !status_dev || !workspace_dev || !graph_exec_out)
         return -EINVAL;
-    int v = grep_args_ok(n_files, n_words, chunk_bytes, threads);
+    int v = batched_args(n_files, n_words, chunk_bytes, threads, workgroups, &parts_per_file, &min_part_bytes);
     if (v) return v;
     if ((v = gpufs_bind(ctx))) return v;
     hipStream_t s = (hipStream_t)gpufs_stream(ctx);
     if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return -EIO;
-    launch_batched(s, names_dev, name_stride, n_files, dict_dev, n_words, chunk_bytes, counts_dev, status_dev,
-                   queue_dev, workgroups, threads, file_times_dev);
-    hipGraph_t g = nullptr;
-    if (hipStreamEndCapture(s, &g) != hipSuccess || !g) return -EIO;
+    v = launch_batched(s, names_dev, name_stride, n_files, dict_dev, n_words, chunk_bytes, counts_dev, status_dev,
+                       workspace_dev, workgroups, threads, parts_per_file, min_part_bytes, file_times_dev);
+    hipGraph_t gr = nullptr;
+    if (hipStreamEndCapture(s, &gr) != hipSuccess || !gr || v) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return v ? v : -EIO;
+    }
     hipGraphExec_t e = nullptr;
-    hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
+    const hipError_t r = hipGraphInstantiate(&e, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
     if (r != hipSuccess) return -EIO;
     *graph_exec_out = (void*)e;
     return 0;

@@ -157,25 +157,33 @@ int gpufs_launch_grep_noalloc(gpufs_ctx* ctx, const char* names_dev, uint32_t na
                               uint32_t parts_per_file, uint64_t min_part_bytes, uint32_t threads,
                               void* stream);
 
-/* Batched form of the grep scan over many files: `workgroups` persistent
- * workgroups build the dictionary table once, then pull file indices from the
- * device counter *queue_dev (reset to 0 by the launch itself) until n_files are
- * done.  Same per-file outputs and status as gpufs_launch_grep.  When
- * file_times_dev is not NULL it receives each file's start and end wall-clock
- * ticks (2 per file, 100 MHz).  No reference counterpart (the reference
- * launches one grid over a fixed file list). */
+/* Batched form of the grep scan over many files of uneven size: `workgroups`
+ * persistent workgroups build the dictionary table once, then take work until
+ * none is left: a new file from a device queue, or an extra part of a large
+ * file another workgroup opened (a file of S bytes is scanned by
+ * min(parts_per_file, max(1, S / min_part_bytes)) workgroups, as in
+ * gpufs_launch_grep_split; parts_per_file 0 = 8, at most 255; min_part_bytes
+ * 0 = max(1 MiB, 4 * chunk_bytes), else >= 256).  workspace_dev: device
+ * memory of gpufs_grep_batched_workspace_bytes(n_files, parts_per_file)
+ * bytes, cleared by the launch itself.  Same per-file counts and status as
+ * gpufs_launch_grep.  When file_times_dev is not NULL it receives each file's
+ * start and end wall-clock ticks (2 per file, 100 MHz).  No reference
+ * counterpart (the reference launches one grid over a fixed file list). */
+uint64_t gpufs_grep_batched_workspace_bytes(uint32_t n_files, uint32_t parts_per_file);
 int gpufs_launch_grep_batched(gpufs_ctx* ctx, const char* names_dev, uint32_t name_stride,
                               uint32_t n_files, const char* dict_dev, uint32_t n_words,
                               uint32_t chunk_bytes, uint32_t* counts_dev, int32_t* status_dev,
-                              uint32_t* queue_dev, uint32_t workgroups, uint32_t threads,
+                              void* workspace_dev, uint32_t workgroups, uint32_t threads,
+                              uint32_t parts_per_file, uint64_t min_part_bytes,
                               uint64_t* file_times_dev, void* stream);
-/* The same two launches (queue reset, persistent kernel) captured on the
+/* The same two launches (workspace clear, persistent kernel) captured on the
  * context stream into an executable hipGraph; replay with gpufs_graph_launch.
  * The host service must be running (gpufs_host_start) while a replay runs. */
 int gpufs_grep_batched_graph(gpufs_ctx* ctx, const char* names_dev, uint32_t name_stride,
                              uint32_t n_files, const char* dict_dev, uint32_t n_words,
                              uint32_t chunk_bytes, uint32_t* counts_dev, int32_t* status_dev,
-                             uint32_t* queue_dev, uint32_t workgroups, uint32_t threads,
+                             void* workspace_dev, uint32_t workgroups, uint32_t threads,
+                             uint32_t parts_per_file, uint64_t min_part_bytes,
                              uint64_t* file_times_dev, void** graph_exec_out);
 int gpufs_graph_launch(void* graph_exec, void* stream);
 int gpufs_graph_destroy(void* graph_exec);

@@ -155,19 +155,26 @@ def gemv_mmap(ctx, path, rows, cols, x, rows_per_wg, threads=1024):
     return y.cpu().numpy(), status.cpu().numpy(), csum.cpu().numpy().view(np.uint64)
 
 
-def grep_batched(ctx, names_t, nfiles, dict_t, nw, chunk, counts, status, queue, workgroups, threads=1024,
-                 graph=False):
-    """names_t/dict_t/counts/status/queue: device tensors.  Returns a graph
-    handle when graph=True (caller replays and destroys it)."""
+def batched_workspace(nfiles, parts):
+    torch = torch_cuda()
+    n = int(_lib.lib().gpufs_grep_batched_workspace_bytes(nfiles, parts))
+    return dev_zeros((n + 7) // 8, torch.int64)
+
+
+def grep_batched(ctx, names_t, nfiles, dict_t, nw, chunk, counts, status, ws, workgroups, threads=1024,
+                 graph=False, parts=8, min_part=0, times=None):
+    """names_t/dict_t/counts/status/ws: device tensors (ws from batched_workspace).
+    Returns a graph handle when graph=True (caller replays and destroys it)."""
     L = _lib.lib()
     args = (ctx.h, ctypes.c_void_p(names_t.data_ptr()), STRIDE, nfiles, ctypes.c_void_p(dict_t.data_ptr()), nw,
             chunk, ctypes.c_void_p(counts.data_ptr()), ctypes.c_void_p(status.data_ptr()),
-            ctypes.c_void_p(queue.data_ptr()), workgroups, threads)
+            ctypes.c_void_p(ws.data_ptr()), workgroups, threads, parts, min_part,
+            ctypes.c_void_p(times.data_ptr()) if times is not None else None)
     if graph:
         h = ctypes.c_void_p()
-        _lib.check("gpufs_grep_batched_graph", L.gpufs_grep_batched_graph(*args, None, ctypes.byref(h)))
+        _lib.check("gpufs_grep_batched_graph", L.gpufs_grep_batched_graph(*args, ctypes.byref(h)))
         return h
-    _lib.check("gpufs_launch_grep_batched", L.gpufs_launch_grep_batched(*args, None, None))
+    _lib.check("gpufs_launch_grep_batched", L.gpufs_launch_grep_batched(*args, None))
     return None
 
 
