@@ -1502,6 +1502,18 @@ __device__ __noinline__ int pin_pages(gfs_dev* g, int fd, uint64_t pb, uint32_t 
     __syncthreads();
     return 0;
 }
+// Whether page `page` of open file `fd` is cached (VALID in a frame), looked
+// up without pinning or creating anything (one thread).  Advisory: the answer
+// can change right after.
+__device__ inline bool page_cached(gfs_dev* g, int fd, uint64_t page) {
+    int e = 0;
+    const uint64_t node = tree_leaf(g, &g->files[fd], page >> GFS_FANOUT_LOG, false, &e);
+    if (!node) return false;
+    const uint64_t v = ld_rlx(&g->nodes[node * GFS_FANOUT + (page & (GFS_FANOUT - 1))]);
+    if (v == 0 || v == GFS_LEAF_RESERVED) return false;
+    const uint64_t m = ld_rlx(&g->meta[leaf_frame(v)]);
+    return meta_gen(m) == leaf_gen(v) && meta_state(m) == GFS_F_VALID;
+}
 // Drops the pins of pin_pages (workgroup-collective); the caller's reads of
 // the frames have all returned.
 __device__ inline void unpin_pages(gfs_dev* g, uint32_t cnt) {

@@ -563,6 +563,8 @@ struct grep_split_state {
     int32_t err;        // first error of a part
     uint32_t cut;       // part 0's chain: some reference window cuts a token
     uint64_t c_last;    // part 0's chain: start of the reference's last window
+    uint32_t kf;        // part 0's decision: K_f + 1 (0 = not decided yet)
+    uint32_t pad;
 };
 
 // Part boundaries: part 0 also walks the reference's window chain before its
@@ -618,9 +620,9 @@ __device__ __noinline__ int grep_split_finish(grep_lds& S, int fd, uint32_t nw, 
 
 // Part j of the file `name` (workgroup-collective).  The file is split into
 // K_f = min(parts, size / min_part) parts (at least 1; min_part >= 256, so no
-// part is empty); workgroups j >= K_f scan nothing.  Workgroup 0 first walks
-// the reference's window chain (grep_chain_last), overlapped with the other
-// parts' scans.  Every one of the `parts` workgroups of the file arrives
+// part is empty) when it is cached, else into one; workgroups j >= K_f scan
+// nothing.  Workgroup 0 also walks the reference's window chain
+// (grep_chain_last).  Every one of the `parts` workgroups of the file arrives
 // exactly once, after adding its counts into the caller-zeroed row (a
 // workgroup whose gopen failed records the error); the last to arrive
 // completes the row (grep_split_finish) and writes status[0].
@@ -648,17 +650,52 @@ __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t n
     uint64_t size = 0, kf = 1, cl = 0;
     int st = fd < 0 ? fd : 0;
     uint32_t lr = 0, cut = 0;
-    if (fd >= 0) {
-        if (t == 0) S.fsize = fstat(fd);
-        __syncthreads();
-        size = gfs::uni((uint64_t)S.fsize);
-        kf = single ? 1 : size / min_part;
-        kf = kf < 1 ? 1 : (kf > parts ? parts : kf);
+    // Part 0 decides the split.  A file that is not cached (its last page is
+    // not) is read by part 0 alone: one workgroup streams a cold file best
+    // (its sequential read-ahead keeps a batch in flight while it scans; cold
+    // reads measured slower the more parts share a file).  A cached file gets
+    // K_f = min(parts, size / min_part) parts, and part 0 walks the chain first,
+    // overlapped with the other parts' scans.  The other parts wait for the
+    // decision: part 0 has the lower workgroup index, so it was dispatched
+    // first, and it publishes a decision on every path (bounded wait).
+    bool warm = true;
+    if (t == 0) {
+        uint32_t d = 0;
+        if (fd >= 0) S.fsize = fstat(fd);
+        if (single || j == 0) {
+            uint64_t k = 1;
+            if (fd >= 0 && !single) {
+                const uint64_t sz = S.fsize;
+                warm = sz == 0 || gfs::page_cached(gfs_g, fd, (sz - 1) >> gfs_g->log_page);
+                k = warm ? sz / min_part : 1;
+                k = k < 1 ? 1 : (k > parts ? parts : k);
+            } else if (fd >= 0) {
+                warm = S.fsize == 0 || gfs::page_cached(gfs_g, fd, (S.fsize - 1) >> gfs_g->log_page);
+            }
+            d = (uint32_t)k + 1;
+            if (!single) __hip_atomic_store(&fs->kf, d, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const uint64_t w0 = gfs::now_ticks();
+            while ((d = __hip_atomic_load(&fs->kf, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) == 0 &&
+                   gfs::now_ticks() - w0 < gfs_g->wait_ticks)
+                __builtin_amdgcn_s_sleep(8);
+            // no decision within the wait limit: this part scans nothing and
+            // the file reports -ETIMEDOUT
+        }
+        fin_flag = d | (warm ? 0x80000000u : 0u);
     }
+    __syncthreads();
+    {
+        const uint32_t d = gfs::uni(fin_flag);
+        warm = (d >> 31) != 0;
+        kf = (d & 0x7FFFFFFFu) ? (d & 0x7FFFFFFFu) - 1 : 0;
+        if ((d & 0x7FFFFFFFu) == 0 && !st) st = -ETIMEDOUT;
+        if (fd >= 0) size = gfs::uni((uint64_t)S.fsize);
+    }
+    __syncthreads();
     on_size((uint32_t)kf);
     if (fd >= 0) {
-        if (j == 0 && size) {
-            // the reference's window chain, while the other parts scan
+        if (j == 0 && size && warm) {
             st = grep_chain_last(S, fd, size, chunk, &cl, &cut);
             if (t == 0 && !single) {
                 fs->c_last = cl;
@@ -675,6 +712,13 @@ __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t n
                     if (v) atomicAdd(&row[q], v);
                 }
                 __threadfence();
+            }
+        }
+        if (j == 0 && size && !st && !warm) {
+            st = grep_chain_last(S, fd, size, chunk, &cl, &cut);
+            if (t == 0 && !single) {
+                fs->c_last = cl;
+                fs->cut = cut;
             }
         }
     }

@@ -23,13 +23,18 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
-// Workgroup b scans part b % parts of file b / parts (grep_file_split): a
-// launch over fewer files than compute units still fills the device.
-__global__ __launch_bounds__(1024) void k_grep(const char* names, uint32_t stride, const char* dict, uint32_t nw,
-                                               uint32_t chunk, uint32_t* counts_out, int32_t* status, uint32_t parts,
-                                               uint64_t min_part, grep_split_state* fstate) {
+// Workgroup b scans part b / n_files of file b % n_files (grep_file_split): a
+// launch over fewer files than compute units still fills the device.  Part 0
+// of every file comes first in dispatch order (it decides the file's split
+// and the other parts wait for it), and consecutive files go to different
+// XCDs (workgroups are dealt to the 8 XCDs round robin), so the files a cold
+// launch reads with part 0 alone spread over the whole device.
+__global__ __launch_bounds__(1024) void k_grep(const char* names, uint32_t stride, uint32_t n_files,
+                                               const char* dict, uint32_t nw, uint32_t chunk, uint32_t* counts_out,
+                                               int32_t* status, uint32_t parts, uint64_t min_part,
+                                               grep_split_state* fstate) {
     __shared__ __attribute__((aligned(16))) grep_lds S;
-    const uint32_t f = blockIdx.x / parts, j = blockIdx.x % parts;
+    const uint32_t f = blockIdx.x % n_files, j = blockIdx.x / n_files;
     grep_build_dict(S, dict, nw);
     [[clang::always_inline]] grep_file_split(S, names + (size_t)f * stride, nw, chunk, counts_out + (size_t)f * nw,
                                              status + f, j, parts, min_part, fstate ? fstate + f : nullptr, parts,
@@ -157,7 +162,7 @@ __global__ void k_zero_u32(unsigned int* p, uint32_t n) {
 __global__ void k_grep_clear(uint32_t* counts, uint64_t n_counts, grep_split_state* fs, uint32_t n_files) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_counts) counts[i] = 0;
-    if (i < n_files) fs[i] = grep_split_state{0, 0, 0, 0, 0};
+    if (i < n_files) fs[i] = grep_split_state{0, 0, 0, 0, 0, 0, 0};
 }
 
 // Parts per file when the caller leaves it to the launcher: enough workgroups
@@ -178,7 +183,7 @@ static int launch_grep_split(gpufs_ctx* ctx, const char* names_dev, uint32_t nam
                              uint32_t threads, hipStream_t s) {
     if (parts == 1) {
         // one workgroup per file: no shared state, nothing to clear first
-        hipLaunchKernelGGL(k_grep, dim3(n_files), dim3(threads), 0, s, names_dev, name_stride, dict_dev, n_words,
+        hipLaunchKernelGGL(k_grep, dim3(n_files), dim3(threads), 0, s, names_dev, name_stride, n_files, dict_dev, n_words,
                            chunk_bytes, counts_dev, status_dev, 1u, min_part, (grep_split_state*)nullptr);
         return hipGetLastError() == hipSuccess ? 0 : -EIO;
     }
@@ -186,7 +191,7 @@ static int launch_grep_split(gpufs_ctx* ctx, const char* names_dev, uint32_t nam
     const uint64_t n_clear = n_counts > n_files ? n_counts : n_files;
     hipLaunchKernelGGL(k_grep_clear, dim3((uint32_t)((n_clear + 255) / 256)), dim3(256), 0, s, counts_dev, n_counts,
                        fs, n_files);
-    hipLaunchKernelGGL(k_grep, dim3(n_files * parts), dim3(threads), 0, s, names_dev, name_stride, dict_dev, n_words,
+    hipLaunchKernelGGL(k_grep, dim3(n_files * parts), dim3(threads), 0, s, names_dev, name_stride, n_files, dict_dev, n_words,
                        chunk_bytes, counts_dev, status_dev, parts, min_part, fs);
     return hipGetLastError() == hipSuccess ? 0 : -EIO;
 }
