@@ -32,7 +32,7 @@ STATS_FIELDS = [
     "page_hits", "page_misses", "page_waits", "evictions", "alloc_fail", "rpc_reads", "rpc_writes",
     "rpc_opens", "rpc_closes", "bytes_filled", "bytes_written", "nodes_used", "files_used",
     "host_requests", "host_bytes_read", "host_bytes_written", "frames", "frames_valid", "timeouts",
-    "host_busy_ns", "vm_errors", "slots_lost",
+    "host_busy_ns", "vm_errors", "slots_lost", "files_reclaimed", "nodes_recycled",
 ]
 
 

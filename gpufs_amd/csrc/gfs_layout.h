@@ -56,6 +56,7 @@
 #define GFS_FS_OPEN    2u
 #define GFS_FS_CLOSING 3u
 #define GFS_FS_CLOSED  4u
+#define GFS_FS_FREE    5u   // reclaimed: on the free list, in no name-index slot
 
 // ---- tuning switches
 #define GFS_TUNE_PAGE_SYNC 1u   // gread: workgroup barrier after each page copied to the caller
@@ -111,6 +112,8 @@ typedef struct gfs_counters {
     unsigned long long timeouts;
     unsigned long long vm_errors;
     unsigned long long slots_lost;   // ring slots quarantined after a timed-out request
+    unsigned long long files_reclaimed;  // closed file-table entries taken back for other names
+    unsigned long long nodes_recycled;   // radix nodes returned to the free list
 } gfs_counters;
 
 typedef struct gfs_dev {
@@ -149,6 +152,14 @@ typedef struct gfs_dev {
     uint64_t wait_ticks;             // bounded waits: wall-clock ticks before -ETIMEDOUT
     uint32_t tune_flags;             // GFS_TUNE_* (GPUFS_TUNE environment variable at init)
     uint32_t pad1;
+    // reclamation (ftable.h)
+    uint32_t* file_free;             // max_files entries: [free_taken, free_n) are FREE entries
+    unsigned long long free_taken;   // entries taken from file_free (may pass free_n)
+    uint32_t free_n;
+    uint32_t reclaim_hand;           // next entry a reclaim pass looks at
+    uint32_t open_gate;              // name-index users in flight [0,31) | exclusive bit 31
+    uint32_t pad2;
+    unsigned long long node_free;    // free radix nodes: stack head, tag [40,64) | node [0,40)
     gfs_counters ctr;
 } gfs_dev;
 

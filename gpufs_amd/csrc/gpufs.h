@@ -42,6 +42,7 @@
 #include <stdint.h>
 #include "gfs_layout.h"
 #include "fstate.h"
+#include "ftable.h"
 #include "../../include/gpufs_amd.h"
 
 typedef unsigned char uchar;
@@ -205,6 +206,22 @@ __device__ __forceinline__ void fence_acq_sys() { __builtin_amdgcn_fence(__ATOMI
 __device__ __forceinline__ void fence_rel_sys() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, ""); }
 __device__ __forceinline__ void backoff() { __builtin_amdgcn_s_sleep(16); }
 __device__ __forceinline__ uint64_t now_ticks() { return wall_clock64(); }
+
+// ftable.h policy: agent-scope atomics, the context's wait limit.
+struct ft_dev {
+    uint64_t t0, limit;
+    template <class T> __device__ T load(T* x) { return ld_acq(x); }
+    template <class T> __device__ T load_rlx(T* x) { return ld_rlx(x); }
+    template <class T> __device__ void store(T* x, typename ident<T>::type v) { st_rel(x, v); }
+    template <class T> __device__ void store_rlx(T* x, typename ident<T>::type v) { st_rlx(x, v); }
+    template <class T> __device__ T fadd(T* x, typename ident<T>::type v) { return gfs::fadd(x, v); }
+    template <class T> __device__ bool cas(T* x, T& e, typename ident<T>::type v) { return gfs::cas(x, e, v); }
+    __device__ uint32_t fand(uint32_t* x, uint32_t v) {
+        return __hip_atomic_fetch_and(x, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ bool expired() { return now_ticks() - t0 > limit; }
+    __device__ void pause() { __builtin_amdgcn_s_sleep(16); }
+};
 // Phase hook for timing studies: expands to nothing unless a build defines it.
 #ifndef GFS_TRACE
 #define GFS_TRACE(id) ((void)0)
@@ -369,10 +386,19 @@ __device__ inline void wg_copy(uint8_t* dst, const uint8_t* src, uint64_t len) {
 __device__ __forceinline__ uint64_t tree_cap(uint32_t h) {
     return h >= GFS_MAX_HEIGHT ? ~0ull : (1ull << (GFS_FANOUT_LOG * h));
 }
+// A zeroed node: from the never-used range, else from the free list that
+// reclaimed files feed (ftable.h; out of line, it is rare).  0 when neither
+// has one.
+__device__ __noinline__ uint64_t node_pop_free(gfs_dev* g) {
+    ft_dev P{0, 0};
+    return ft_node_pop(g, P);
+}
 __device__ inline uint64_t node_alloc(gfs_dev* g) {
-    unsigned long long n = fadd_rlx(&g->node_next, 1ull);
-    if (n >= g->node_cap) return 0;
-    return n;
+    if (ld_rlx(&g->node_next) < g->node_cap) {
+        unsigned long long n = fadd_rlx(&g->node_next, 1ull);
+        if (n < g->node_cap) return n;
+    }
+    return node_pop_free(g);
 }
 
 // Returns the node index of the leaf covering pages [leaf*64, leaf*64+64), creating
@@ -697,11 +723,8 @@ __device__ __forceinline__ bool fd_valid(gfs_dev* g, int fd) {
     return fd >= 0 && (unsigned long long)fd < ld_rlx(&g->files_next) && (uint32_t)fd < g->max_files;
 }
 // A path as GFS_NAME_MAX/8 little-endian words, zero from the terminator on.
-struct gname {
-    uint64_t w[GFS_NAME_MAX / 8];
-    int len;          // strlen, or GFS_NAME_MAX when no terminator in the first GFS_NAME_MAX bytes
-    uint32_t hash;
-};
+// (len: strlen, or GFS_NAME_MAX when no terminator in the first GFS_NAME_MAX bytes)
+typedef ft_name gname;
 __device__ __forceinline__ uint64_t zero_bytes(uint64_t v) {
     // high bit of each zero byte; the lowest flagged byte is always a zero byte
     return (v - 0x0101010101010101ull) & ~v & 0x8080808080808080ull;
@@ -753,14 +776,6 @@ __device__ inline void name_load(const char* s, gname& n) {
     n.len = len;
     n.hash = (uint32_t)(h ^ (h >> 32));
 }
-// Entry names are stored zero-padded, so equality is sixteen word compares.
-__device__ inline bool name_eq(const gfs_file* e, const gname& n) {
-    const uint64_t* w = (const uint64_t*)e->name;
-    uint64_t d = 0;
-#pragma unroll
-    for (int i = 0; i < GFS_NAME_MAX / 8; ++i) d |= w[i] ^ n.w[i];
-    return d == 0;
-}
 
 // fstate.h policy: agent-scope atomics, the context's wait limit.
 struct fst_dev {
@@ -783,6 +798,17 @@ struct fst_dev {
     }
 };
 
+// The rare paths of the file table, kept out of line so that the open path
+// inlined into every kernel stays small.
+__device__ __noinline__ void tree_drop(gfs_dev* g, uint64_t rw) {
+    ft_dev F{0, 0};
+    ft_tree_free(g, rw, F);
+}
+__device__ __noinline__ int reclaim_files(gfs_dev* g, uint64_t t0) {
+    ft_dev F{t0, g->wait_ticks};
+    return ft_reclaim(g, F);
+}
+
 // Host open of entry e (new or reopened), whose state is OPENING.  Fills the
 // host fields and moves the state to OPEN, or to CLOSED dropping this
 // caller's open on failure.
@@ -804,9 +830,12 @@ __device__ inline int open_host(gfs_dev* g, gfs_file* e, int flags, bool reopen)
     }
     bool changed = false;
     if (reopen && (e->ctime_ns != ct || e->mtime_ns != mt || e->size != size || e->ino != ino)) {
-        // the file changed on the host: drop every cached page of the old version
+        // the file changed on the host: drop every cached page of the old
+        // version (the entry was CLOSED, so nobody walks its old tree)
         changed = true;
+        const uint64_t rw = ld_rlx(&e->root);
         st_rlx(&e->root, 0ull);
+        tree_drop(g, rw);
     }
     e->host_fd = (int32_t)r;
     e->flags = flags;
@@ -820,64 +849,43 @@ __device__ inline int open_host(gfs_dev* g, gfs_file* e, int flags, bool reopen)
 }
 
 // Single-thread open (reference: single_thread_open, libgpufs/fs_calls.cu:152-225).
-__device__ inline int open_single(const char* name, int flags) {
+// The name is looked up (or inserted) and joined inside the open gate, so no
+// reclaim pass (ftable.h) can take the entry back in between; a full table
+// runs one and looks again.
+__device__ __noinline__ int open_single(const char* name, int flags) {
     gfs_dev* g = gfs_g;
     gname nm;
     name_load(name, nm);
     if (nm.len == 0) return -ENOENT;
     if (nm.len >= GFS_NAME_MAX) return -ENAMETOOLONG;
-    const uint32_t h = nm.hash;
-    const uint32_t mask = g->name_cap - 1;
     const uint64_t t0 = now_ticks();
-    uint32_t pos = h & mask;
-    int idx = -1;
-    for (uint32_t probe = 0; probe <= mask; ++probe, pos = (pos + 1) & mask) {
-        uint32_t v = ld_acq(&g->name_index[pos]);
-        if (v == 0) {
-            // reserve the slot first, so only the workgroup that publishes
-            // the name takes a file-table entry
-            uint32_t exp = 0;
-            if (cas(&g->name_index[pos], exp, GFS_NAME_RESERVED)) {
-                const unsigned long long m = fadd_rlx(&g->files_next, 1ull);
-                if (m >= g->max_files) {
-                    st_rel(&g->name_index[pos], 0u);
-                    return E_FSTABLE_FULL;
-                }
-                gfs_file* e = &g->files[m];
-                uint64_t* en = (uint64_t*)e->name;
-                for (int i = 0; i < GFS_NAME_MAX / 8; ++i) en[i] = nm.w[i];
-                e->name_hash = h;
-                e->root = 0;
-                e->host_fd = -1;
-                st_rlx(&e->st, fst_make(GFS_FS_OPENING, 1, 0));
-                st_rel(&g->name_index[pos], (uint32_t)m + 1u);   // publish the entry
-                int r = open_host(g, e, flags, false);
-                return r < 0 ? E_IPC_OPEN_ERROR : (int)m;
-            }
-            v = exp;
+    ft_dev F{t0, g->wait_ticks};
+    for (;;) {
+        int r = ft_gate_enter(g, F);
+        if (r) return r;
+        uint32_t idx = 0;
+        r = ft_find_or_insert(g, nm, F, &idx);
+        if (r == FT_NEW) {
+            ft_gate_leave(g, F);
+            return open_host(g, &g->files[idx], flags, false) < 0 ? E_IPC_OPEN_ERROR : (int)idx;
         }
-        while (v == GFS_NAME_RESERVED) {   // another workgroup is publishing this slot
-            if (now_ticks() - t0 > g->wait_ticks) return -ETIMEDOUT;
-            backoff();
-            v = ld_acq(&g->name_index[pos]);
+        if (r == FT_FOUND) {
+            fst_dev P{t0, g->wait_ticks};
+            GFS_TRACE(25);
+            const int o = fst_open(&g->files[idx].st, P);
+            GFS_TRACE(26);
+            ft_gate_leave(g, F);
+            if (o == FST_JOINED) return (int)idx;
+            if (o < 0) return o;
+            return open_host(g, &g->files[idx], flags, true) < 0 ? E_IPC_OPEN_ERROR : (int)idx;
         }
-        if (v == 0) {   // the reservation was given up (table full): look at the slot again
-            --probe;
-            pos = (pos - 1) & mask;
-            continue;
-        }
-        gfs_file* e = &g->files[v - 1];
-        if (e->name_hash == h && name_eq(e, nm)) { idx = (int)v - 1; break; }
+        ft_gate_leave(g, F);
+        if (r != E_FSTABLE_FULL) return r;
+        const int c = reclaim_files(g, t0);
+        if (c < 0) return c;
+        if (c == 0) return E_FSTABLE_FULL;
+        if (F.expired()) return -ETIMEDOUT;
     }
-    if (idx < 0) return E_FSTABLE_FULL;
-    gfs_file* e = &g->files[idx];
-    fst_dev P{t0, g->wait_ticks};
-    GFS_TRACE(25);
-    const int r = fst_open(&e->st, P);
-    GFS_TRACE(26);
-    if (r == FST_JOINED) return idx;
-    if (r < 0) return r;
-    return open_host(g, e, flags, true) < 0 ? E_IPC_OPEN_ERROR : idx;
 }
 
 } // namespace gfs

@@ -391,6 +391,7 @@ struct grep_text_args {
     uint8_t* scratch;            // per workgroup: words | list | name | lines
     uint64_t stride, off_list, off_name, off_lines, list_cap;
     unsigned long long* out_off;
+    uint32_t* trunc;             // 0, 1 while the output is truncated, 2 done, 3 failed
     int32_t* status;
 };
 
@@ -430,6 +431,28 @@ __global__ __launch_bounds__(1024) void k_grep_text(grep_text_args a) {
     const int fd_o = gopen(a.out, O_GWRONCE);
     const int fd_src = gopen(a.src, O_GRDONLY);
     if (fd_dbs < 0 || fd_o < 0 || fd_src < 0) st = fd_dbs < 0 ? fd_dbs : (fd_o < 0 ? fd_o : fd_src);
+    if (st == 0) {
+        // The output starts empty: the first workgroup here truncates it from
+        // one thread, as the reference does (grep_text_gpu.cu:258-268), and the
+        // others wait for that before anything is written (bounded wait).
+        if (t == 0) {
+            int r = 0;
+            if (atomicCAS(a.trunc, 0u, 1u) == 0u) {
+                r = single_thread_ftruncate(fd_o, 0);
+                __hip_atomic_store(a.trunc, r ? 3u : 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                const uint64_t w0 = gfs::now_ticks();
+                uint32_t v;
+                while ((v = __hip_atomic_load(a.trunc, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) == 1u &&
+                       gfs::now_ticks() - w0 < gfs_g->wait_ticks)
+                    __builtin_amdgcn_s_sleep(8);
+                r = v == 2u ? 0 : (v == 1u ? -ETIMEDOUT : -EIO);
+            }
+            wst = r;
+        }
+        __syncthreads();
+        st = (int)gfs::uni((uint32_t)wst);
+    }
     if (st == 0 && nw > 0) {
         size_t got = gread(fd_src, first * 32, (size_t)nw * 32, (uchar*)words);
         if (got != (size_t)nw * 32) st = (int64_t)got < 0 ? (int)(int64_t)got : -EIO;
@@ -532,10 +555,6 @@ extern "C" int gpufs_grep_text(gpufs_ctx* ctx, const char* src_path, const char*
     uint64_t wpc = total / grid;
     if (wpc == 0) wpc = 1;
     if (grid > total) grid = total;   // every workgroup has a record
-    // the output starts empty (the reference truncates it from the GPU once)
-    int ofd = open(out_path, O_WRONLY | O_CREAT | O_TRUNC | O_CLOEXEC, 0644);
-    if (ofd < 0) return -errno;
-    close(ofd);
     int v = gpufs_bind(ctx);
     if (v) return v;
     const uint64_t words_b = (uint64_t)GREP_MAX_WORDS * 32;
@@ -547,21 +566,29 @@ extern "C" int gpufs_grep_text(gpufs_ctx* ctx, const char* src_path, const char*
     uint8_t* scratch = nullptr;
     char* names = nullptr;
     unsigned long long* off = nullptr;
+    // names | output offset (8 B) | truncation flag (8 B), zeroed by the copy
+    char hn[3 * GFS_NAME_MAX + 16];
+    auto release = [&](int r) {
+        if (scratch) (void)hipFreeAsync(scratch, s);
+        if (names) (void)hipFreeAsync(names, s);
+        return r;
+    };
     if (hipMallocAsync((void**)&scratch, stride * grid, s) != hipSuccess) return -ENOMEM;
-    if (hipMallocAsync((void**)&names, 3 * GFS_NAME_MAX + 8, s) != hipSuccess) return -ENOMEM;
+    if (hipMallocAsync((void**)&names, sizeof(hn), s) != hipSuccess) {
+        names = nullptr;
+        return release(-ENOMEM);
+    }
     off = (unsigned long long*)(names + 3 * GFS_NAME_MAX);
-    char hn[3 * GFS_NAME_MAX + 8];
     memset(hn, 0, sizeof(hn));
     memcpy(hn, src_path, ln);
     memcpy(hn + GFS_NAME_MAX, out_path, lo);
     memcpy(hn + 2 * GFS_NAME_MAX, dbs_path, ld);
-    if (hipMemcpyAsync(names, hn, sizeof(hn), hipMemcpyHostToDevice, s) != hipSuccess) return -EIO;
-    if (hipStreamSynchronize(s) != hipSuccess) return -EIO;   // hn is a stack buffer
+    if (hipMemcpyAsync(names, hn, sizeof(hn), hipMemcpyHostToDevice, s) != hipSuccess) return release(-EIO);
+    if (hipStreamSynchronize(s) != hipSuccess) return release(-EIO);   // hn is a stack buffer
     grep_text_args a{names, names + GFS_NAME_MAX, names + 2 * GFS_NAME_MAX, total, (uint32_t)wpc, chunk_bytes,
-                     scratch, stride, off_list, off_name, off_lines, list_cap, off, status_dev};
+                     scratch, stride, off_list, off_name, off_lines, list_cap, off,
+                     (uint32_t*)(names + 3 * GFS_NAME_MAX + 8), status_dev};
     hipLaunchKernelGGL(k_grep_text, dim3((uint32_t)grid), dim3(threads), 0, s, a);
-    if (hipGetLastError() != hipSuccess) return -EIO;
-    (void)hipFreeAsync(scratch, s);
-    (void)hipFreeAsync(names, s);
-    return (int)grid;
+    const int lr = hipGetLastError() == hipSuccess ? (int)grid : -EIO;
+    return release(lr);
 }

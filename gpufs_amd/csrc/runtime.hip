@@ -21,6 +21,7 @@
 #include <vector>
 #include "../../include/gpufs_amd.h"
 #include "gfs_layout.h"
+#include "fstate.h"
 #include "host_service.h"
 
 struct gpufs_ctx {
@@ -88,6 +89,7 @@ static void free_all(gpufs_ctx* c) {
     (void)hipFree(h.nodes);
     (void)hipFree(h.files);
     (void)hipFree(h.name_index);
+    (void)hipFree(h.file_free);
     (void)hipFree(h.slot_free);
     (void)hipHostFree(h.slots);
     (void)hipHostFree(h.sq);
@@ -190,6 +192,7 @@ extern "C" int gpufs_init(const gpufs_config* cfg_in, gpufs_ctx** out) {
     TRY_ALLOC(hipMalloc(&h.nodes, (size_t)h.node_cap * GFS_FANOUT * sizeof(uint64_t)));
     TRY_ALLOC(hipMalloc(&h.files, sizeof(gfs_file) * (size_t)h.max_files));
     TRY_ALLOC(hipMalloc(&h.name_index, sizeof(uint32_t) * (size_t)h.name_cap));
+    TRY_ALLOC(hipMalloc(&h.file_free, sizeof(uint32_t) * (size_t)h.max_files));
     TRY_ALLOC(hipMalloc(&h.slot_free, sizeof(uint64_t) * h.slot_words * GFS_SLOT_WORD_STRIDE));
     const unsigned hflags = hipHostMallocMapped | hipHostMallocCoherent;
     TRY_ALLOC(hipHostMalloc((void**)&h.slots, sizeof(gfs_rpc_slot) * h.ring_depth, hflags));
@@ -214,6 +217,11 @@ extern "C" int gpufs_init(const gpufs_config* cfg_in, gpufs_ctx** out) {
     h.node_next = 1;
     h.files_next = 0;
     h.sq_tail = 0;
+    h.free_taken = 0;
+    h.free_n = 0;
+    h.reclaim_hand = 0;
+    h.open_gate = 0;
+    h.node_free = 0;
     int wall_khz = 100000;
     if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || wall_khz <= 0)
         wall_khz = 100000;
@@ -322,6 +330,7 @@ extern "C" int gpufs_cache_drop(gpufs_ctx* c) {
     cur.fresh_next = 0;
     cur.clock_hand = 0;
     cur.node_next = 1;
+    cur.node_free = 0;   // the nodes were zeroed: no tree and no free list remains
     HIP_TRY(hipMemcpyAsync(c->dev, &cur, sizeof(gfs_dev), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
@@ -345,8 +354,12 @@ extern "C" int gpufs_stats_get(gpufs_ctx* c, gpufs_stats* o) {
     o->rpc_closes = cur.ctr.rpc_closes;
     o->bytes_filled = cur.ctr.bytes_filled;
     o->bytes_written = cur.ctr.bytes_written;
-    o->nodes_used = cur.node_next - 1;
-    o->files_used = cur.files_next;
+    o->nodes_used = std::min<unsigned long long>(cur.node_next, c->host.node_cap) - 1;
+    {
+        const uint64_t nf = std::min<unsigned long long>(cur.files_next, c->host.max_files);
+        const uint64_t left = cur.free_n > cur.free_taken ? cur.free_n - cur.free_taken : 0;
+        o->files_used = nf - left;
+    }
     auto& hc = c->service.counters();
     o->host_requests = hc.requests.load();
     o->host_bytes_read = hc.bytes_read.load();
@@ -361,6 +374,8 @@ extern "C" int gpufs_stats_get(gpufs_ctx* c, gpufs_stats* o) {
     o->host_busy_ns = hc.busy_ns.load();
     o->vm_errors = cur.ctr.vm_errors;
     o->slots_lost = cur.ctr.slots_lost;
+    o->files_reclaimed = cur.ctr.files_reclaimed;
+    o->nodes_recycled = cur.ctr.nodes_recycled;
     return 0;
 }
 
@@ -406,6 +421,7 @@ extern "C" int gpufs_check_invariants(gpufs_ctx* c) {
     for (uint32_t f = 0; f < h.n_frames; ++f)
         if (((meta[f] >> GFS_ST_SHIFT) & 0xF) == GFS_F_FILLING) return -1002;   // a fill never finished
     std::vector<uint8_t> seen(h.n_frames, 0);
+    std::vector<uint8_t> nseen(nn, 0);   // nodes in a tree or on the free list
     for (uint32_t fi = 0; fi < nf; ++fi) {
         uint64_t rw = files[fi].root;
         if (!rw) continue;
@@ -418,6 +434,8 @@ extern "C" int gpufs_check_invariants(gpufs_ctx* c) {
         while (!stack.empty()) {
             It it = stack.back();
             stack.pop_back();
+            if (nseen[it.node]) return -1013;   // a node in two trees
+            nseen[it.node] = 1;
             for (uint32_t k = 0; k < GFS_FANOUT; ++k) {
                 uint64_t v = nodes[it.node * GFS_FANOUT + k];
                 if (!v) continue;
@@ -440,6 +458,63 @@ extern "C" int gpufs_check_invariants(gpufs_ctx* c) {
             }
         }
     }
+    // radix-node free list: zeroed nodes, each once, in no tree
+    {
+        uint64_t n = cur.node_free & ((1ull << 40) - 1);
+        for (uint64_t steps = 0; n; ++steps) {
+            if (n >= nn || steps >= nn || nseen[n]) return -1013;
+            nseen[n] = 1;
+            for (uint32_t k = 1; k < GFS_FANOUT; ++k)
+                if (nodes[n * GFS_FANOUT + k]) return -1013;
+            n = nodes[n * GFS_FANOUT] & ((1ull << 40) - 1);
+        }
+    }
+    // file table and name index: every entry in use is found by probing from
+    // its home slot, free entries are in no slot, and nobody is inside the gate
+    {
+        if (cur.open_gate != 0) return -1011;
+        std::vector<uint32_t> index(h.name_cap);
+        HIP_TRY(hipMemcpy(index.data(), h.name_index, 4ull * h.name_cap, hipMemcpyDeviceToHost));
+        const uint32_t mask = h.name_cap - 1;
+        uint64_t in_index = 0, live = 0;
+        for (uint32_t i = 0; i < h.name_cap; ++i) {
+            const uint32_t v = index[i];
+            if (!v) continue;
+            if (v == GFS_NAME_RESERVED || v > nf) return -1012;
+            if (fst_state(files[v - 1].st) == GFS_FS_FREE) return -1012;
+            ++in_index;
+        }
+        for (uint32_t fi = 0; fi < nf; ++fi) {
+            const uint32_t st = fst_state(files[fi].st);
+            if (st == GFS_FS_FREE) {
+                if (files[fi].root) return -1012;
+                continue;
+            }
+            if (st == GFS_FS_EMPTY) continue;
+            ++live;
+            uint32_t pos = files[fi].name_hash & mask;
+            bool found = false;
+            for (uint32_t probe = 0; probe <= mask; ++probe, pos = (pos + 1) & mask) {
+                if (index[pos] == fi + 1) {
+                    found = true;
+                    break;
+                }
+                if (index[pos] == 0) break;
+            }
+            if (!found) return -1012;
+        }
+        if (live != in_index) return -1012;
+        const uint32_t fn = cur.free_n;
+        if (fn > h.max_files) return -1012;
+        std::vector<uint32_t> fl(fn);
+        if (fn) HIP_TRY(hipMemcpy(fl.data(), h.file_free, 4ull * fn, hipMemcpyDeviceToHost));
+        std::vector<uint8_t> listed(nf, 0);
+        for (uint64_t i = std::min<uint64_t>(cur.free_taken, fn); i < fn; ++i) {
+            const uint32_t m = fl[i];
+            if (m >= nf || listed[m] || fst_state(files[m].st) != GFS_FS_FREE) return -1012;
+            listed[m] = 1;
+        }
+    }
     return 0;
 }
 
@@ -458,6 +533,9 @@ extern "C" const char* gpufs_strerror(int code) {
     case -1008: return "invariant: frame shared by two leaves";
     case -1009: return "invariant: leaf left reserved";
     case -1010: return "invariant: ring slot not returned";
+    case -1011: return "invariant: open gate not clear";
+    case -1012: return "invariant: file table and name index disagree";
+    case -1013: return "invariant: radix node lost, shared or on the free list twice";
     default: break;
     }
     if (code < 0 && code > -4096) return strerror(-code);

@@ -61,8 +61,8 @@ typedef struct gpufs_stats {
     uint64_t rpc_closes;     /* CLOSE requests */
     uint64_t bytes_filled;   /* bytes copied from pinned staging into frames */
     uint64_t bytes_written;  /* bytes copied from frames into staging for write-back */
-    uint64_t nodes_used;     /* radix-tree nodes allocated */
-    uint64_t files_used;     /* file-table entries in use */
+    uint64_t nodes_used;     /* radix-tree nodes taken from the never-used range (high-water mark) */
+    uint64_t files_used;     /* file-table entries in use (not free) */
     uint64_t host_requests;  /* requests served by the host service */
     uint64_t host_bytes_read;
     uint64_t host_bytes_written;
@@ -73,6 +73,10 @@ typedef struct gpufs_stats {
     uint64_t vm_errors;      /* FatPointer dereferences whose page could not be mapped */
     uint64_t slots_lost;     /* ring slots kept out of the ring after a request timed out (the host
                                 may still answer it); the ring is that much smaller until destroy */
+    uint64_t files_reclaimed;  /* closed entries taken back for other names when the table was full
+                                  (libgpufs/fs_structures.cu:364-369) */
+    uint64_t nodes_recycled;   /* radix nodes returned to the free list (trees of reclaimed files and
+                                  of files found changed on reopen) */
 } gpufs_stats;
 
 /* ---- lifecycle: replaces initializer(GPUGlobals_ptr*) (libgpufs/fs_initializer.cu:116)
