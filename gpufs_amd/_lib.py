@@ -61,6 +61,7 @@ SIGNATURES = {
     "gpufs_run_handler": (I, [P]),
     "gpufs_host_start": (I, [P]),
     "gpufs_host_stop": (I, [P]),
+    "gpufs_host_kick": (I, [P]),
     "gpufs_cache_drop": (I, [P]),
     "gpufs_stats_get": (I, [P, ctypes.POINTER(Stats)]),
     "gpufs_stats_reset": (I, [P]),

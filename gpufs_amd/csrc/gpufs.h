@@ -2359,6 +2359,7 @@ __device__ inline int gmsync(volatile void* addr, size_t length, int flags) {
 static inline int gpufs_bind(gpufs_ctx* ctx) {
     static void* bound = nullptr;
     static int bound_dev = -1;
+    (void)gpufs_host_kick(ctx);   // a launch follows: have the host service awake for it
     void* p = gpufs_device_state(ctx);
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return -ENODEV;

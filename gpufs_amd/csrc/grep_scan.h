@@ -184,6 +184,19 @@ __device__ inline uint64_t grep_seg_mask(const uint8_t* corpus, uint32_t base, u
     return am;
 }
 
+// Length of the token that starts at bit b of the segment mask `am`; a token
+// that runs past the segment takes the rest of its length from the next
+// segment's mask `nx`.
+__device__ __forceinline__ uint32_t grep_tok_len(uint64_t am, uint64_t nx, uint32_t b) {
+    const uint64_t rest = ~(am >> b);
+    uint32_t len = rest ? (uint32_t)__ffsll((unsigned long long)rest) - 1 : 64;
+    if (b + len >= 64) {
+        const uint64_t nr = ~nx;
+        len = (64 - b) + (nr ? (uint32_t)__ffsll((unsigned long long)nr) - 1 : 64);
+    }
+    return len;
+}
+
 // Count tokens of window [0, dsz) of S.corpus.  Bytes from dsz on count as
 // separators, so every token ends inside the window.
 // Each lane classifies one 64-byte segment and walks its tokens.  The loop is
@@ -193,10 +206,41 @@ __device__ inline uint64_t grep_seg_mask(const uint8_t* corpus, uint32_t base, u
 // the byte before a segment comes from the previous lane's mask.  A byte-wise
 // LDS walk there (one load and wait per byte, taken by almost every lane in the
 // same iteration) cost a third of the scan.
+// A lane takes GREP_UNROLL tokens per iteration: their LDS reads (first eight
+// bytes, then the filter word) are independent and issue together, so one
+// iteration waits for LDS about as long as a one-token iteration did.
 // The window's first `skip` bytes (< 64) are not part of it: they are masked
 // out as if they were separators.
-__device__ inline void grep_scan_window(grep_lds& S, uint32_t dsz, uint32_t skip = 0) {
+#ifndef GREP_UNROLL
+#define GREP_UNROLL 2
+#endif
+// Optionally, the next window is loaded meanwhile: `nunits` 16-byte units
+// from file offset `wbase` (16-aligned), read straight from frames pinned by
+// pin_pages (page pb + i in gfs_ctl.pinf[i]) into registers before the scan
+// and stored into S.corpus after it (GREP_PF units per thread at most).
+#define GREP_PF 5
+typedef __attribute__((address_space(3))) grep_lds grep_lds_as3;
+__device__ __noinline__ void grep_scan_window_lds(grep_lds_as3* SP, uint32_t dsz, uint32_t skip, uint64_t wbase,
+                                                  uint64_t pb, uint32_t nunits) {
+    grep_lds& S = *(grep_lds*)SP;
+    constexpr int U = GREP_UNROLL;
     const uint32_t T = gfs::nthreads(), t = gfs::tid(), ln = gfs::lane();
+    gfs::v4u pf[GREP_PF];
+    if (nunits) {
+        const gfs_dev* g = gfs_g;
+        const uint8_t* pool = g->pool;
+        const uint32_t lp = g->log_page;
+        const uint64_t pm = (1ull << lp) - 1;
+#pragma unroll
+        for (int i = 0; i < GREP_PF; ++i) {
+            const uint32_t u = t + T * (uint32_t)i;
+            if (u < nunits) {
+                const uint64_t pos = wbase + 16ull * u;
+                const uint32_t fr = gfs_ctl.pinf[(pos >> lp) - pb];
+                pf[i] = *(const gfs::v4u*)(pool + ((uint64_t)fr << lp) + (pos & pm));
+            }
+        }
+    }
     const uint32_t nseg = (dsz + 63) >> 6;
     const uint32_t lm = S.lenmask;
     for (uint32_t sg0 = t - ln; sg0 < nseg; sg0 += T) {
@@ -211,26 +255,60 @@ __device__ inline void grep_scan_window(grep_lds& S, uint32_t dsz, uint32_t skip
         if (ln == 0) pv = (base > 0 && g_alpha(S.corpus[base - 1])) ? 1ull : 0ull;
         uint64_t starts = am & ~((am << 1) | pv);
         while (starts) {
-            const uint32_t b = (uint32_t)__ffsll((unsigned long long)starts) - 1;
-            starts &= starts - 1;
-            const uint32_t p = base + b;
-            const uint64_t rest = ~(am >> b);
-            uint32_t len = rest ? (uint32_t)__ffsll((unsigned long long)rest) - 1 : 64;
-            if (b + len >= 64) {   // the token runs past the segment: its length continues in the next mask
-                const uint64_t nr = ~nx;
-                len = (64 - b) + (nr ? (uint32_t)__ffsll((unsigned long long)nr) - 1 : 64);
+            uint32_t p[U], len[U], h[U];
+            uint64_t t0[U];
+            uint32_t okb = 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                // a lane with fewer tokens left takes bit 0 again, not counted
+                const uint32_t have = starts != 0 ? 1u : 0u;
+                const uint32_t b = (uint32_t)__builtin_ctzll(starts | (1ull << 63)) & (have ? 63u : 0u);
+                starts &= starts - 1;
+                len[u] = grep_tok_len(am, nx, b);
+                p[u] = base + b;
+                // len in [1, 32] and some query word has this length
+                okb |= (have & (uint32_t)(len[u] <= 32) & (lm >> ((len[u] - 1) & 31))) << u;
             }
-            // len in [1, 32] and some query word has this length, tested with one branch
-            const uint32_t ok = (uint32_t)(len <= 32) & (lm >> ((len - 1) & 31));
-            if (!(ok & 1u)) continue;
-            const uint32_t kb = len < 8 ? len : 8u;
-            const uint64_t t0 = lds_u64_nb(S.corpus, p) & (~0ull >> (64 - 8 * kb));
-            const uint32_t h = tok_hash8(t0, len);
-            const uint32_t fi = h >> (32 - GREP_FILT_LOG);
-            if (!((S.filt[fi >> 5] >> (fi & 31)) & 1u)) continue;
-            grep_probe(S, p, len, t0, h);
+            if (!okb) continue;
+            // every token's reads are made whether or not it passed, so that
+            // they issue together instead of one behind the other's branch
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t kb = len[u] < 8 ? len[u] : 8u;
+                t0[u] = lds_u64_nb(S.corpus, p[u]) & (~0ull >> (64 - 8 * kb));
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) h[u] = tok_hash8(t0[u], len[u]);
+            uint32_t hit = 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t fw = S.filt[h[u] >> (32 - GREP_FILT_LOG + 5)];
+                hit |= ((fw >> ((h[u] >> (32 - GREP_FILT_LOG)) & 31)) & 1u) << u;
+            }
+            hit &= okb;
+            if (!hit) continue;
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if ((hit >> u) & 1u) grep_probe(S, p[u], len[u], t0[u], h[u]);
         }
     }
+    if (nunits) {
+        __syncthreads();   // every wave is done with the current window
+        typedef __attribute__((address_space(3))) gfs::v4u lv4;
+        lv4* dst = (lv4*)S.corpus;
+#pragma unroll
+        for (int i = 0; i < GREP_PF; ++i) {
+            const uint32_t u = t + T * (uint32_t)i;
+            if (u < nunits) dst[u] = pf[i];
+        }
+        __syncthreads();
+    }
+}
+// Out of line, with the window's LDS address space kept in the pointer type,
+// so that the scan has the register file to itself instead of sharing it with
+// the caller's read loop (ds_ instructions still, not flat ones).
+__device__ __forceinline__ void grep_scan_window(grep_lds& S, uint32_t dsz, uint32_t skip = 0) {
+    grep_scan_window_lds((grep_lds_as3*)&S, dsz, skip, 0, 0, 0);
 }
 
 // The chunk loop of one open file (grep_text_gpu.cu:298-318): windows of
@@ -278,7 +356,7 @@ __device__ inline int grep_file_windows(grep_lds& S, int fd, uint32_t chunk, OnW
         }
         const uint32_t dsz = mis + (last ? left : chunk + ov);
         cursor += (uint64_t)chunk + ov;
-        [[clang::always_inline]] grep_scan_window(S, dsz, mis);
+        grep_scan_window(S, dsz, mis);
         // A callback that reads the counts waits for every wave's scan.  With
         // none, the next gread's barrier keeps the window until every wave is
         // done with it, and thread 0's round trip for that read overlaps the
@@ -394,7 +472,7 @@ __device__ inline int grep_part_windows(grep_lds& S, int fd, uint32_t chunk, uin
             if (ov >= 32) lr = 1;
             dsz = mis + chunk + ov;
         }
-        [[clang::always_inline]] grep_scan_window(S, dsz, skip);
+        grep_scan_window(S, dsz, skip);
         if (final) break;
         cursor += (uint64_t)chunk + ov;
         // a window whose look-ahead run reached p_end ended where this part
@@ -404,6 +482,98 @@ __device__ inline int grep_part_windows(grep_lds& S, int fd, uint32_t chunk, uin
     __syncthreads();
     const int st = gfs::uni(S.status);
     __syncthreads();
+    *longrun = lr;
+    return st;
+}
+
+// grep_part_windows for a part whose pages are cached: the part's pages are
+// pinned (at most 64 at a time) and each window is loaded from the frames
+// into registers while the previous window is scanned (grep_scan_window_lds),
+// instead of a gread per window between the scans.  Same windows, same
+// results.  Needs a workgroup large enough that GREP_PF 16-byte units per
+// thread cover the largest window; returns 1 (nothing done) otherwise.
+__device__ inline int grep_part_pinned(grep_lds& S, int fd, uint32_t chunk, uint64_t size, uint64_t p_begin,
+                                       uint64_t p_end, uint32_t* longrun) {
+    gfs_dev* g = gfs_g;
+    const uint32_t T = gfs::nthreads();
+    if ((uint64_t)T * GREP_PF * 16 < (uint64_t)chunk + 64 + 16 + 16) return 1;
+    const uint32_t lp = g->log_page;
+    const uint64_t last_page = (size - 1) >> lp;
+    // window at cursor c: bytes [c - mis, rend) land at corpus[0, len)
+    auto window = [&](uint64_t c, bool& final, uint64_t& wbase, uint32_t& len) {
+        final = p_end - c <= chunk;
+        const uint64_t want = final ? p_end + 64 : c + chunk + 32;
+        const uint64_t rend = want < size ? want : size;
+        wbase = c & ~15ull;
+        len = (uint32_t)(rend - wbase);
+    };
+    uint64_t cursor = p_begin, wbase;
+    bool final, first = p_begin > 0;
+    uint32_t len, lr = 0;
+    window(cursor, final, wbase, len);
+    uint64_t pb = wbase >> lp;
+    uint32_t cnt = (uint32_t)(last_page - pb + 1 < 64 ? last_page - pb + 1 : 64);
+    int st = gfs::pin_pages(g, fd, pb, cnt);
+    if (st) {
+        *longrun = 0;
+        return st;
+    }
+    grep_scan_window_lds((grep_lds_as3*)&S, 0, 0, wbase, pb, (len + 15) >> 4);   // load only
+    for (;;) {
+        const uint32_t mis = (uint32_t)(cursor & 15);
+        uint32_t skip = mis;
+        if (first) {
+            // the token that runs into p_begin belongs to the previous part
+            const uint32_t r = lds_word_run(S.corpus, mis, len - mis < 64 ? len - mis : 64);
+            if (r >= 32) lr = 1;
+            else skip = mis + r;
+            first = false;
+        }
+        uint32_t dsz, ov = 0;
+        if (final) {
+            uint64_t e = size;
+            if (p_end < size) {
+                const uint32_t xo = mis + (uint32_t)(p_end - cursor);
+                const uint32_t r = lds_word_run(S.corpus, xo, len - xo < 64 ? len - xo : 64);
+                if (r >= 32) lr = 1;
+                else e = p_end + r;
+            }
+            dsz = mis + (uint32_t)(e - cursor);
+        } else {
+            const uint32_t xo = mis + chunk;
+            ov = lds_word_run(S.corpus, xo, len - xo < 32 ? len - xo : 32);
+            if (ov >= 32) lr = 1;
+            dsz = mis + chunk + ov;
+        }
+        // the next window (a window whose look-ahead run reached p_end ended
+        // where this part ends)
+        const uint64_t ncur = cursor + (uint64_t)chunk + ov;
+        const bool more = !final && ncur < p_end;
+        bool nfinal = false;
+        uint64_t nwbase = 0;
+        uint32_t nlen = 0;
+        if (more) {
+            window(ncur, nfinal, nwbase, nlen);
+            if (((nwbase + nlen - 1) >> lp) >= pb + cnt) {
+                // it leaves the pinned pages; this window is in LDS already
+                gfs::unpin_pages(g, cnt);
+                pb = nwbase >> lp;
+                cnt = (uint32_t)(last_page - pb + 1 < 64 ? last_page - pb + 1 : 64);
+                st = gfs::pin_pages(g, fd, pb, cnt);
+                if (st) {
+                    cnt = 0;
+                    break;
+                }
+            }
+        }
+        grep_scan_window_lds((grep_lds_as3*)&S, dsz, skip, nwbase, pb, more ? (nlen + 15) >> 4 : 0u);
+        if (!more) break;
+        cursor = ncur;
+        final = nfinal;
+        wbase = nwbase;
+        len = nlen;
+    }
+    if (cnt) gfs::unpin_pages(g, cnt);
     *longrun = lr;
     return st;
 }
@@ -551,7 +721,7 @@ __device__ __noinline__ int grep_tail(grep_lds& S, int fd, uint64_t from, uint64
     [[clang::always_inline]] got = gread<gfs::SPACE_LDS>(fd, from - mis, len, S.corpus);
     if (got != len) return (int64_t)got < 0 ? (int)(int64_t)got : -EIO;
     for (uint64_t x = from; x < size; x += chunk)
-        [[clang::always_inline]] grep_scan_window(S, len, mis + (uint32_t)(x - from));
+        grep_scan_window(S, len, mis + (uint32_t)(x - from));
     __syncthreads();
     return 0;
 }
@@ -704,7 +874,10 @@ __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t n
         }
         if (j < kf && size && !st) {
             const uint64_t pb = split_point(size, j, kf), pe = split_point(size, j + 1, kf);
-            st = grep_part_windows(S, fd, chunk, size, pb, pe, &lr);
+            // a cached file is read from pinned frames, a cold one with gread
+            // (whose read-ahead streams it from the host)
+            st = warm ? grep_part_pinned(S, fd, chunk, size, pb, pe, &lr) : 1;
+            if (st == 1) st = grep_part_windows(S, fd, chunk, size, pb, pe, &lr);
             if (!single) {
                 for (uint32_t q = t; q < nw; q += T) {
                     const uint32_t k = S.canon[q];

@@ -253,6 +253,7 @@ static inline uint64_t mono_ns() {
 struct WakeState {
     std::atomic<uint32_t> word{0};
     std::atomic<uint32_t> sleepers{0};
+    std::atomic<uint32_t> kicks{0};   // launches announced by the host (Service::kick)
 };
 
 static inline void wake_all(WakeState* w) {
@@ -277,6 +278,7 @@ static inline void worker_loop(RingView rv, std::atomic<uint64_t>* head, std::at
     prctl(PR_SET_TIMERSLACK, 1000UL, 0, 0, 0);
     WorkerBufs wb;
     uint64_t idle = 0;
+    uint32_t kicks = wk->kicks.load(std::memory_order_relaxed);
     for (;;) {
         uint32_t s;
         int64_t close_fd;
@@ -293,6 +295,12 @@ static inline void worker_loop(RingView rv, std::atomic<uint64_t>* head, std::at
         // device posted without waiting are not left behind
         if (stop->load(std::memory_order_relaxed)) break;
         ++idle;
+        // a launch was announced: spin again, its first requests are close
+        const uint32_t k = wk->kicks.load(std::memory_order_relaxed);
+        if (k != kicks) {
+            kicks = k;
+            idle = 0;
+        }
         if (index == 0 || idle < 20000) {
 #if defined(__x86_64__)
             __builtin_ia32_pause();
@@ -327,6 +335,14 @@ public:
         running_ = false;
     }
     bool running() const { return running_; }
+    // Wakes sleeping workers and keeps every worker spinning for its spin
+    // period (20,000 polls, a millisecond or more), so that the requests of a
+    // kernel launched right after are served without a wake-up delay.
+    void kick() {
+        if (!running_) return;
+        wake_.kicks.fetch_add(1, std::memory_order_relaxed);
+        wake_all(&wake_);
+    }
     ServiceCounters& counters() { return ctr_; }
     uint64_t head() const { return head_.load(); }
     ~Service() { stop(); }

@@ -267,6 +267,11 @@ extern "C" int gpufs_host_start(gpufs_ctx* c) {
     c->service.start(ring_view(c), c->cfg.host_threads);
     return 0;
 }
+extern "C" int gpufs_host_kick(gpufs_ctx* c) {
+    if (!c) return -EINVAL;
+    c->service.kick();
+    return 0;
+}
 extern "C" int gpufs_host_stop(gpufs_ctx* c) {
     if (!c) return -EINVAL;
     std::lock_guard<std::mutex> g(c->mu);

@@ -100,6 +100,12 @@ int   gpufs_get_config(gpufs_ctx* ctx, gpufs_config* out);
 int   gpufs_run_handler(gpufs_ctx* ctx);
 int   gpufs_host_start(gpufs_ctx* ctx);
 int   gpufs_host_stop(gpufs_ctx* ctx);
+/* No reference counterpart (the reference's handler polls continuously): wakes
+ * the running service's idle worker threads and keeps them polling for about a
+ * millisecond, so a kernel launched right after has its first requests (opens)
+ * served without a thread wake-up delay.  gpufs.h's gpufs_bind() calls it;
+ * a no-op when the service is not running. */
+int   gpufs_host_kick(gpufs_ctx* ctx);
 
 /* ---- cache control and introspection (no reference counterpart; used by the
  * cold-read measurements and the invariant tests). */
