@@ -237,7 +237,7 @@ __device__ __noinline__ void grep_scan_window_lds(grep_lds_as3* SP, uint32_t dsz
             if (u < nunits) {
                 const uint64_t pos = wbase + 16ull * u;
                 const uint32_t fr = gfs_ctl.pinf[(pos >> lp) - pb];
-                pf[i] = *(const gfs::v4u*)(pool + ((uint64_t)fr << lp) + (pos & pm));
+                pf[i] = gfs::ld16_coh(pool + ((uint64_t)fr << lp) + (pos & pm));
             }
         }
     }
@@ -493,7 +493,7 @@ __device__ inline int grep_part_windows(grep_lds& S, int fd, uint32_t chunk, uin
 // results.  Needs a workgroup large enough that GREP_PF 16-byte units per
 // thread cover the largest window; returns 1 (nothing done) otherwise.
 __device__ inline int grep_part_pinned(grep_lds& S, int fd, uint32_t chunk, uint64_t size, uint64_t p_begin,
-                                       uint64_t p_end, uint32_t* longrun) {
+                                       uint64_t p_end, uint32_t* longrun, bool held) {
     gfs_dev* g = gfs_g;
     const uint32_t T = gfs::nthreads();
     if ((uint64_t)T * GREP_PF * 16 < (uint64_t)chunk + 64 + 16 + 16) return 1;
@@ -511,9 +511,11 @@ __device__ inline int grep_part_pinned(grep_lds& S, int fd, uint32_t chunk, uint
     bool final, first = p_begin > 0;
     uint32_t len, lr = 0;
     window(cursor, final, wbase, len);
-    uint64_t pb = wbase >> lp;
+    // `held`: every page of the file is pinned by the caller, page i's frame in
+    // pinf[i], and stays pinned after the return
+    uint64_t pb = held ? 0 : wbase >> lp;
     uint32_t cnt = (uint32_t)(last_page - pb + 1 < 64 ? last_page - pb + 1 : 64);
-    int st = gfs::pin_pages(g, fd, pb, cnt);
+    int st = held ? 0 : gfs::pin_pages(g, fd, pb, cnt);
     if (st) {
         *longrun = 0;
         return st;
@@ -573,7 +575,10 @@ __device__ inline int grep_part_pinned(grep_lds& S, int fd, uint32_t chunk, uint
         wbase = nwbase;
         len = nlen;
     }
-    if (cnt) gfs::unpin_pages(g, cnt);
+    // every wave's counts are in S.counts before the caller reads them (the
+    // last scan call has no barrier after it)
+    __syncthreads();
+    if (cnt && !held) gfs::unpin_pages(g, cnt);
     *longrun = lr;
     return st;
 }
@@ -602,7 +607,7 @@ typedef __attribute__((address_space(3))) uint16_t lds_u16;
 // word-byte masks in LDS; then wave 0 walks the steps over the masks.
 // Workgroup-collective; returns 0 or a negative code.
 __device__ __noinline__ int grep_chain_last(grep_lds& S, int fd, uint64_t size, uint32_t chunk, uint64_t* c_last,
-                                            uint32_t* cut) {
+                                            uint32_t* cut, bool held) {
     using namespace gfs;
     gfs_dev* g = gfs_g;
     // read once into scalars: reloaded inside the load loop, they made every
@@ -620,10 +625,13 @@ __device__ __noinline__ int grep_chain_last(grep_lds& S, int fd, uint64_t size, 
     uint64_t c = 0;
     uint32_t sp = 0;
     while (size - c >= (uint64_t)chunk + 32 && !sp) {
-        const uint64_t P = (c + chunk) >> L;
+        // `held`: every page of the file is pinned already, page i's frame in pinf[i]
+        const uint64_t P = held ? 0 : (c + chunk) >> L;
         const uint32_t cnt = (uint32_t)(npages - P < maxpin ? npages - P : maxpin);
-        const int r = pin_pages(g, fd, P, cnt);
-        if (r) return r;
+        if (!held) {
+            const int r = pin_pages(g, fd, P, cnt);
+            if (r) return r;
+        }
         const uint64_t hi0 = (P + cnt) << L;
         const uint64_t hi = hi0 < size ? hi0 : size;
         // this round's steps: as many as fit the unit budget and the pinned pages
@@ -657,7 +665,7 @@ __device__ __noinline__ int grep_chain_last(grep_lds& S, int fd, uint64_t size, 
                 if (on && ad[2 * k + h] < hi) {
                     const uint64_t a = ad[2 * k + h];
                     const uint32_t f = gfs_ctl.pinf[(a >> L) - P];
-                    val[2 * k + h] = *(const gv4u*)(pool + ((uint64_t)f << L) + (a & (PS - 1)));
+                    val[2 * k + h] = ld16_coh(pool + ((uint64_t)f << L) + (a & (PS - 1)));
                 }
             }
         }
@@ -701,7 +709,7 @@ __device__ __noinline__ int grep_chain_last(grep_lds& S, int fd, uint64_t size, 
         __syncthreads();
         c = uni((uint64_t)S.fsize);
         sp = (uint32_t)uni(S.status);
-        unpin_pages(g, cnt);
+        if (!held) unpin_pages(g, cnt);
     }
     *c_last = c;
     *cut = sp;
@@ -865,8 +873,20 @@ __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t n
     __syncthreads();
     on_size((uint32_t)kf);
     if (fd >= 0) {
-        if (j == 0 && size && warm) {
-            st = grep_chain_last(S, fd, size, chunk, &cl, &cut);
+        // Part 0 of a cached file that fits one pin batch pins all of it once
+        // for both the chain and its own part.
+        uint32_t held = 0;
+        if (j == 0 && size && warm && !st) {
+            const uint64_t np = (size + gfs_g->page_size - 1) >> gfs_g->log_page;
+            uint32_t maxpin = gfs_g->n_frames / 8;
+            maxpin = maxpin > 64 ? 64 : (maxpin < 2 ? 2 : maxpin);
+            if (np <= maxpin) {
+                st = gfs::pin_pages(gfs_g, fd, 0, (uint32_t)np);
+                if (!st) held = (uint32_t)np;
+            }
+        }
+        if (j == 0 && size && warm && !st) {
+            st = grep_chain_last(S, fd, size, chunk, &cl, &cut, held != 0);
             if (t == 0 && !single) {
                 fs->c_last = cl;
                 fs->cut = cut;
@@ -876,7 +896,7 @@ __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t n
             const uint64_t pb = split_point(size, j, kf), pe = split_point(size, j + 1, kf);
             // a cached file is read from pinned frames, a cold one with gread
             // (whose read-ahead streams it from the host)
-            st = warm ? grep_part_pinned(S, fd, chunk, size, pb, pe, &lr) : 1;
+            st = warm ? grep_part_pinned(S, fd, chunk, size, pb, pe, &lr, held != 0) : 1;
             if (st == 1) st = grep_part_windows(S, fd, chunk, size, pb, pe, &lr);
             if (!single) {
                 for (uint32_t q = t; q < nw; q += T) {
@@ -887,8 +907,9 @@ __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t n
                 __threadfence();
             }
         }
+        if (held) gfs::unpin_pages(gfs_g, held);
         if (j == 0 && size && !st && !warm) {
-            st = grep_chain_last(S, fd, size, chunk, &cl, &cut);
+            st = grep_chain_last(S, fd, size, chunk, &cl, &cut, false);
             if (t == 0 && !single) {
                 fs->c_last = cl;
                 fs->cut = cut;
