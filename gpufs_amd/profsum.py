@@ -2,6 +2,7 @@
 
     python -m gpufs_amd.profsum trace   <dir> <kernel-substring> [--skip N]
     python -m gpufs_amd.profsum counter <dir> <kernel-substring> [--skip N]
+    python -m gpufs_amd.profsum sq      <dir> <kernel-substring> [--skip N]
 
 `trace` reads <dir>/*_kernel_trace.csv and prints every dispatch of the
 kernel with its duration, then the mean over the dispatches after the first
@@ -78,14 +79,57 @@ def counter(d, kernel, skip):
     return res
 
 
+def sq(d, kernel, skip, xcds=8, cus=256):
+    """Shader counters of each dispatch (summed over instances), then the mean
+    over the dispatches after the first `skip` with the ratios they give:
+    elapsed cycles per XCD (GRBM_GUI_ACTIVE / xcds), busy CUs, VALU and LDS
+    instructions per wave, waves' share of time waiting, and bank-conflict
+    cycles over LDS-active cycles."""
+    rows = list(csv.DictReader(open(_one(os.path.join(d, "*counter_collection.csv")))))
+    per, order = {}, {}
+    for r in rows:
+        if kernel not in r["Kernel_Name"]:
+            continue
+        di = int(r["Dispatch_Id"])
+        order[di] = int(r["Start_Timestamp"])
+        per.setdefault(di, {})
+        per[di][r["Counter_Name"]] = per[di].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    ds = sorted(per, key=lambda k: order[k])
+    for i, di in enumerate(ds):
+        vals = "  ".join(f"{c}={v:.0f}" for c, v in sorted(per[di].items()))
+        print(f"dispatch {di:>5}{' (skipped)' if i < skip else ''}: {vals}")
+    keep = ds[skip:]
+    if not keep:
+        return {}
+    m = {c: sum(per[di].get(c, 0.0) for di in keep) / len(keep) for c in per[keep[0]]}
+    print(f"mean over {len(keep)} dispatches after the first {skip}:")
+    for c, v in sorted(m.items()):
+        print(f"  {c} = {v:.0f}")
+    g = m.get
+    if g("GRBM_GUI_ACTIVE"):
+        el = g("GRBM_GUI_ACTIVE") / xcds
+        print(f"  elapsed cycles per XCD = {el:.0f}")
+        if g("SQ_BUSY_CU_CYCLES"):
+            print(f"  SQ_BUSY_CU_CYCLES / (elapsed x {cus} CUs) = {g('SQ_BUSY_CU_CYCLES') / (el * cus):.3f}")
+    if g("SQ_WAVES"):
+        for c in ("SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_INSTS_SALU", "SQ_INSTS_VMEM_RD"):
+            if g(c):
+                print(f"  {c} per wave = {g(c) / g('SQ_WAVES'):.0f}")
+    if g("SQ_WAIT_ANY") and g("SQ_WAVE_CYCLES"):
+        print(f"  SQ_WAIT_ANY / SQ_WAVE_CYCLES = {g('SQ_WAIT_ANY') / g('SQ_WAVE_CYCLES'):.3f}")
+    if g("SQ_LDS_BANK_CONFLICT") and g("SQ_LDS_IDX_ACTIVE"):
+        print(f"  SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE = {g('SQ_LDS_BANK_CONFLICT') / g('SQ_LDS_IDX_ACTIVE'):.3f}")
+    return m
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["trace", "counter"])
+    ap.add_argument("mode", choices=["trace", "counter", "sq"])
     ap.add_argument("dir")
     ap.add_argument("kernel")
     ap.add_argument("--skip", type=int, default=0)
     a = ap.parse_args(argv)
-    (trace if a.mode == "trace" else counter)(a.dir, a.kernel, a.skip)
+    {"trace": trace, "counter": counter, "sq": sq}[a.mode](a.dir, a.kernel, a.skip)
 
 
 if __name__ == "__main__":
