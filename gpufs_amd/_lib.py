@@ -82,6 +82,7 @@ SIGNATURES = {
     "gpufs_serve_pages": (I, [P, P, P]),
     "gpufs_launch_mmap_warp_records": (I, [P, P, U64, U64, P, P, U32, U32, P]),
     "gpufs_grep_text": (I, [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, U32, U32, U32, P, P]),
+    "gpufs_grep_text_records": (I, [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, U32, U32, U32, P, P]),
     "gpufs_launch_vm_records": (I, [P, P, U64, P, U64, P, P, U32, U32, U32, P]),
     "gpufs_launch_gemv_mmap": (I, [P, P, U64, U64, P, P, U32, P, P, U32, P]),
     "gpufs_fnv1a64_ranges": (I, [P, U32, U64, P, P]),

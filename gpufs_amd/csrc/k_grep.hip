@@ -393,7 +393,14 @@ struct grep_text_args {
     unsigned long long* out_off;
     uint32_t* trunc;             // 0, 1 while the output is truncated, 2 done, 3 failed
     int32_t* status;
+    uint32_t records;            // 0: text lines; 1: fixed 100-byte binary records
 };
+
+// Binary record of one (query record, window) match, the layout that the
+// reference's record printer reads (samples/utils/display.cpp:16-25): the
+// query's 32 bytes as stored, the count as a 32-bit integer, the database
+// name NUL-padded to 64 bytes.
+#define GREP_REC_BYTES 100u
 
 // get_next() of grep_text_gpu.cu:163-186 on a NUL-terminated list: skips
 // ' ', '\t', ',', '\r', '\n'; a name ends at '\n', '\r', ',' or NUL; returns
@@ -489,6 +496,13 @@ __global__ __launch_bounds__(1024) void k_grep_text(grep_text_args a) {
                     const uint32_t k = S.canon[q];
                     const uint32_t c = k == 0xFFFF ? 0u : S.counts[k];
                     if (!c) continue;
+                    if (a.records) {
+                        char* o = lines + atomicAdd(&line_bytes, GREP_REC_BYTES);
+                        for (uint32_t i = 0; i < 32; ++i) o[i] = words[(size_t)q * 32 + i];
+                        for (uint32_t i = 0; i < 4; ++i) o[32 + i] = (char)(c >> (8 * i));
+                        for (uint32_t i = 0; i < 64; ++i) o[36 + i] = i < nl ? name[i] : 0;
+                        continue;
+                    }
                     char num[10];
                     uint32_t nd = 0;
                     for (uint32_t v = c; v; v /= 10) num[nd++] = (char)('0' + v % 10);
@@ -527,9 +541,9 @@ __global__ __launch_bounds__(1024) void k_grep_text(grep_text_args a) {
     if (t == 0 && (st || co)) atomicMin(a.status, st ? st : co);
 }
 
-extern "C" int gpufs_grep_text(gpufs_ctx* ctx, const char* src_path, const char* out_path, const char* dbs_path,
-                               uint32_t chunk_bytes, uint32_t workgroups, uint32_t threads, int32_t* status_dev,
-                               void* stream) {
+static int grep_text_launch(gpufs_ctx* ctx, const char* src_path, const char* out_path, const char* dbs_path,
+                            uint32_t chunk_bytes, uint32_t workgroups, uint32_t threads, int32_t* status_dev,
+                            void* stream, uint32_t records) {
     if (!ctx || !src_path || !out_path || !dbs_path || !status_dev) return -EINVAL;
     if (chunk_bytes == 0 || chunk_bytes > GREP_MAX_CHUNK) return -EINVAL;
     if (threads < 64 || threads > 1024 || threads % 64 || workgroups == 0) return -EINVAL;
@@ -587,8 +601,22 @@ extern "C" int gpufs_grep_text(gpufs_ctx* ctx, const char* src_path, const char*
     if (hipStreamSynchronize(s) != hipSuccess) return release(-EIO);   // hn is a stack buffer
     grep_text_args a{names, names + GFS_NAME_MAX, names + 2 * GFS_NAME_MAX, total, (uint32_t)wpc, chunk_bytes,
                      scratch, stride, off_list, off_name, off_lines, list_cap, off,
-                     (uint32_t*)(names + 3 * GFS_NAME_MAX + 8), status_dev};
+                     (uint32_t*)(names + 3 * GFS_NAME_MAX + 8), status_dev, records};
     hipLaunchKernelGGL(k_grep_text, dim3((uint32_t)grid), dim3(threads), 0, s, a);
     const int lr = hipGetLastError() == hipSuccess ? (int)grid : -EIO;
     return release(lr);
+}
+
+extern "C" int gpufs_grep_text(gpufs_ctx* ctx, const char* src_path, const char* out_path, const char* dbs_path,
+                               uint32_t chunk_bytes, uint32_t workgroups, uint32_t threads, int32_t* status_dev,
+                               void* stream) {
+    return grep_text_launch(ctx, src_path, out_path, dbs_path, chunk_bytes, workgroups, threads, status_dev, stream,
+                            0);
+}
+
+extern "C" int gpufs_grep_text_records(gpufs_ctx* ctx, const char* src_path, const char* out_path,
+                                       const char* dbs_path, uint32_t chunk_bytes, uint32_t workgroups,
+                                       uint32_t threads, int32_t* status_dev, void* stream) {
+    return grep_text_launch(ctx, src_path, out_path, dbs_path, chunk_bytes, workgroups, threads, status_dev, stream,
+                            1);
 }

@@ -260,6 +260,14 @@ int gpufs_launch_vm_records(gpufs_ctx* ctx, const char* path_dev, uint64_t recor
 int gpufs_grep_text(gpufs_ctx* ctx, const char* src_path, const char* out_path, const char* dbs_path,
                     uint32_t chunk_bytes, uint32_t workgroups, uint32_t threads, int32_t* status_dev,
                     void* stream);
+/* The same, writing a fixed 100-byte binary record per (record, window) match
+ * instead of a text line: the query's 32 bytes as stored, the count as a
+ * little-endian int32, the database name NUL-padded to 64 bytes: the record
+ * layout the reference's printer reads (samples/utils/display.cpp:16-25;
+ * gpufs_amd.display prints such a file the same way). */
+int gpufs_grep_text_records(gpufs_ctx* ctx, const char* src_path, const char* out_path, const char* dbs_path,
+                            uint32_t chunk_bytes, uint32_t workgroups, uint32_t threads, int32_t* status_dev,
+                            void* stream);
 
 /* Record checksums through the wave-level mapping gmmap_warp / gmunmap_warp
  * (samples/mosaic/mosaic_gpufs.cu:612-651 pattern; device API

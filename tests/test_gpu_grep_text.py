@@ -22,12 +22,12 @@ def short_dir():
     shutil.rmtree(d, ignore_errors=True)
 
 
-def _run(ctx, src, out, dbs, chunk, workgroups, threads=1024):
+def _run(ctx, src, out, dbs, chunk, workgroups, threads=1024, records=False):
     torch = torch_cuda()
     st = dev_zeros(1, torch.int32)
     torch.cuda.synchronize()
-    grid = _lib.lib().gpufs_grep_text(ctx.h, src.encode(), out.encode(), dbs.encode(), chunk, workgroups, threads,
-                                      st.data_ptr(), None)
+    fn = _lib.lib().gpufs_grep_text_records if records else _lib.lib().gpufs_grep_text
+    grid = fn(ctx.h, src.encode(), out.encode(), dbs.encode(), chunk, workgroups, threads, st.data_ptr(), None)
     _lib.check("gpufs_grep_text", grid)
     finish(ctx)
     return grid, int(st.cpu().item())
@@ -99,4 +99,39 @@ def test_grep_text_missing_database_fails_cleanly(short_dir):
     with make_ctx() as ctx:
         grid, st = _run(ctx, src, os.path.join(short_dir, "o.txt"), lst, 65536, 2)
         assert st < 0
+        assert ctx.check_invariants() == 0
+
+
+def test_grep_text_binary_records(orc, short_dir):
+    """The same scan writing display.cpp's 100-byte records: printed with
+    gpufs_amd.display they are the text form's lines (as a multiset), and the
+    record fields match the oracle's lines."""
+    say("test_grep_text_binary_records")
+    from gpufs_amd import display
+    words = np.fromfile(os.path.join(GOLDEN, "dict_subset.bin"), dtype=np.uint8).reshape(-1, 32)
+    src = os.path.join(short_dir, "dict.gpu")
+    words.tofile(src)
+    text = open(os.path.join(GOLDEN, "pg100_small.txt"), "rb").read()
+    dbs = []
+    for i, body in enumerate([text, text[::-1]]):
+        p = os.path.join(short_dir, f"db{i}.txt")
+        open(p, "wb").write(body)
+        dbs.append(p)
+    lst = os.path.join(short_dir, "dbs.txt")
+    open(lst, "w").write("\n".join(dbs) + "\n")
+    out = os.path.join(short_dir, "out.bin")
+    want = orc.grep_text_lines(words, open(lst, "rb").read(), 4096)
+    open(out, "wb").write(b"\x7f" * 12345)   # an older, longer output: truncated first
+    with make_ctx(page_size=64 << 10, cache_bytes=64 << 20) as ctx:
+        grid, st = _run(ctx, src, out, lst, 4096, 8, records=True)
+        assert st == 0
+        data = open(out, "rb").read()
+        assert len(data) % display.RECORD_BYTES == 0 and len(data) // display.RECORD_BYTES == len(want.splitlines())
+        recs = display.records(data)
+        got = sorted(f"{w.decode()} {c} {n.decode()}" for w, c, n in recs)
+        assert got == sorted(want.decode().splitlines())
+        for (w, c, n), line in zip(recs, display.lines(data)):
+            assert c > 0 and n.decode() in dbs
+            if len(w) < 32:   # a full 32-byte query prints on into its count, as display.cpp does
+                assert line == w + b" " + str(c).encode() + b" " + n + b"\n"
         assert ctx.check_invariants() == 0
