@@ -5,9 +5,9 @@
 // segment is read with four ds_read_b128, word bytes ([A-Za-z_]) are flagged
 // four at a time with SWAR arithmetic into a 64-bit mask, token starts are
 // mask & ~(mask << 1 | previous byte).  A token is skipped at once when no
-// query word has its length; otherwise its first eight bytes (a funnel shift
-// of two aligned LDS words, bytes past the token zeroed) and its length are
-// hashed with 32-bit arithmetic.  The hash's top bits index a 2^18-bit
+// query word has its length; otherwise its first eight bytes (byte-aligned
+// from three LDS dwords, bytes past the token zeroed) and its length are
+// hashed with one 32-bit multiply.  The hash's top bits index a 2^18-bit
 // membership filter of the dictionary: a token whose bit is clear is no query
 // word (almost all tokens; measured: without the filter, walking the table's
 // probe chains for them took a third of the scan).  Its low bits pick a slot
@@ -54,13 +54,15 @@ __device__ __forceinline__ uint32_t alpha4(uint32_t d) {
     return ((v >> 7) & 1u) | ((v >> 14) & 2u) | ((v >> 21) & 4u) | ((v >> 28) & 8u);
 }
 
-// Hash of a token's first eight bytes and its length, 32-bit arithmetic.
+// Hash of a token's first eight bytes and its length: one 32-bit multiply
+// (a quarter-rate instruction: the three of a full 32-bit finaliser cost more
+// than the rest of the hash).  The filter takes the product's high bits,
+// which depend on every input bit; the table slot takes its low bits after
+// the high half is folded into them.
 __device__ __forceinline__ uint32_t tok_hash8(uint64_t w0, uint32_t len) {
-    uint32_t h = (uint32_t)w0 ^ ((uint32_t)(w0 >> 32) * 0x9E3779B1u) ^ (len << 26);
-    h *= 0x85EBCA6Bu;
-    h ^= h >> 15;
-    h *= 0xC2B2AE35u;
-    return h ^ (h >> 16);
+    const uint32_t lo = (uint32_t)w0, hi = (uint32_t)(w0 >> 32);
+    const uint32_t h = (lo ^ ((hi << 13) | (hi >> 19)) ^ (len << 26)) * 0x9E3779B1u;
+    return h ^ (h >> 15);
 }
 
 // Eight bytes of the LDS window starting at byte p (two aligned loads).
@@ -75,13 +77,14 @@ __device__ __forceinline__ uint64_t keep_bytes(uint64_t v, int n) {
     return n >= 8 ? v : (n <= 0 ? 0ull : (v & ((1ull << (8 * n)) - 1)));
 }
 // Eight bytes of the LDS window starting at byte p, without a branch on the
-// alignment (reads both aligned words; the second shift is split so that an
-// aligned p shifts the high word out entirely).
+// alignment: three aligned dwords and two byte-aligns.
 __device__ __forceinline__ uint64_t lds_u64_nb(const uint8_t* corpus, uint32_t p) {
-    const uint64_t* W = (const uint64_t*)corpus;
-    const uint32_t a = p >> 3, sh = (p & 7) * 8;
-    const uint64_t lo = W[a], hi = W[a + 1];
-    return (lo >> sh) | ((hi << 1) << (63 - sh));
+    const uint32_t* W = (const uint32_t*)corpus;
+    const uint32_t a = p >> 2, sh = p & 3;
+    const uint32_t w0 = W[a], w1 = W[a + 1], w2 = W[a + 2];
+    const uint32_t lo = __builtin_amdgcn_alignbyte(w1, w0, sh);
+    const uint32_t hi = __builtin_amdgcn_alignbyte(w2, w1, sh);
+    return (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
 __device__ inline void grep_build_dict(grep_lds& S, const char* dict, uint32_t nw) {
