@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--cold-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grep", action="store_true")
+    ap.add_argument("--no-write", action="store_true")
+    ap.add_argument("--write-steps", type=int, default=3)
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched leg (10,000 uneven files, one hipGraph-captured persistent kernel)")
     ap.add_argument("--no-serve", action="store_true")
@@ -106,12 +108,14 @@ def _warm_host_cache(paths, chunk=64 << 20):
                 pass
 
 
-def link_peak_gbs(torch, nbytes=1 << 30, trials=5):
+def link_peak_gbs(torch, nbytes=1 << 30, trials=5, to_host=False):
     """Pinned host -> device copy of 1 GiB, best of `trials` (the role of the
-    reference's pure-H2D option, samples/memcopy/cp.cpp:189-199)."""
-    src = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
-    src.fill_(1)
-    dst = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    reference's pure-H2D option, samples/memcopy/cp.cpp:189-199); device ->
+    pinned host with `to_host`."""
+    host = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+    host.fill_(1)
+    dev = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    src, dst = (dev, host) if to_host else (host, dev)
     s = torch.cuda.Stream()
     best = 0.0
     for _ in range(trials):
@@ -122,8 +126,32 @@ def link_peak_gbs(torch, nbytes=1 << 30, trials=5):
             e1.record(s)
         e1.synchronize()
         best = max(best, nbytes / (e0.elapsed_time(e1) / 1e3) / 1e9)
-    del src, dst
+    del src, dst, host, dev
     return best
+
+
+def host_pwrite_gbs(path, size, threads, piece=4 << 20):
+    """The host's own write rate into the page cache: `threads` threads pwrite
+    `size` bytes in `piece`-byte pieces from one buffer (the host half of
+    every GPU write-back, which ends in the same pwrite)."""
+    from concurrent.futures import ThreadPoolExecutor
+    buf = bytes(np.random.default_rng(1).integers(0, 256, size=piece, dtype=np.uint8))
+    fd = os.open(path, os.O_WRONLY | os.O_CREAT, 0o644)
+    try:
+        n = size // piece
+
+        def work(k):
+            for i in range(k, n, threads):
+                os.pwrite(fd, buf, i * piece)
+
+        t = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(work, range(threads)))
+        dt = time.perf_counter() - t
+    finally:
+        os.close(fd)
+        os.unlink(path)
+    return size / dt / 1e9
 
 
 def copy_ceiling_gbs(torch, nbytes=1 << 30, trials=5):
@@ -173,6 +201,8 @@ def main():
 
     link = link_peak_gbs(torch)
     log(f"link peak (pinned H2D 1 GiB, best of 5): {link:.1f} GB/s")
+    link_d2h = link_peak_gbs(torch, to_host=True)
+    log(f"link peak (pinned D2H 1 GiB, best of 5): {link_d2h:.1f} GB/s")
     copy_ceiling = None
     if not args.no_copy_ceiling:
         copy_ceiling = copy_ceiling_gbs(torch)
@@ -277,6 +307,12 @@ def main():
     cold_median = size / float(np.median(cold)) / 1e9 if cold else None
     log(f"warm: {value:.1f} GB/s payload (kernel avg {avg_k * 1e3:.3f} ms, {achieved:.0f} GB/s HBM moved)")
 
+    write_res = None
+    if not args.no_write:
+        write_res = bench_write(args, ctx, torch, L, _lib, dst, size, n, rng_bytes, stream, gpu_fnv, cpu_fnv)
+        write_res["d2h_link_GBps"] = round(link_d2h, 2)
+        write_res["frac_of_d2h_link_median"] = round(write_res["GBps_median"] / link_d2h, 4)
+
     grep_res = None
     if not args.no_grep:
         grep_res = bench_grep(args, ctx, torch, L, _lib, dg)
@@ -322,12 +358,68 @@ def main():
                  "frac_of_link_best": round(cold_best / link, 4) if cold_best else None},
         "bit_exact": {"cold_fnv": cold_ok, "warm_fnv": warm_ok},
         "cpu_baseline": cpu,
+        "write": write_res,
         "grep": grep_res,
         "batched": batched_res,
         "gemv": gemv_res,
         "served": serve_res,
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_write(args, ctx, torch, L, _lib, src, size, n, rng_bytes, stream, gpu_fnv, cpu_fnv):
+    """Sequential gwrite + gfsync + gclose: the buffer the gread passes filled
+    (the bytes of the 8 GiB file) is written to a new file, workgroup w its
+    range w, each pass on an emptied page cache (every page takes a fresh
+    frame and crosses the link to the host).  Then the file is read back cold
+    through gread and its per-range FNV-1a-64 compared with the CPU's of the
+    source file."""
+    wpath = os.path.join(args.workdir, f"w_{size}.bin")
+    if os.path.exists(wpath):
+        os.unlink(wpath)
+    name = _names_dev(torch, [wpath])
+    out = torch.zeros(n, dtype=torch.int64, device="cuda")
+    flags = os.O_WRONLY | os.O_CREAT
+    each = []
+    for i in range(args.write_steps):
+        ctx.cache_drop()
+        ctx.stats_reset()
+        out.zero_()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        _lib.check("gpufs_launch_gwrite_ranges",
+                   L.gpufs_launch_gwrite_ranges(ctx.h, ctypes.c_void_p(name.data_ptr()), flags, n, rng_bytes,
+                                                ctypes.c_void_p(src.data_ptr()), 1,
+                                                ctypes.c_void_p(out.data_ptr()), args.threads, None))
+        e1.record(stream)
+        e1.synchronize()
+        dt = e0.elapsed_time(e1) / 1e3
+        o = out.cpu().numpy()
+        assert (o == rng_bytes).all(), o[:8]
+        st = ctx.stats()
+        each.append(size / dt / 1e9)
+        log(f"write pass {i}: {each[-1]:.1f} GB/s  bytes_written={st['bytes_written']} "
+            f"rpc_writes={st['rpc_writes']}")
+    assert os.path.getsize(wpath) == size
+    # read the file back cold into the same buffer and compare checksums
+    ctx.cache_drop()
+    _lib.check("gpufs_launch_gread_ranges",
+               L.gpufs_launch_gread_ranges(ctx.h, ctypes.c_void_p(name.data_ptr()), n, rng_bytes,
+                                           ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                           args.threads, None))
+    torch.cuda.synchronize()
+    assert (out.cpu().numpy() == rng_bytes).all()
+    exact = bool((gpu_fnv() == cpu_fnv).all()) if cpu_fnv is not None else None
+    assert exact is not False, "file written by gwrite differs from its source"
+    os.unlink(wpath)
+    hp = host_pwrite_gbs(os.path.join(args.workdir, "pwrite_probe.bin"), size, args.host_threads)
+    log(f"host pwrite ({args.host_threads} threads, 4 MiB pieces): {hp:.1f} GB/s")
+    return {"workload": f"gwrite + gfsync + gclose: new {size / 2**30:.0f} GiB file, {n} workgroups x "
+                        f"{rng_bytes >> 20} MiB ranges, emptied page cache per pass",
+            "GBps_median": round(float(np.median(each)), 2), "GBps_each": [round(x, 2) for x in each],
+            "bytes_exact_fnv": exact, "host_pwrite_GBps": round(hp, 2),
+            "frac_of_host_pwrite_median": round(float(np.median(each)) / hp, 4)}
 
 
 def bench_grep(args, ctx, torch, L, _lib, dg):

@@ -68,6 +68,7 @@ SIGNATURES = {
     "gpufs_check_invariants": (I, [P]),
     "gpufs_strerror": (ctypes.c_char_p, [I]),
     "gpufs_launch_gread_ranges": (I, [P, P, U32, U64, P, P, U32, P]),
+    "gpufs_launch_gwrite_ranges": (I, [P, P, I32, U32, U64, P, I32, P, U32, P]),
     "gpufs_launch_grep": (I, [P, P, U32, U32, P, U32, U32, P, P, U32, P]),
     "gpufs_launch_grep_split": (I, [P, P, U32, U32, P, U32, U32, P, P, U32, U64, U32, P]),
     "gpufs_launch_grep_noalloc": (I, [P, P, U32, U32, P, U32, U32, P, P, U32, U64, U32, P]),

@@ -160,6 +160,7 @@ typedef struct gfs_dev {
     uint32_t open_gate;              // name-index users in flight [0,31) | exclusive bit 31
     uint32_t pad2;
     unsigned long long node_free;    // free radix nodes: stack head, tag [40,64) | node [0,40)
+    long long dirty_frames;          // frames whose DIRTY bit is set (the writers' write-back throttle)
     gfs_counters ctr;
 } gfs_dev;
 

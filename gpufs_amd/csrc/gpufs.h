@@ -89,7 +89,7 @@ struct wg_ctl {
     // gread's entry results (thread 0 writes them before gread's first barrier,
     // so no other call reads them after its own last barrier)
     int64_t gr_ret;
-    uint64_t gr_fs, gr_root;
+    uint64_t gr_root;
     uint32_t gr_ra;
     uint32_t pinf[64];       // pin_pages: frame of each pinned page
 };
@@ -243,6 +243,21 @@ __device__ __forceinline__ void wave_count(unsigned long long* c, bool pred) {
     uint64_t m = __ballot(pred);
     if (m && lane() == first_lane(m)) fadd_rlx(c, (unsigned long long)popc(m));
 }
+// The count of dirty frames (gfs_dev::dirty_frames) follows every change of a
+// frame's DIRTY bit: the wave's lanes whose frame changed (pred) are added
+// (up) or taken off with one atomic.
+__device__ __forceinline__ void dirty_tally(gfs_dev* g, bool pred, bool up) {
+    const uint64_t m = __ballot(pred);
+    if (m && lane() == first_lane(m)) {
+        const unsigned long long k = (unsigned long long)popc(m);
+        fadd_rlx(&g->dirty_frames, up ? k : (unsigned long long)0 - k);
+    }
+}
+// Sets frame f's DIRTY bit; true when the frame was clean.
+__device__ __forceinline__ bool set_dirty(gfs_dev* g, uint32_t f) {
+    return !(__hip_atomic_fetch_or(&g->meta[f], GFS_FL_DIRTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+             GFS_FL_DIRTY);
+}
 
 // ======================================================================
 // Copy engine: workgroup-cooperative byte copies with 16-byte vector bodies.
@@ -372,6 +387,14 @@ __device__ inline void copy_seg(uint8_t* dst, const uint8_t* src, uint64_t len, 
         }
         store16(D + 16 * u, o, dl, nt_store);
     }
+}
+
+// Zeroes bytes [from, to) of a frame (`to` a multiple of 16) with thread j of
+// T: the head up to a 16-byte boundary byte by byte, then 16-byte stores.
+__device__ inline void zero_tail(uint8_t* fp, uint64_t from, uint64_t to, uint32_t j, uint32_t T) {
+    const uint64_t a = (from + 15) & ~15ull;
+    if (j < a - from && from + j < to) fp[from + j] = 0;
+    for (uint64_t k = (a >> 4) + j; k < (to >> 4); k += T) *(v4u*)(fp + 16 * k) = v4u{0, 0, 0, 0};
 }
 
 // Whole-workgroup copy of one buffer.
@@ -910,6 +933,20 @@ __device__ inline bool wg_timed_out(gfs_dev* g, uint64_t t0) {
     return uni(gfs_ctl.tmo) != 0;
 }
 
+// Workgroup-uniform: more than three quarters of the frames are dirty.  A
+// writer that sees it writes its file back before it dirties more pages, so
+// readers keep finding clean frames to evict (the eviction sweep takes only
+// clean ones).  Call with all threads.
+__device__ inline bool wg_dirty_high(gfs_dev* g) {
+    __syncthreads();
+    if (tid() == 0) {
+        const long long d = (long long)ld_rlx(&g->dirty_frames);
+        gfs_ctl.tmo = d > (long long)(g->n_frames - (g->n_frames >> 2)) ? 1u : 0u;
+    }
+    __syncthreads();
+    return uni(gfs_ctl.tmo) != 0;
+}
+
 // ======================================================================
 // Page resolution for a batch of up to 64 consecutive pages (wave 0 only).
 // On return gfs_ctl.kind[i] is HIT (pinned VALID frame), OWN (pinned FILLING
@@ -944,6 +981,7 @@ __device__ inline void resolve_batch(gfs_dev* g, int fd, uint64_t pb, uint32_t c
     uint64_t* sp = &g->nodes[node * GFS_FANOUT + (page & (GFS_FANOUT - 1))];
     uint32_t kind = GFS_K_DONE, f = GFS_NOFRAME, fgen = 0;
     uint64_t v = 0;
+    bool newly_dirty = false;
     if (mine) {
         v = ld_rlx(sp);
         kind = GFS_K_MISS;
@@ -957,7 +995,7 @@ __device__ inline void resolve_batch(gfs_dev* g, int fd, uint64_t pb, uint32_t c
                 f = vf;
                 fgen = vg;
                 g->fref[vf] = 1;   // CLOCK reference bit: a store, no load to wait for
-                if (for_write) __hip_atomic_fetch_or(&g->meta[vf], GFS_FL_DIRTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (for_write) newly_dirty = set_dirty(g, vf);
             } else {
                 fadd_rlx(&g->meta[vf], ~0ull);
                 if (meta_gen(old) == vg && meta_state(old) == GFS_F_FILLING) kind = GFS_K_WAIT;
@@ -1002,6 +1040,7 @@ __device__ inline void resolve_batch(gfs_dev* g, int fd, uint64_t pb, uint32_t c
     wave_count(&g->ctr.page_waits, kind == GFS_K_WAIT && mine);
     wave_count(&g->ctr.page_hits, kind == GFS_K_HIT);
     wave_count(&g->ctr.page_misses, kind == GFS_K_OWN);
+    dirty_tally(g, newly_dirty, true);
     {
         const uint64_t hm = __ballot(kind == GFS_K_HIT), om = __ballot(kind == GFS_K_OWN);
         if (ln == 0) { gfs_ctl.hits += popc(hm); gfs_ctl.misses += popc(om); }
@@ -1022,9 +1061,14 @@ __device__ inline void resolve_batch(gfs_dev* g, int fd, uint64_t pb, uint32_t c
         cq = (uint32_t)__shfl(s, leader);
         if (ln == leader) sys_st(&g->cq[cq], 0u);
         if (post) {
+            // A reader asks for the bytes below the file size.  A writer asks
+            // for the whole page: other writers of the file may have written
+            // back bytes past the size it knows, and a page it reads in is
+            // written back whole.  Past the host's end of file the read is
+            // short and the frame is zero there.
             uint64_t poff = page << g->log_page;
             uint64_t plen = g->page_size;
-            if (poff + plen > file->size) plen = file->size > poff ? file->size - poff : 0;
+            if (!for_write && poff + plen > file->size) plen = file->size > poff ? file->size - poff : 0;
             rpc_post(g, s, GFS_OP_READ, file->host_fd, 0, poff, plen, cq | (ln << 16));
         }
         wave_count(&g->ctr.rpc_reads, post);
@@ -1079,8 +1123,9 @@ __device__ __forceinline__ uint8_t* staging_ptr(gfs_dev* g, uint32_t s) {
 // while the host works on the rest.  The slot whose completion word the batch
 // uses is kept to the end (finish_batch returns it).  Pages whose read failed
 // or came up short are marked ERR and gfs_ctl.err is set.
-__device__ inline void fill_from_host(gfs_dev* g, uint32_t cnt, uint64_t pb, uint64_t fsize) {
+__device__ inline void fill_from_host(gfs_dev* g, uint32_t cnt, uint64_t pb) {
     (void)cnt;
+    (void)pb;
     const uint64_t own = uni(gfs_ctl.own);
     if (own == 0) return;
     const uint32_t cq = uni(gfs_ctl.cq);
@@ -1114,11 +1159,10 @@ __device__ inline void fill_from_host(gfs_dev* g, uint32_t cnt, uint64_t pb, uin
         if (wave() == 0) {
             const uint32_t ln = lane();
             if ((got >> ln) & 1ull) {
+                // the host's bytes of the page: short at the host's end of
+                // file (the rest of the frame is zeroed below)
                 int64_t r = tmo ? (int64_t)-ETIMEDOUT : sys_ld(&g->slots[gfs_ctl.slot[ln]].result);
-                const uint64_t poff = (pb + ln) << g->log_page;
-                uint64_t want = fsize > poff ? fsize - poff : 0;
-                if (want > g->page_size) want = g->page_size;
-                if (r < 0 || (uint64_t)r != want) {
+                if (r < 0 || (uint64_t)r > g->page_size) {
                     gfs_ctl.kind[ln] = GFS_K_ERR;
                     atomicCAS(&gfs_ctl.err, 0, r < 0 ? (int)r : -EIO);
                 } else {
@@ -1146,6 +1190,7 @@ __device__ inline void fill_from_host(gfs_dev* g, uint32_t cnt, uint64_t pb, uin
             // read loop, and eight would cost the warm path register spills
             copy_seg<true, false, SPACE_GLOBAL, SPACE_GLOBAL, 4>(frame_ptr(g, f), staging_ptr(g, gfs_ctl.slot[i]),
                                                                  bytes, (t + T - rot) % T, T);
+            if (bytes < g->page_size) zero_tail(frame_ptr(g, f), bytes, g->page_size, t, T);
             rot = (rot + (uint32_t)(bytes >> 4)) % T;
             filled += bytes;
         }
@@ -1301,7 +1346,6 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
             gfs_hist.fetched = fetched;
         }
         gfs_ctl.gr_ret = r;
-        gfs_ctl.gr_fs = r > 0 ? fs : 0;
         gfs_ctl.gr_root = r > 0 ? root : 0;
         gfs_ctl.gr_ra = ext;
         gfs_ctl.memo_f = mf;
@@ -1309,7 +1353,6 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
     }
     __syncthreads();
     const int64_t n = uni(gfs_ctl.gr_ret);
-    const uint64_t fsize = uni(gfs_ctl.gr_fs);
     const uint64_t root_hint = uni(gfs_ctl.gr_root);
     const uint32_t ra_ext = uni(gfs_ctl.gr_ra);
     const uint32_t memo_f = uni(gfs_ctl.memo_f);   // n > 0 whenever a frame was pinned
@@ -1374,7 +1417,7 @@ __device__ inline size_t gread(int fd, size_t offset, size_t size, uchar* buffer
             __syncthreads();
             if (uni(gfs_ctl.err)) { err = uni(gfs_ctl.err); break; }
             GFS_TRACE(1);
-            fill_from_host(g, cnt, pb, fsize);
+            fill_from_host(g, cnt, pb);
             GFS_TRACE(2);
             // copy every requested pinned page (hit or freshly filled) into the caller's buffer
             const uint32_t T = nthreads(), t = tid();
@@ -1454,12 +1497,10 @@ __device__ __noinline__ int pin_pages(gfs_dev* g, int fd, uint64_t pb, uint32_t 
     __syncthreads();
     if (tid() == 0) {
         gfs_memo.fd = -1;   // the batch arrays are reused below
-        gfs_ctl.gr_fs = g->files[fd].size;
         // relaxed: every tree node and leaf word below it is read atomically
         gfs_ctl.gr_root = ld_rlx(&g->files[fd].root);
     }
     __syncthreads();
-    const uint64_t fsize = uni(gfs_ctl.gr_fs);
     const uint64_t root_hint = uni(gfs_ctl.gr_root);
     const uint64_t t0 = now_ticks();
     uint32_t idle = 0;
@@ -1472,7 +1513,7 @@ __device__ __noinline__ int pin_pages(gfs_dev* g, int fd, uint64_t pb, uint32_t 
         }
         __syncthreads();
         if (uni(gfs_ctl.err)) { err = uni(gfs_ctl.err); break; }
-        fill_from_host(g, cnt, pb, fsize);
+        fill_from_host(g, cnt, pb);
         if (wave() == 0) {
             const uint32_t ln = lane();
             if (gfs_ctl.err) fail_own(g, fd, pb, cnt);
@@ -1546,9 +1587,20 @@ __device__ inline void unpin_pages(gfs_dev* g, uint32_t cnt) {
 }
 } // namespace gfs
 
+namespace gfs {
+__device__ inline int flush_file(gfs_dev* g, int fd, uint64_t min_size);
+// flush_file out of line, for gwrite's write-back of its own file when the
+// cache runs short of clean frames (a rare path: gwrite's loop stays small).
+__device__ __noinline__ int flush_file_stalled(gfs_dev* g, int fd, uint64_t upto) {
+    return flush_file(g, fd, upto);
+}
+} // namespace gfs
+
 // Writes [offset, offset+size) of `buffer` through the page cache
 // (libgpufs/fs_calls.cu:739-802).  Pages are fetched from the host first
 // unless the file is O_GWRONCE (zero-filled) or the page is fully overwritten.
+// When more than three quarters of the frames are dirty, or no frame is left
+// for a page, the file's dirty pages are written back first.
 __device__ inline size_t gwrite(int fd, size_t offset, size_t size, uchar* buffer) {
     using namespace gfs;
     gfs_dev* g = gfs_g;
@@ -1558,12 +1610,10 @@ __device__ inline size_t gwrite(int fd, size_t offset, size_t size, uchar* buffe
         if (!fd_valid(g, fd) || fst_state(ld_acq(&g->files[fd].st)) != GFS_FS_OPEN) r = -EBADF;
         else if ((g->files[fd].flags & O_ACCMODE) == O_RDONLY) r = -EBADF;
         gfs_ctl.ret = r;
-        gfs_ctl.a = r > 0 ? g->files[fd].size : 0;
         gfs_ctl.flag = r > 0 ? (uint32_t)g->files[fd].flags : 0;
     }
     __syncthreads();
     const int64_t n = uni(gfs_ctl.ret);
-    const uint64_t fsize0 = uni(gfs_ctl.a);
     const bool wronce = (uni(gfs_ctl.flag) & (O_CREAT | O_ACCMODE)) == (uint32_t)O_GWRONCE;
     if (n <= 0) return (size_t)n;
     const uint32_t L = g->log_page;
@@ -1574,12 +1624,19 @@ __device__ inline size_t gwrite(int fd, size_t offset, size_t size, uchar* buffe
     for (uint64_t pb = p_first; pb <= p_last && !err; pb += bmax) {
         const uint32_t cnt = (uint32_t)((p_last - pb + 1) < bmax ? (p_last - pb + 1) : bmax);
         uint64_t pending = cnt == 64 ? ~0ull : ((1ull << cnt) - 1);
-        // pages that need no host data: O_GWRONCE, fully overwritten, or wholly past EOF
+        if (wg_dirty_high(g)) {
+            err = flush_file_stalled(g, fd, offset + n);
+            if (err) break;
+        }
+        // Pages that need no host data: O_GWRONCE (OR-merged on disk) or
+        // fully overwritten.  A page past the end of file is read too: other
+        // writers may have written it back and had it evicted since this
+        // call read the size, and the page is written back whole.
         uint64_t nohost = 0;
         for (uint32_t i = 0; i < cnt; ++i) {
             uint64_t poff = (pb + i) << L;
             bool full = offset <= poff && offset + n >= poff + PS;
-            if (wronce || full || poff >= fsize0) nohost |= 1ull << i;
+            if (wronce || full) nohost |= 1ull << i;
         }
         uint32_t idle = 0;
         uint64_t t0 = now_ticks();
@@ -1591,7 +1648,7 @@ __device__ inline size_t gwrite(int fd, size_t offset, size_t size, uchar* buffe
             }
             __syncthreads();
             if (uni(gfs_ctl.err)) { err = uni(gfs_ctl.err); break; }
-            fill_from_host(g, cnt, pb, fsize0);
+            fill_from_host(g, cnt, pb);
             const uint32_t T = nthreads(), t = tid();
             // zero the never-filled part of host-less OWN pages, then copy the caller's bytes
             const uint64_t posted = uni(gfs_ctl.own);
@@ -1615,11 +1672,12 @@ __device__ inline size_t gwrite(int fd, size_t offset, size_t size, uchar* buffe
             __syncthreads();
             if (wave() == 0) {
                 const uint32_t ln = lane();
+                bool nd = false;
                 if (ln < cnt) {
                     uint32_t k = gfs_ctl.kind[ln];
-                    if (k == GFS_K_OWN || k == GFS_K_HIT)
-                        __hip_atomic_fetch_or(&g->meta[gfs_ctl.frame[ln]], GFS_FL_DIRTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (k == GFS_K_OWN || k == GFS_K_HIT) nd = set_dirty(g, gfs_ctl.frame[ln]);
                 }
+                dirty_tally(g, nd, true);
                 if (gfs_ctl.err) fail_own(g, fd, pb, cnt);
                 finish_batch(g, cnt);
                 uint64_t waitm = __ballot(ln < cnt && gfs_ctl.kind[ln] == GFS_K_WAIT);
@@ -1630,6 +1688,16 @@ __device__ inline size_t gwrite(int fd, size_t offset, size_t size, uchar* buffe
             uint64_t nxt = uni(gfs_ctl.a);
             if (nxt == pending) {
                 if (++idle > 4) backoff();
+                // Frames run out when the cache is full of dirty pages, which
+                // the eviction sweep never takes (the reference's allocator
+                // does not either, mallocfree.cu:81-226), and the check at the
+                // batch's start cannot rule that out while other writers are
+                // busy.  This workgroup writes back the file's dirty pages so
+                // far, its own among them, which makes their frames evictable.
+                if ((idle & 31) == 16) {
+                    const int fr = flush_file_stalled(g, fd, offset + n);
+                    if (fr) { err = fr; break; }
+                }
                 if ((idle & 63) == 0 && wg_timed_out(g, t0)) { err = -ETIMEDOUT; break; }
             } else {
                 idle = 0;
@@ -1651,20 +1719,29 @@ namespace gfs {
 // The flag is cleared before the page is copied out: a gwrite after that
 // point sets it again, so its bytes are written back by a later flush.
 __device__ inline uint32_t take_dirty(gfs_dev* g, uint64_t v) {
-    if (v == 0 || v == GFS_LEAF_RESERVED) return GFS_NOFRAME;
-    const uint32_t f = leaf_frame(v);
-    const uint64_t old = fadd_rlx(&g->meta[f], 1ull);
-    if (meta_gen(old) == leaf_gen(v) && meta_state(old) == GFS_F_VALID && (old & GFS_FL_DIRTY)) {
-        __hip_atomic_fetch_and(&g->meta[f], ~GFS_FL_DIRTY, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        return f;
+    uint32_t got = GFS_NOFRAME;
+    if (v != 0 && v != GFS_LEAF_RESERVED) {
+        const uint32_t f = leaf_frame(v);
+        // a plain look first: clean pages cost no atomic
+        const uint64_t m = ld_rlx(&g->meta[f]);
+        if (meta_gen(m) == leaf_gen(v) && meta_state(m) == GFS_F_VALID && (m & GFS_FL_DIRTY)) {
+            const uint64_t old = fadd_rlx(&g->meta[f], 1ull);
+            if (meta_gen(old) == leaf_gen(v) && meta_state(old) == GFS_F_VALID &&
+                (__hip_atomic_fetch_and(&g->meta[f], ~GFS_FL_DIRTY, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) &
+                 GFS_FL_DIRTY))
+                got = f;
+            else
+                fadd_rlx(&g->meta[f], ~0ull);
+        }
     }
-    fadd_rlx(&g->meta[f], ~0ull);
-    return GFS_NOFRAME;
+    dirty_tally(g, got != GFS_NOFRAME, false);
+    return got;
 }
 // A page whose write-back did not happen is marked dirty again, so it is
 // neither evicted nor forgotten: a later gfsync / gclose retries it.
 __device__ __forceinline__ void redirty(gfs_dev* g, uint32_t f) {
-    __hip_atomic_fetch_or(&g->meta[f], GFS_FL_DIRTY, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t old = __hip_atomic_fetch_or(&g->meta[f], GFS_FL_DIRTY, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    dirty_tally(g, !(old & GFS_FL_DIRTY), true);
 }
 
 // Write back every dirty page of `fd` (workgroup-collective).  Pages of
@@ -1675,16 +1752,22 @@ __device__ __forceinline__ void redirty(gfs_dev* g, uint32_t f) {
 // one shared completion mask, and thread 0 waits for that mask once.  A dirty
 // page that found no free slot is taken in a later round.  Returns 0 or the
 // first error; pages whose write failed stay dirty.
-__device__ inline int flush_file(gfs_dev* g, int fd) {
+__device__ inline int flush_file(gfs_dev* g, int fd, uint64_t min_size = 0) {
     gfs_file* file = &g->files[fd];
     const bool wronce = (file->flags & (O_CREAT | O_ACCMODE)) == O_GWRONCE;
-    const uint64_t size = file->size;
+    const uint64_t size = file->size > min_size ? file->size : min_size;
     const int32_t host_fd = file->host_fd;
     const uint64_t npages = (size + g->page_size - 1) >> g->log_page;
     const uint32_t T = nthreads(), t = tid();
     if (t == 0) gfs_memo.fd = -1;   // the batch arrays are reused below
     int err = 0;
-    for (uint64_t leaf = 0; (leaf << GFS_FANOUT_LOG) < npages && !err; ++leaf) {
+    // Workgroups of one grid that write back the same file start at leaves
+    // spread in proportion to their index (where a grid that writes its share
+    // of a file by index has its own pages), then wrap round.
+    const uint64_t nleaves = (npages + GFS_FANOUT - 1) >> GFS_FANOUT_LOG;
+    const uint64_t first = nleaves ? (uint64_t)blockIdx.x * nleaves / gridDim.x : 0;
+    for (uint64_t li = 0; li < nleaves && !err; ++li) {
+        const uint64_t leaf = first + li < nleaves ? first + li : first + li - nleaves;
         const uint64_t base = leaf << GFS_FANOUT_LOG;
         const uint64_t in_leaf = npages - base < 64 ? npages - base : 64;
         uint64_t pending = in_leaf == 64 ? ~0ull : ((1ull << in_leaf) - 1);
@@ -2052,7 +2135,8 @@ __device__ inline volatile void* gmmap(void* addr, size_t size, int prot, int fl
     const bool writable = (oflags & O_ACCMODE) != O_RDONLY;
     const bool wronce = (oflags & (O_CREAT | O_ACCMODE)) == O_GWRONCE;
     const uint64_t page = (uint64_t)offset >> g->log_page;
-    const bool nohost = wronce || (page << g->log_page) >= fsize;
+    // a writable mapping reads a page past the end of file too (see gwrite)
+    const bool nohost = wronce || (!writable && (page << g->log_page) >= fsize);
     uint64_t t0 = now_ticks();
     for (;;) {
         __syncthreads();
@@ -2062,7 +2146,7 @@ __device__ inline volatile void* gmmap(void* addr, size_t size, int prot, int fl
         }
         __syncthreads();
         if (uni(gfs_ctl.err)) return GMAP_FAILED;
-        fill_from_host(g, 1, page, fsize);
+        fill_from_host(g, 1, page);
         if (uni((uint32_t)gfs_ctl.kind[0]) == GFS_K_OWN && (uni(gfs_ctl.own) & 1ull) == 0) {
             uint8_t* fp = frame_ptr(g, gfs_ctl.frame[0]);
             for (uint64_t k = tid(); k < (g->page_size >> 4); k += nthreads()) *(v4u*)(fp + 16 * k) = v4u{0, 0, 0, 0};
@@ -2086,7 +2170,7 @@ __device__ inline volatile void* gmmap(void* addr, size_t size, int prot, int fl
                 gfs_ctl.ret = 0;
             }
             if (gfs_ctl.ret == 1 && writable) {
-                __hip_atomic_fetch_or(&g->meta[f], GFS_FL_DIRTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                dirty_tally(g, set_dirty(g, f), true);
                 uint32_t end = (uint32_t)(((uint64_t)offset & (g->page_size - 1)) + size);
                 atomicMax(&g->fbytes[f], end);
                 g->files[fd].dirty = 1;
@@ -2154,7 +2238,8 @@ __device__ inline volatile void* gmmap_warp(void* addr, size_t size, int prot, i
     const bool writable = (oflags & O_ACCMODE) != O_RDONLY;
     const bool wronce = (oflags & (O_CREAT | O_ACCMODE)) == O_GWRONCE;
     const uint64_t page = (uint64_t)offset >> L;
-    const bool nohost = wronce || (page << L) >= fsize;
+    // a writable mapping reads a page past the end of file too (see gwrite)
+    const bool nohost = wronce || (!writable && (page << L) >= fsize);
     const uint64_t t0 = now_ticks();
     uint32_t f = GFS_NOFRAME;
     for (;;) {
@@ -2229,7 +2314,7 @@ __device__ inline volatile void* gmmap_warp(void* addr, size_t size, int prot, i
                     if (!nohost) {
                         const uint64_t poff = page << L;
                         uint64_t want = fsize > poff ? fsize - poff : 0;
-                        if (want > g->page_size) want = g->page_size;
+                        if (writable || want > g->page_size) want = g->page_size;   // as in resolve_batch
                         int w = 0;
                         if (ln == 0) {
                             sys_st(&g->cq[s], 0ull);
@@ -2237,10 +2322,13 @@ __device__ inline volatile void* gmmap_warp(void* addr, size_t size, int prot, i
                             count(&g->ctr.rpc_reads, 1);
                             w = rpc_wait(g, (uint32_t)s, 1ull);
                             r = w ? (int64_t)w : sys_ld(&g->slots[s].result);
-                            if (r >= 0 && (uint64_t)r != want) r = -EIO;
+                            if (r >= 0 && (uint64_t)r > want) r = -EIO;
                         }
                         r = uni(r);
-                        if (r >= 0) copy_seg<true, false>(frame_ptr(g, nf), staging_ptr(g, (uint32_t)s), (uint64_t)r, ln, 64);
+                        if (r >= 0) {
+                            copy_seg<true, false>(frame_ptr(g, nf), staging_ptr(g, (uint32_t)s), (uint64_t)r, ln, 64);
+                            if ((uint64_t)r < g->page_size) zero_tail(frame_ptr(g, nf), (uint64_t)r, g->page_size, ln, 64);
+                        }
                         // the wave's staging loads have all returned (their data was stored);
                         // a request that timed out keeps its slot (rpc_quarantine)
                         if (ln == 0) {
@@ -2283,7 +2371,7 @@ __device__ inline volatile void* gmmap_warp(void* addr, size_t size, int prot, i
         backoff();
     }
     if (writable && ln == 0) {
-        __hip_atomic_fetch_or(&g->meta[f], GFS_FL_DIRTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        dirty_tally(g, set_dirty(g, f), true);
         atomicMax(&g->fbytes[f], (uint32_t)(((uint64_t)offset & (g->page_size - 1)) + size));
         g->files[fd].dirty = 1;
         atomicMax((unsigned long long*)&g->files[fd].size, (unsigned long long)(offset + size));
@@ -2331,7 +2419,11 @@ __device__ inline int gmsync(volatile void* addr, size_t length, int flags) {
             backoff();
         }
         gfs_ctl.slot[0] = s < 0 ? 0xFFFFFFFFu : (uint32_t)s;
-        if (s >= 0) __hip_atomic_fetch_and(&g->meta[f], ~GFS_FL_DIRTY, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (s >= 0)
+            dirty_tally(g,
+                        (__hip_atomic_fetch_and(&g->meta[f], ~GFS_FL_DIRTY, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) &
+                         GFS_FL_DIRTY) != 0,
+                        false);
     }
     __syncthreads();
     const uint32_t s = uni(gfs_ctl.slot[0]);

@@ -4,7 +4,8 @@
 // gread of a disjoint byte range per workgroup, the access pattern of
 // samples/memcopy/cp.cu:73-108 scaled to large ranges); gpufs_launch_read /
 // gpufs_launch_write run a single open/read|write/close sequence in one
-// workgroup for the API tests.
+// workgroup for the API tests; gpufs_launch_gwrite_ranges is the sequential
+// write workload (one gopen + gwrite + gfsync + gclose per workgroup).
 #include "gpufs.h"
 
 static int check_threads(uint32_t threads) {
@@ -37,6 +38,36 @@ extern "C" int gpufs_launch_gread_ranges(gpufs_ctx* ctx, const char* path_dev, u
     if ((v = gpufs_bind(ctx))) return v;
     hipLaunchKernelGGL(k_gread_ranges, dim3(n_ranges), dim3(threads), 0, pick_stream(ctx, stream), path_dev,
                        range_bytes, (uint8_t*)dst_dev, bytes_out_dev);
+    return hipGetLastError() == hipSuccess ? 0 : -EIO;
+}
+
+// The write-side twin of k_gread_ranges: workgroup i opens the file, writes
+// range i of `src` at offset i * range, optionally gfsyncs it, and closes it;
+// out[i] = bytes written or the first error (gwrite, then gfsync, then gclose).
+__global__ __launch_bounds__(1024) void k_gwrite_ranges(const char* path, int flags, uint64_t range,
+                                                        const uint8_t* src, int do_fsync, int64_t* out) {
+    int fd = gopen(path, flags);
+    if (fd < 0) {
+        if (gfs::tid() == 0) out[blockIdx.x] = fd;
+        return;
+    }
+    const size_t r = gwrite(fd, (size_t)blockIdx.x * range, range, (uchar*)(src + (size_t)blockIdx.x * range));
+    const int s = (do_fsync && (int64_t)r >= 0) ? gfsync(fd) : 0;
+    const int c = gclose(fd);
+    if (gfs::tid() == 0) out[blockIdx.x] = (int64_t)r < 0 ? (int64_t)r : (s < 0 ? (int64_t)s : (c < 0 ? (int64_t)c : (int64_t)r));
+}
+
+extern "C" int gpufs_launch_gwrite_ranges(gpufs_ctx* ctx, const char* path_dev, int32_t open_flags,
+                                          uint32_t n_ranges, uint64_t range_bytes, const void* src_dev,
+                                          int32_t do_fsync, int64_t* bytes_out_dev, uint32_t threads,
+                                          void* stream) {
+    if (!ctx || !path_dev || !src_dev || !bytes_out_dev || n_ranges == 0) return -EINVAL;
+    if ((open_flags & O_ACCMODE) == O_RDONLY) return -EINVAL;
+    int v = check_threads(threads);
+    if (v) return v;
+    if ((v = gpufs_bind(ctx))) return v;
+    hipLaunchKernelGGL(k_gwrite_ranges, dim3(n_ranges), dim3(threads), 0, pick_stream(ctx, stream), path_dev,
+                       open_flags, range_bytes, (const uint8_t*)src_dev, do_fsync, bytes_out_dev);
     return hipGetLastError() == hipSuccess ? 0 : -EIO;
 }
 

@@ -222,6 +222,7 @@ extern "C" int gpufs_init(const gpufs_config* cfg_in, gpufs_ctx** out) {
     h.reclaim_hand = 0;
     h.open_gate = 0;
     h.node_free = 0;
+    h.dirty_frames = 0;
     int wall_khz = 100000;
     if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || wall_khz <= 0)
         wall_khz = 100000;
@@ -336,6 +337,7 @@ extern "C" int gpufs_cache_drop(gpufs_ctx* c) {
     cur.clock_hand = 0;
     cur.node_next = 1;
     cur.node_free = 0;   // the nodes were zeroed: no tree and no free list remains
+    cur.dirty_frames = 0;
     HIP_TRY(hipMemcpyAsync(c->dev, &cur, sizeof(gfs_dev), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
@@ -425,6 +427,11 @@ extern "C" int gpufs_check_invariants(gpufs_ctx* c) {
         if (meta[f] & GFS_REF_MASK) return -1001;   // a frame is still pinned
     for (uint32_t f = 0; f < h.n_frames; ++f)
         if (((meta[f] >> GFS_ST_SHIFT) & 0xF) == GFS_F_FILLING) return -1002;   // a fill never finished
+    {
+        long long nd = 0;
+        for (uint32_t f = 0; f < h.n_frames; ++f) nd += (meta[f] & GFS_FL_DIRTY) ? 1 : 0;
+        if (nd != cur.dirty_frames) return -1014;
+    }
     std::vector<uint8_t> seen(h.n_frames, 0);
     std::vector<uint8_t> nseen(nn, 0);   // nodes in a tree or on the free list
     for (uint32_t fi = 0; fi < nf; ++fi) {
@@ -541,6 +548,7 @@ extern "C" const char* gpufs_strerror(int code) {
     case -1011: return "invariant: open gate not clear";
     case -1012: return "invariant: file table and name index disagree";
     case -1013: return "invariant: radix node lost, shared or on the free list twice";
+    case -1014: return "invariant: dirty-frame count differs from the frames marked dirty";
     default: break;
     }
     if (code < 0 && code > -4096) return strerror(-code);

@@ -129,6 +129,17 @@ int gpufs_launch_gread_ranges(gpufs_ctx* ctx, const char* path_dev, uint32_t n_r
                               uint64_t range_bytes, void* dst_dev, int64_t* bytes_out_dev,
                               uint32_t threads, void* stream);
 
+/* Sequential gwrite, the write side of the same pattern (the gwrite loop of
+ * samples/memcopy/cp.cu:73-108 with one range per workgroup): workgroup w
+ * gopens `path_dev` with open_flags (O_GWRONLY / O_GRDWR / O_GWRONCE, with
+ * O_GCREAT as needed), gwrites src_dev + w*range_bytes at w*range_bytes,
+ * gfsyncs when do_fsync, and gcloses.  bytes_out_dev[w] = bytes written, or
+ * the first error of gwrite, gfsync, gclose.  -EINVAL for a read-only mode. */
+int gpufs_launch_gwrite_ranges(gpufs_ctx* ctx, const char* path_dev, int32_t open_flags,
+                               uint32_t n_ranges, uint64_t range_bytes, const void* src_dev,
+                               int32_t do_fsync, int64_t* bytes_out_dev, uint32_t threads,
+                               void* stream);
+
 /* grep_text-style scan (samples/grep_text/grep_text_gpu.cu:193-387, scan
  * semantics :79,139-160, stitching :298-318): file f is names_dev +
  * f*name_stride; its whole-word match counts of the n_words dictionary words

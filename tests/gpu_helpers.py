@@ -115,6 +115,21 @@ def gread_ranges(ctx, path, n_ranges, range_bytes, threads=1024, dst=None):
     return out.cpu().numpy(), dst
 
 
+def gwrite_ranges(ctx, path, flags, n_ranges, range_bytes, src, fsync=True, threads=1024):
+    """gpufs_launch_gwrite_ranges: workgroup w writes src[w*range:(w+1)*range]
+    at w*range.  `src` is a device uint8 tensor.  Returns the per-workgroup
+    results."""
+    torch = torch_cuda()
+    name = dev_names([path])
+    out = dev_zeros(n_ranges, torch.int64)
+    rc = _lib.lib().gpufs_launch_gwrite_ranges(ctx.h, ctypes.c_void_p(name.data_ptr()), flags, n_ranges,
+                                               range_bytes, ctypes.c_void_p(src.data_ptr()), 1 if fsync else 0,
+                                               ctypes.c_void_p(out.data_ptr()), threads, None)
+    _lib.check("gpufs_launch_gwrite_ranges", rc)
+    finish(ctx)
+    return out.cpu().numpy()
+
+
 def grep(ctx, paths, dict_words, chunk, threads=1024, parts=None, min_part=0):
     """gpufs_launch_grep, or gpufs_launch_grep_split when `parts` is given.
     The count rows start as garbage: the call writes them entirely."""

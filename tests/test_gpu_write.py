@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from gpufs_amd import O_GRDWR, O_GWRONCE, O_GWRONLY, O_GCREAT
-from tests.gpu_helpers import make_ctx, read_file, say, write_file
+from tests.gpu_helpers import dev_bytes, gwrite_ranges, make_ctx, read_file, say, write_file
 
 pytestmark = pytest.mark.gpu
 
@@ -83,3 +83,59 @@ def test_wronce_or_merges_with_disk(tmp_path):
     got = p.read_bytes()
     assert got[:2000] == bytes([0x03, 0x11, 0x80, 0xF0]) * 500
     assert got[2000:] == bytes([0x01, 0x10, 0x00, 0xF0]) * 500
+
+
+def test_gwrite_larger_than_cache(tmp_path):
+    """One gwrite of three times the page cache, no gfsync: the cache fills
+    with dirty pages, which the eviction sweep never takes, so the writer
+    writes its file back to free frames and the call completes."""
+    say("test_gwrite_larger_than_cache")
+    p = tmp_path / "big.bin"
+    data = np.random.default_rng(5).integers(0, 256, size=(12 << 20) + 777, dtype=np.uint8).tobytes()
+    with make_ctx(page_size=65536, cache_bytes=4 << 20) as ctx:
+        r = write_file(ctx, str(p), 0, data, O_GWRONLY | O_GCREAT, threads=1024)
+        assert r[0] == len(data) and r[1] == len(data) and r[2] == 0, r
+        assert p.read_bytes() == data
+        assert ctx.check_invariants() == 0
+        assert ctx.stats()["bytes_written"] == len(data)
+
+
+@pytest.mark.parametrize("fsync", [True, False])
+def test_gwrite_ranges_new_file(tmp_path, fsync):
+    """64 workgroups write a new file, one ragged range each (ranges share
+    pages with their neighbours), with and without gfsync, through a cache
+    large enough to hold the file."""
+    say(f"test_gwrite_ranges_new_file fsync={fsync}")
+    p = tmp_path / "ranges.bin"
+    n, rng = 64, (256 << 10) + 4099
+    data = np.random.default_rng(6).integers(0, 256, size=n * rng, dtype=np.uint8)
+    with make_ctx(page_size=65536, cache_bytes=64 << 20) as ctx:
+        out = gwrite_ranges(ctx, str(p), O_GWRONLY | O_GCREAT, n, rng, dev_bytes(data.tobytes()), fsync=fsync)
+        assert (out == rng).all(), out
+        assert p.read_bytes() == data.tobytes()
+        assert ctx.check_invariants() == 0
+
+
+@pytest.mark.parametrize("mode", ["wronly", "wronce"])
+def test_gwrite_ranges_through_small_cache(tmp_path, mode):
+    """48 workgroups write 24 MiB through an 8 MiB cache without gfsync:
+    writers write back when three quarters of the frames are dirty, or when
+    a page finds no frame.  O_GWRONCE OR-merges with the bytes on disk."""
+    say(f"test_gwrite_ranges_through_small_cache {mode}")
+    p = tmp_path / "small_cache.bin"
+    n, rng = 48, (512 << 10) + 8191
+    rs = np.random.default_rng(7)
+    data = rs.integers(0, 256, size=n * rng, dtype=np.uint8)
+    if mode == "wronce":
+        base = rs.integers(0, 256, size=n * rng, dtype=np.uint8)
+        base.tofile(str(p))
+        flags, expect = O_GWRONCE, (base | data).tobytes()
+    else:
+        flags, expect = O_GWRONLY | O_GCREAT, data.tobytes()
+    with make_ctx(page_size=65536, cache_bytes=8 << 20) as ctx:
+        out = gwrite_ranges(ctx, str(p), flags, n, rng, dev_bytes(data.tobytes()), fsync=False)
+        assert (out == rng).all(), out
+        assert p.read_bytes() == expect
+        assert ctx.check_invariants() == 0
+        s = ctx.stats()
+        assert s["bytes_written"] >= n * rng, s
