@@ -1793,12 +1793,10 @@ __device__ inline int flush_file(gfs_dev* g, int fd, uint64_t min_size = 0) {
                 const bool post = f != GFS_NOFRAME;
                 const uint64_t postm = __ballot(post);
                 uint32_t bytes = 0;
-                if (post) {
-                    const uint64_t poff = (base + ln) << g->log_page;
-                    const uint64_t b = g->fbytes[f];
-                    const uint64_t lim = size > poff ? size - poff : 0;
-                    bytes = (uint32_t)(b < lim ? b : lim);
-                }
+                // The frame's extent, not clipped at the file size: the size
+                // grows only when a gwrite ends, so another writer's bytes in
+                // the page may lie past it, and the dirty flag is already off.
+                if (post) bytes = g->fbytes[f];
                 fence_acq_agent();
                 gfs_ctl.kind[ln] = post ? GFS_K_OWN : GFS_K_DONE;
                 gfs_ctl.frame[ln] = f;
@@ -1898,8 +1896,7 @@ __device__ inline int flush_file_single(gfs_dev* g, int fd) {
             const uint32_t f = take_dirty(g, ld_rlx(&g->nodes[node * GFS_FANOUT + i]));
             if (f == GFS_NOFRAME) continue;
             const uint64_t poff = ((leaf << GFS_FANOUT_LOG) + i) << g->log_page;
-            const uint64_t b = g->fbytes[f], lim = size > poff ? size - poff : 0;
-            const uint64_t bytes = b < lim ? b : lim;
+            const uint64_t bytes = g->fbytes[f];   // the frame's extent (see flush_file)
             int s = -1;
             const uint64_t t0 = now_ticks();
             while ((s = rpc_claim(g, true, blockIdx.x)) < 0 && now_ticks() - t0 <= g->wait_ticks) backoff();
@@ -2409,7 +2406,6 @@ __device__ inline int gmsync(volatile void* addr, size_t length, int flags) {
     gfs_file* file = &g->files[fd];
     const bool wronce = (file->flags & (O_CREAT | O_ACCMODE)) == O_GWRONCE;
     uint64_t poff = page << g->log_page;
-    uint64_t bytes = g->fbytes[f];
     __syncthreads();
     if (tid() == 0) {
         int s = -1;
@@ -2419,15 +2415,20 @@ __device__ inline int gmsync(volatile void* addr, size_t length, int flags) {
             backoff();
         }
         gfs_ctl.slot[0] = s < 0 ? 0xFFFFFFFFu : (uint32_t)s;
-        if (s >= 0)
+        if (s >= 0) {
             dirty_tally(g,
                         (__hip_atomic_fetch_and(&g->meta[f], ~GFS_FL_DIRTY, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) &
                          GFS_FL_DIRTY) != 0,
                         false);
+            // the extent is read after the flag is cleared: a writer that
+            // grew it before setting the flag is seen, a later one re-dirties
+            gfs_ctl.ret = (int64_t)g->fbytes[f];
+        }
     }
     __syncthreads();
     const uint32_t s = uni(gfs_ctl.slot[0]);
     if (s == 0xFFFFFFFFu) return -ETIMEDOUT;
+    const uint64_t bytes = (uint64_t)uni(gfs_ctl.ret);
     wg_copy(staging_ptr(g, s), frame_ptr(g, (uint32_t)f), bytes);
     __threadfence_system();
     __syncthreads();

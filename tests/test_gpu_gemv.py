@@ -34,8 +34,12 @@ def test_gemv_mmap_with_eviction(tmp_path, orc, page_kib, cache_mib):
         assert (cs == _word_sums(A, rpw)).all()
         err = np.abs(y.astype(np.float64) - y64).max() / np.abs(y64).max()
         assert err <= 1e-5, err
-        y2, st2, _ = gemv_mmap(ctx, p, rows, cols, x, rpw)
-        assert (st2 == 0).all() and (y2 == y).all()
+        y2, st2, cs2 = gemv_mmap(ctx, p, rows, cols, x, rpw)
+        bad_wg = np.nonzero(cs2 != _word_sums(A, rpw))[0]
+        bad_rows = np.nonzero(y2 != y)[0]
         s = ctx.stats()
+        assert (st2 == 0).all() and bad_wg.size == 0 and bad_rows.size == 0, (
+            f"second pass: workgroups with wrong mapped bytes {bad_wg.tolist()[:16]}, "
+            f"rows that differ {bad_rows.tolist()[:16]} of {bad_rows.size}, stats {s}")
         assert s["evictions"] > 0
         assert ctx.check_invariants() == 0
