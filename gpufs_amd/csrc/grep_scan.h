@@ -571,6 +571,7 @@ __device__ inline int grep_part_pinned(grep_lds& S, int fd, uint32_t chunk, uint
                 }
             }
         }
+        GFS_TRACE(50);
         grep_scan_window_lds((grep_lds_as3*)&S, dsz, skip, nwbase, pb, more ? (nlen + 15) >> 4 : 0u);
         if (!more) break;
         cursor = ncur;
@@ -737,15 +738,18 @@ __device__ __noinline__ int grep_tail(grep_lds& S, int fd, uint64_t from, uint64
     return 0;
 }
 
-// Per-file state of a split scan, zeroed before the launch.
+// Per-file state of a split scan.  Nothing needs clearing before a launch:
+// part 0 resets the other fields (and zeroes the file's count row) before it
+// publishes its decision, and the decision carries the launch's epoch, so a
+// decision left from an earlier launch in the same memory is never taken for
+// this one's.
 struct grep_split_state {
     uint32_t arrived;   // parts done
     uint32_t longrun;   // some part met a run of 32 or more word bytes at a boundary
     int32_t err;        // first error of a part
     uint32_t cut;       // part 0's chain: some reference window cuts a token
     uint64_t c_last;    // part 0's chain: start of the reference's last window
-    uint32_t kf;        // part 0's decision: K_f + 1 (0 = not decided yet)
-    uint32_t pad;
+    uint64_t tag;       // part 0's decision: epoch << 32 | (K_f + 1)
 };
 
 // Part boundaries: part 0 also walks the reference's window chain before its
@@ -806,7 +810,8 @@ __device__ __noinline__ int grep_split_finish(grep_lds& S, int fd, uint32_t nw, 
 // (grep_chain_last).  Every one of the `parts` workgroups of the file arrives
 // exactly once, after adding its counts into the caller-zeroed row (a
 // workgroup whose gopen failed records the error); the last to arrive
-// completes the row (grep_split_finish) and writes status[0].
+// completes the row (grep_split_finish) and writes status[0].  `epoch`
+// identifies the launch (see grep_split_state); part 0 zeroes the row.
 //
 // `target` is the number of arrivals that completes the file: `parts` when a
 // workgroup is launched for every part slot (gpufs_launch_grep), or 0 for
@@ -822,12 +827,27 @@ __device__ __noinline__ int grep_split_finish(grep_lds& S, int fd, uint32_t nw, 
 template <class OnSize>
 __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t nw, uint32_t chunk, uint32_t* row,
                                        int32_t* status, uint32_t j, uint32_t parts, uint64_t min_part,
-                                       grep_split_state* fs, uint32_t target, OnSize&& on_size) {
+                                       grep_split_state* fs, uint32_t target, uint32_t epoch, OnSize&& on_size) {
     const uint32_t T = gfs::nthreads(), t = gfs::tid();
     __shared__ uint32_t fin_flag;
     const bool single = fs == nullptr;
     for (uint32_t q = t; q < nw; q += T) S.counts[q] = 0;
+    if (!single && j == 0) {
+        // The other parts add into the row and touch the state only after
+        // they have read the decision, which is published with a release
+        // after gopen's barriers: each wave's stores are performed first.
+        for (uint32_t q = t; q < nw; q += T) row[q] = 0;
+        if (t == 0) {
+            fs->arrived = 0;
+            fs->longrun = 0;
+            fs->err = 0;
+            fs->cut = 0;
+            fs->c_last = 0;
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+    }
     const int fd = gopen(name, O_GRDONLY);
+    GFS_TRACE(42);
     uint64_t size = 0, kf = 1, cl = 0;
     int st = fd < 0 ? fd : 0;
     uint32_t lr = 0, cut = 0;
@@ -854,12 +874,16 @@ __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t n
                 warm = S.fsize == 0 || gfs::page_cached(gfs_g, fd, (S.fsize - 1) >> gfs_g->log_page);
             }
             d = (uint32_t)k + 1;
-            if (!single) __hip_atomic_store(&fs->kf, d, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (!single)
+                __hip_atomic_store(&fs->tag, ((uint64_t)epoch << 32) | d, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             const uint64_t w0 = gfs::now_ticks();
-            while ((d = __hip_atomic_load(&fs->kf, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) == 0 &&
-                   gfs::now_ticks() - w0 < gfs_g->wait_ticks)
+            for (;;) {
+                const uint64_t tg = __hip_atomic_load(&fs->tag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                d = (tg >> 32) == epoch ? (uint32_t)tg : 0u;
+                if (d || gfs::now_ticks() - w0 >= gfs_g->wait_ticks) break;
                 __builtin_amdgcn_s_sleep(8);
+            }
             // no decision within the wait limit: this part scans nothing and
             // the file reports -ETIMEDOUT
         }
@@ -874,6 +898,7 @@ __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t n
         if (fd >= 0) size = gfs::uni((uint64_t)S.fsize);
     }
     __syncthreads();
+    GFS_TRACE(43);
     on_size((uint32_t)kf);
     if (fd >= 0) {
         // Part 0 of a cached file that fits one pin batch pins all of it once
@@ -888,6 +913,7 @@ __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t n
                 if (!st) held = (uint32_t)np;
             }
         }
+        GFS_TRACE(44);
         if (j == 0 && size && warm && !st) {
             st = grep_chain_last(S, fd, size, chunk, &cl, &cut, held != 0);
             if (t == 0 && !single) {
@@ -895,6 +921,7 @@ __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t n
                 fs->cut = cut;
             }
         }
+        GFS_TRACE(45);
         if (j < kf && size && !st) {
             const uint64_t pb = split_point(size, j, kf), pe = split_point(size, j + 1, kf);
             // a cached file is read from pinned frames, a cold one with gread
@@ -907,9 +934,15 @@ __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t n
                     const uint32_t v = k == 0xFFFF ? 0u : S.counts[k];
                     if (v) atomicAdd(&row[q], v);
                 }
-                __threadfence();
+                // The adds commute, so no fence orders them for the other
+                // parts; each wave only waits for its adds to be performed, so
+                // that a finisher that overwrites the row (grep_recount) comes
+                // after them.  (A fence per thread here, an L2 write-back and
+                // invalidate in every wave of 256 workgroups, cost ~40 us.)
+                __builtin_amdgcn_s_waitcnt(0);
             }
         }
+        GFS_TRACE(46);
         if (held) gfs::unpin_pages(gfs_g, held);
         if (j == 0 && size && !st && !warm) {
             st = grep_chain_last(S, fd, size, chunk, &cl, &cut, false);
@@ -943,6 +976,7 @@ __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t n
         fin_flag = old + 1 == (target ? target : (uint32_t)kf) ? 1u : 0u;
     }
     __syncthreads();
+    GFS_TRACE(47);
     if (gfs::uni(fin_flag)) {
         // every part's counts are in the row: their adds precede their arrival
         if (t == 0) {
@@ -955,12 +989,15 @@ __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t n
         const uint32_t redo = gfs::uni(fin_flag);
         const uint64_t cl = gfs::uni((uint64_t)S.fsize);
         if (!st && size) st = grep_split_finish(S, fd, nw, chunk, size, row, redo, cl);
+        GFS_TRACE(48);
         const int c = fd >= 0 ? gclose(fd) : 0;
         if (t == 0) *status = st ? st : c;
         __syncthreads();
+        GFS_TRACE(49);
         return;
     }
     if (fd >= 0) gclose(fd);
+    GFS_TRACE(49);
 }
 
 static int grep_args_ok(uint32_t n_files, uint32_t n_words, uint32_t chunk_bytes, uint32_t threads) {

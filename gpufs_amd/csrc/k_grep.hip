@@ -18,6 +18,7 @@
 // hash table of the dictionary (O(bytes)); equal query words share a counter.
 #include "grep_scan.h"
 
+#include <atomic>
 #include <cstring>
 #include <fcntl.h>
 #include <sys/stat.h>
@@ -32,13 +33,15 @@
 __global__ __launch_bounds__(1024) void k_grep(const char* names, uint32_t stride, uint32_t n_files,
                                                const char* dict, uint32_t nw, uint32_t chunk, uint32_t* counts_out,
                                                int32_t* status, uint32_t parts, uint64_t min_part,
-                                               grep_split_state* fstate) {
+                                               grep_split_state* fstate, uint32_t epoch) {
     __shared__ __attribute__((aligned(16))) grep_lds S;
     const uint32_t f = blockIdx.x % n_files, j = blockIdx.x / n_files;
+    GFS_TRACE(40);
     grep_build_dict(S, dict, nw);
+    GFS_TRACE(41);
     [[clang::always_inline]] grep_file_split(S, names + (size_t)f * stride, nw, chunk, counts_out + (size_t)f * nw,
                                              status + f, j, parts, min_part, fstate ? fstate + f : nullptr, parts,
-                                             [](uint32_t) {});
+                                             epoch, [](uint32_t) {});
 }
 
 // ---------------------------------------------------------------------------
@@ -134,7 +137,7 @@ __global__ __launch_bounds__(1024) void k_grep_batched(const char* names, uint32
         if (j == 0 && t == 0 && times) times[2 * (size_t)f] = gfs::now_ticks();
         [[clang::always_inline]] grep_file_split(
             S, names + (size_t)f * stride, nw, chunk, counts_out + (size_t)f * nw, status + f, j, parts, min_part,
-            fs + f, j == 0 ? 0u : kfi, [&](uint32_t kf) {
+            fs + f, j == 0 ? 0u : kfi, 0u, [&](uint32_t kf) {
                 if (j != 0 || t != 0) return;
                 if (kf > 1) {
                     const uint32_t r0 = atomicAdd(&q->tail, kf - 1);
@@ -157,12 +160,15 @@ __global__ void k_zero_u32(unsigned int* p, uint32_t n) {
     if (i < n) p[i] = 0;
 }
 
-// Zeroes the count rows and the per-file split state before a split scan
-// (a fill kernel, so the launch can also be captured in a graph).
-__global__ void k_grep_clear(uint32_t* counts, uint64_t n_counts, grep_split_state* fs, uint32_t n_files) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_counts) counts[i] = 0;
-    if (i < n_files) fs[i] = grep_split_state{0, 0, 0, 0, 0, 0, 0};
+// Epoch of a split launch (grep_split_state::tag): distinct for every launch
+// of the process, never 0 (the batched form's workspace is zeroed per launch
+// and uses epoch 0).
+static uint32_t grep_next_epoch() {
+    static std::atomic<uint32_t> e{0};
+    uint32_t v;
+    while ((v = e.fetch_add(1, std::memory_order_relaxed) + 1) == 0) {
+    }
+    return v;
 }
 
 // Parts per file when the caller leaves it to the launcher: enough workgroups
@@ -182,17 +188,14 @@ static int launch_grep_split(gpufs_ctx* ctx, const char* names_dev, uint32_t nam
                              int32_t* status_dev, uint32_t parts, uint64_t min_part, grep_split_state* fs,
                              uint32_t threads, hipStream_t s) {
     if (parts == 1) {
-        // one workgroup per file: no shared state, nothing to clear first
+        // one workgroup per file: no shared state
         hipLaunchKernelGGL(k_grep, dim3(n_files), dim3(threads), 0, s, names_dev, name_stride, n_files, dict_dev, n_words,
-                           chunk_bytes, counts_dev, status_dev, 1u, min_part, (grep_split_state*)nullptr);
+                           chunk_bytes, counts_dev, status_dev, 1u, min_part, (grep_split_state*)nullptr, 0u);
         return hipGetLastError() == hipSuccess ? 0 : -EIO;
     }
-    const uint64_t n_counts = (uint64_t)n_files * n_words;
-    const uint64_t n_clear = n_counts > n_files ? n_counts : n_files;
-    hipLaunchKernelGGL(k_grep_clear, dim3((uint32_t)((n_clear + 255) / 256)), dim3(256), 0, s, counts_dev, n_counts,
-                       fs, n_files);
+    // nothing to clear first: part 0 of each file zeroes its row and state
     hipLaunchKernelGGL(k_grep, dim3(n_files * parts), dim3(threads), 0, s, names_dev, name_stride, n_files, dict_dev, n_words,
-                       chunk_bytes, counts_dev, status_dev, parts, min_part, fs);
+                       chunk_bytes, counts_dev, status_dev, parts, min_part, fs, grep_next_epoch());
     return hipGetLastError() == hipSuccess ? 0 : -EIO;
 }
 
@@ -226,9 +229,10 @@ extern "C" int gpufs_launch_grep_split(gpufs_ctx* ctx, const char* names_dev, ui
 // code under graph capture, serving loops).  Its per-file state lives in
 // `scratch`, a file-scope device array; every GPU holds its own copy at its
 // own address, which gpufs_init looks up once per device
-// (grep_scratch_setup).  Calls on one GPU share the array: each call clears
-// the entries it uses with a fill kernel first, and calls that overlap in
-// time on one GPU race.
+// (grep_scratch_setup).  Calls on one GPU share the array: an entry left by
+// an earlier call is never taken for this call's (each launch has its own
+// epoch, and part 0 of a file resets the file's entry before the other parts
+// read it), and calls that overlap in time on one GPU race.
 #define GREP_NOALLOC_MAX_FILES 65536u
 #define GREP_MAX_DEVICES 64
 static __device__ __align__(256) grep_split_state scratch[GREP_NOALLOC_MAX_FILES];
