@@ -5,6 +5,8 @@
 // policy type P supplies the operations:
 //   uint64_t P::fadd(uint64_t* w, uint64_t v)    fetch-add, acquire-release
 //   uint64_t P::load(uint64_t* w)                acquire load
+//   uint64_t P::load_rlx(uint64_t* w)            relaxed load (polls: the read-modify-write that
+//                                                follows a change acquires)
 //   bool     P::cas(uint64_t* w, uint64_t& exp, uint64_t v)   updates exp on failure
 //   bool     P::expired()                        the caller's wait limit passed
 //   void     P::pause()                          brief back-off
@@ -62,10 +64,12 @@ template <class P> GFS_HD int fst_open(uint64_t* w, P& p) {
             if (p.cas(w, cur, cur + fst_delta(GFS_FS_CLOSED, GFS_FS_OPENING) + FST_ONE)) return FST_MUST_OPEN;
             continue;   // cur holds the current word
         }
-        // OPENING or CLOSING: another workgroup is at the host
+        // OPENING or CLOSING: another workgroup is at the host.  Relaxed
+        // polls: an acquire load at agent scope invalidates the L2 of the
+        // poller's XCD each time; the fadd or cas that follows acquires.
         if (p.expired()) return -ETIMEDOUT;
         p.pause();
-        cur = p.load(w);
+        cur = p.load_rlx(w);
     }
 }
 

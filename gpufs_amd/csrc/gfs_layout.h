@@ -45,6 +45,7 @@
 #define GFS_ST_SHIFT   20
 #define GFS_ST_MASK    (0xFull << GFS_ST_SHIFT)
 #define GFS_FL_DIRTY   (1ull << 24)
+#define GFS_FL_WB      (1ull << 25)   // a write-back of the frame's page is in flight
 #define GFS_GEN_SHIFT  32
 #define GFS_F_FREE     0ull
 #define GFS_F_FILLING  1ull

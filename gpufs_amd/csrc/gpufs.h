@@ -58,6 +58,7 @@ typedef __attribute__((address_space(1))) v4u gv4u;
 typedef __attribute__((address_space(3))) v4u lv4u;
 
 #define GFS_NOFRAME 0xFFFFFFFFu
+#define GFS_BUSYFRAME 0xFFFFFFFEu   // take_dirty: look at the page again later
 #define GFS_K_NONE 0
 #define GFS_K_HIT 1
 #define GFS_K_OWN 2
@@ -436,14 +437,19 @@ __device__ inline uint64_t node_alloc(gfs_dev* g) {
 __device__ inline uint64_t tree_leaf(gfs_dev* g, gfs_file* f, uint64_t leaf, bool create, int* err,
                                      uint64_t root_hint = 0) {
     const uint64_t page = leaf << GFS_FANOUT_LOG;
-    uint64_t rw = root_hint ? root_hint : ld_acq(&f->root);
+    // Relaxed loads throughout: agent-scope atomic loads are coherent (they
+    // do not read a stale line of the XCD's L2), each node word is read
+    // through an address taken from the word above it, and a node is zeroed
+    // before the CAS (a release) that links it.  An acquire load at agent
+    // scope would also invalidate the XCD's L2, once per level.
+    uint64_t rw = root_hint ? root_hint : ld_rlx(&f->root);
     bool fresh = root_hint == 0;
     uint64_t spare = 0;
     for (int guard = 0; guard < 16; ++guard) {
         uint32_t h = (uint32_t)(rw >> 40);
         if (rw != 0 && page < tree_cap(h)) break;
         if (!fresh) {
-            rw = ld_acq(&f->root);
+            rw = ld_rlx(&f->root);
             fresh = true;
             continue;
         }
@@ -473,7 +479,7 @@ __device__ inline uint64_t tree_leaf(gfs_dev* g, gfs_file* f, uint64_t leaf, boo
     uint64_t node = rw & ((1ull << 40) - 1);
     for (int lvl = (int)h - 1; lvl >= 1; --lvl) {
         uint64_t* sp = &g->nodes[node * GFS_FANOUT + ((page >> (GFS_FANOUT_LOG * lvl)) & (GFS_FANOUT - 1))];
-        uint64_t c = ld_acq(sp);
+        uint64_t c = ld_rlx(sp);
         if (!c) {
             if (!create) return 0;
             uint64_t n = node_alloc(g);
@@ -806,6 +812,7 @@ struct fst_dev {
     uint32_t spins = 0;
     __device__ uint64_t fadd(uint64_t* w, uint64_t v) { return gfs::fadd(w, v); }
     __device__ uint64_t load(uint64_t* w) { return ld_acq(w); }
+    __device__ uint64_t load_rlx(uint64_t* w) { return ld_rlx(w); }
     __device__ bool cas(uint64_t* w, uint64_t& exp, uint64_t v) { return gfs::cas(w, exp, v); }
     __device__ bool expired() { return now_ticks() - t0 > limit; }
     // Waits for another workgroup's host open or close: after a few short
@@ -1714,34 +1721,49 @@ __device__ inline size_t gwrite(int fd, size_t offset, size_t size, uchar* buffe
 }
 
 namespace gfs {
-// Pins the frame of leaf word `v` if it holds a live, VALID, dirty page and
-// clears the page's dirty flag; returns the frame (pinned) or GFS_NOFRAME.
-// The flag is cleared before the page is copied out: a gwrite after that
-// point sets it again, so its bytes are written back by a later flush.
+// Takes the page of leaf word `v` for write-back: when the frame holds the
+// live, VALID page, dirty and with no write-back of it in flight, one CAS pins
+// it, clears DIRTY and sets WB; returns the frame.  GFS_NOFRAME: nothing to
+// write (clean, or not cached).  GFS_BUSYFRAME: the page is dirty but its CAS
+// kept failing, or another write-back of it is in flight; look again later.
+// One write-back of a page at a time: two WRITEs of one page in flight at once
+// could reach the disk in either order, the older bytes last.  DIRTY is
+// cleared before the page is copied out: a gwrite after that sets it again,
+// so its bytes are written back by a later flush.
 __device__ inline uint32_t take_dirty(gfs_dev* g, uint64_t v) {
     uint32_t got = GFS_NOFRAME;
     if (v != 0 && v != GFS_LEAF_RESERVED) {
         const uint32_t f = leaf_frame(v);
         // a plain look first: clean pages cost no atomic
-        const uint64_t m = ld_rlx(&g->meta[f]);
-        if (meta_gen(m) == leaf_gen(v) && meta_state(m) == GFS_F_VALID && (m & GFS_FL_DIRTY)) {
-            const uint64_t old = fadd_rlx(&g->meta[f], 1ull);
-            if (meta_gen(old) == leaf_gen(v) && meta_state(old) == GFS_F_VALID &&
-                (__hip_atomic_fetch_and(&g->meta[f], ~GFS_FL_DIRTY, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) &
-                 GFS_FL_DIRTY))
+        uint64_t m = ld_rlx(&g->meta[f]);
+        for (int tries = 0; tries < 8; ++tries) {
+            if (meta_gen(m) != leaf_gen(v) || meta_state(m) != GFS_F_VALID) break;
+            if (m & GFS_FL_WB) {
+                got = GFS_BUSYFRAME;
+                break;
+            }
+            if (!(m & GFS_FL_DIRTY)) break;
+            uint64_t exp = m;
+            if (cas(&g->meta[f], exp, ((m | GFS_FL_WB) & ~GFS_FL_DIRTY) + 1ull)) {
                 got = f;
-            else
-                fadd_rlx(&g->meta[f], ~0ull);
+                break;
+            }
+            m = exp;
+            got = GFS_BUSYFRAME;   // if the tries run out
         }
     }
-    dirty_tally(g, got != GFS_NOFRAME, false);
+    dirty_tally(g, got < GFS_BUSYFRAME, false);
     return got;
 }
-// A page whose write-back did not happen is marked dirty again, so it is
-// neither evicted nor forgotten: a later gfsync / gclose retries it.
-__device__ __forceinline__ void redirty(gfs_dev* g, uint32_t f) {
-    const uint64_t old = __hip_atomic_fetch_or(&g->meta[f], GFS_FL_DIRTY, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    dirty_tally(g, !(old & GFS_FL_DIRTY), true);
+// Ends the write-back of frame f begun by take_dirty: WB off and its pin
+// dropped; a page whose write did not happen (!ok) is marked dirty again, so
+// it is neither evicted nor forgotten and a later gfsync / gclose retries it.
+__device__ __forceinline__ void wb_end(gfs_dev* g, uint32_t f, bool ok) {
+    bool nd = false;
+    if (!ok) nd = !(__hip_atomic_fetch_or(&g->meta[f], GFS_FL_DIRTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+                    GFS_FL_DIRTY);
+    dirty_tally(g, nd, true);
+    fadd(&g->meta[f], (uint64_t)0 - GFS_FL_WB - 1ull);
 }
 
 // Write back every dirty page of `fd` (workgroup-collective).  Pages of
@@ -1750,7 +1772,9 @@ __device__ __forceinline__ void redirty(gfs_dev* g, uint32_t f) {
 // travel together: wave 0 pins them and claims a ring slot for each, the
 // workgroup copies them into staging, every page's WRITE is posted at once on
 // one shared completion mask, and thread 0 waits for that mask once.  A dirty
-// page that found no free slot is taken in a later round.  Returns 0 or the
+// page that found no free slot, or whose write-back by another workgroup is
+// still in flight, is taken in a later round (bounded wait), so that when the
+// call returns every page dirtied before it is on disk.  Returns 0 or the
 // first error; pages whose write failed stay dirty.
 __device__ inline int flush_file(gfs_dev* g, int fd, uint64_t min_size = 0) {
     gfs_file* file = &g->files[fd];
@@ -1783,11 +1807,12 @@ __device__ inline int flush_file(gfs_dev* g, int fd, uint64_t min_size = 0) {
                 node = __shfl(node, 0);
                 const bool mine = node && ((pending >> ln) & 1ull);
                 uint32_t f = mine ? take_dirty(g, ld_rlx(&g->nodes[node * GFS_FANOUT + ln])) : GFS_NOFRAME;
+                const bool busy = f == GFS_BUSYFRAME;
+                if (busy) f = GFS_NOFRAME;
                 const int s = rpc_claim_wave(g, f != GFS_NOFRAME, blockIdx.x * 3u);
                 const bool noslot = f != GFS_NOFRAME && s < 0;
                 if (noslot) {   // stays dirty and pending
-                    redirty(g, f);
-                    fadd_rlx(&g->meta[f], ~0ull);
+                    wb_end(g, f, false);
                     f = GFS_NOFRAME;
                 }
                 const bool post = f != GFS_NOFRAME;
@@ -1804,7 +1829,7 @@ __device__ inline int flush_file(gfs_dev* g, int fd, uint64_t min_size = 0) {
                 gfs_ctl.gen[ln] = bytes;   // bytes to write (the arrays' generation field is free here)
                 const uint32_t cq = postm ? (uint32_t)__shfl(s, (int)first_lane(postm)) : 0u;
                 if (postm && ln == first_lane(postm)) sys_st(&g->cq[cq], 0ull);
-                const uint64_t again = __ballot(noslot);
+                const uint64_t again = __ballot(noslot || busy);
                 if (ln == 0) {
                     gfs_ctl.own = postm;
                     gfs_ctl.cq = cq;
@@ -1841,13 +1866,10 @@ __device__ inline int flush_file(gfs_dev* g, int fd, uint64_t min_size = 0) {
                         const uint32_t f = gfs_ctl.frame[ln];
                         const bool got = (answered >> ln) & 1ull;
                         const int64_t r = got ? sys_ld(&g->slots[gfs_ctl.slot[ln]].result) : (int64_t)-ETIMEDOUT;
-                        if (r < 0 || (uint64_t)r != gfs_ctl.gen[ln]) {
-                            perr = r < 0 ? (int)r : -EIO;
-                            redirty(g, f);
-                        } else {
-                            count(&g->ctr.bytes_written, (unsigned long long)r);
-                        }
-                        fadd(&g->meta[f], ~0ull);   // unpin
+                        const bool ok = r >= 0 && (uint64_t)r == gfs_ctl.gen[ln];
+                        if (!ok) perr = r < 0 ? (int)r : -EIO;
+                        else count(&g->ctr.bytes_written, (unsigned long long)r);
+                        wb_end(g, f, ok);
                     }
                     // Slots go back to the ring, except after a timeout those the
                     // host may still answer: the unanswered ones and the one
@@ -1865,7 +1887,8 @@ __device__ inline int flush_file(gfs_dev* g, int fd, uint64_t min_size = 0) {
             err = (int)uni(gfs_ctl.ret);
             const uint64_t nxt = uni(gfs_ctl.a);
             if (!err && nxt) {
-                // dirty pages found no free slot: wait for slots, bounded
+                // dirty pages found no free slot, or their write-back by
+                // another workgroup is in flight: wait, bounded
                 if (postm == 0) {
                     if (++idle > 4) backoff();
                     if ((idle & 63) == 0 && wg_timed_out(g, t0)) err = -ETIMEDOUT;
@@ -1893,7 +1916,15 @@ __device__ inline int flush_file_single(gfs_dev* g, int fd) {
         const uint64_t node = tree_leaf(g, file, leaf, false, &e);
         if (!node) continue;
         for (uint32_t i = 0; i < GFS_FANOUT && (leaf << GFS_FANOUT_LOG) + i < npages; ++i) {
-            const uint32_t f = take_dirty(g, ld_rlx(&g->nodes[node * GFS_FANOUT + i]));
+            uint32_t f;
+            const uint64_t tb = now_ticks();
+            while ((f = take_dirty(g, ld_rlx(&g->nodes[node * GFS_FANOUT + i]))) == GFS_BUSYFRAME &&
+                   now_ticks() - tb <= g->wait_ticks)
+                backoff();
+            if (f == GFS_BUSYFRAME) {
+                if (!err) err = -ETIMEDOUT;
+                continue;
+            }
             if (f == GFS_NOFRAME) continue;
             const uint64_t poff = ((leaf << GFS_FANOUT_LOG) + i) << g->log_page;
             const uint64_t bytes = g->fbytes[f];   // the frame's extent (see flush_file)
@@ -1918,13 +1949,10 @@ __device__ inline int flush_file_single(gfs_dev* g, int fd) {
                     rpc_release(g, s);
                 }
             }
-            if (r < 0 || (uint64_t)r != bytes) {
-                redirty(g, f);
-                if (!err) err = r < 0 ? (int)r : -EIO;
-            } else {
-                count(&g->ctr.bytes_written, bytes);
-            }
-            fadd(&g->meta[f], ~0ull);
+            const bool ok = r >= 0 && (uint64_t)r == bytes;
+            if (!ok && !err) err = r < 0 ? (int)r : -EIO;
+            if (ok) count(&g->ctr.bytes_written, bytes);
+            wb_end(g, f, ok);
         }
     }
     return err;
@@ -1943,8 +1971,19 @@ __device__ inline void discard_pages(gfs_dev* g, int fd) {
         const uint64_t node = tree_leaf(g, e, leaf, false, &er);
         if (!node) continue;
         for (uint32_t i = 0; i < GFS_FANOUT; ++i) {
-            const uint32_t f = take_dirty(g, ld_rlx(&g->nodes[node * GFS_FANOUT + i]));
-            if (f != GFS_NOFRAME) fadd(&g->meta[f], ~0ull);
+            // the dirty flag of the page's live frame goes (a CAS that checks
+            // the generation: the frame may be holding another page by now)
+            const uint64_t v = ld_rlx(&g->nodes[node * GFS_FANOUT + i]);
+            if (v == 0 || v == GFS_LEAF_RESERVED) continue;
+            const uint32_t f = leaf_frame(v);
+            uint64_t m = ld_rlx(&g->meta[f]);
+            bool cleared = false;
+            while (meta_gen(m) == leaf_gen(v) && (m & GFS_FL_DIRTY) && !cleared) {
+                uint64_t exp = m;
+                cleared = cas(&g->meta[f], exp, m & ~GFS_FL_DIRTY);
+                m = exp;
+            }
+            dirty_tally(g, cleared, false);
         }
     }
     st_rlx(&e->root, 0ull);
@@ -2414,16 +2453,33 @@ __device__ inline int gmsync(volatile void* addr, size_t length, int flags) {
             if (now_ticks() - t0 > g->wait_ticks) break;
             backoff();
         }
-        gfs_ctl.slot[0] = s < 0 ? 0xFFFFFFFFu : (uint32_t)s;
         if (s >= 0) {
-            dirty_tally(g,
-                        (__hip_atomic_fetch_and(&g->meta[f], ~GFS_FL_DIRTY, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) &
-                         GFS_FL_DIRTY) != 0,
-                        false);
-            // the extent is read after the flag is cleared: a writer that
-            // grew it before setting the flag is seen, a later one re-dirties
-            gfs_ctl.ret = (int64_t)g->fbytes[f];
+            // The page is taken for write-back as take_dirty does (WB set,
+            // DIRTY cleared, a pin), once no other write-back of it is in
+            // flight (bounded wait; the mapping keeps the frame this page's).
+            uint64_t m = ld_rlx(&g->meta[f]);
+            bool took = false;
+            while (!took) {
+                if (m & GFS_FL_WB) {
+                    if (now_ticks() - t0 > g->wait_ticks) break;
+                    backoff();
+                    m = ld_rlx(&g->meta[f]);
+                    continue;
+                }
+                uint64_t exp = m;
+                took = cas(&g->meta[f], exp, ((m | GFS_FL_WB) & ~GFS_FL_DIRTY) + 1ull);
+                if (took) dirty_tally(g, (m & GFS_FL_DIRTY) != 0, false);
+                else m = exp;
+            }
+            if (!took) {
+                rpc_release(g, s);
+                s = -1;
+            }
         }
+        gfs_ctl.slot[0] = s < 0 ? 0xFFFFFFFFu : (uint32_t)s;
+        // the extent is read after the flag is cleared: a writer that grew it
+        // before setting the flag is seen, a later one re-dirties
+        if (s >= 0) gfs_ctl.ret = (int64_t)g->fbytes[f];
     }
     __syncthreads();
     const uint32_t s = uni(gfs_ctl.slot[0]);
@@ -2445,13 +2501,10 @@ __device__ inline int gmsync(volatile void* addr, size_t length, int flags) {
             r = sys_ld(&g->slots[s].result);
             rpc_release(g, (int)s);
         }
-        if (r < 0 || (uint64_t)r != bytes) {
-            redirty(g, (uint32_t)f);   // not on disk: the page stays dirty
-            gfs_ctl.ret = r < 0 ? r : -EIO;
-        } else {
-            count(&g->ctr.bytes_written, bytes);
-            gfs_ctl.ret = 0;
-        }
+        const bool ok = r >= 0 && (uint64_t)r == bytes;
+        if (ok) count(&g->ctr.bytes_written, bytes);
+        gfs_ctl.ret = ok ? 0 : (r < 0 ? r : -EIO);
+        wb_end(g, (uint32_t)f, ok);   // not on disk: the page stays dirty
     }
     __syncthreads();
     return (int)uni(gfs_ctl.ret);

@@ -878,12 +878,15 @@ __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t n
                 __hip_atomic_store(&fs->tag, ((uint64_t)epoch << 32) | d, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             const uint64_t w0 = gfs::now_ticks();
+            // relaxed polls, one acquire once the decision is seen (an acquire
+            // load at agent scope invalidates the poller's L2 on every poll)
             for (;;) {
-                const uint64_t tg = __hip_atomic_load(&fs->tag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t tg = __hip_atomic_load(&fs->tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 d = (tg >> 32) == epoch ? (uint32_t)tg : 0u;
                 if (d || gfs::now_ticks() - w0 >= gfs_g->wait_ticks) break;
                 __builtin_amdgcn_s_sleep(8);
             }
+            gfs::fence_acq_agent();
             // no decision within the wait limit: this part scans nothing and
             // the file reports -ETIMEDOUT
         }

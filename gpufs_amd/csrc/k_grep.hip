@@ -88,16 +88,19 @@ __global__ __launch_bounds__(1024) void k_grep_batched(const char* names, uint32
         if (t == 0) {
             uint64_t it = BATCH_STOP;
             const uint64_t t0 = gfs::now_ticks();
+            // Relaxed polls throughout (an acquire load at agent scope
+            // invalidates this XCD's L2 each time); one acquire below once an
+            // item is taken.
             for (uint32_t spins = 0;; ++spins) {
                 // an extra part first: its file is open already
-                uint32_t h = __hip_atomic_load(&q->head, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                const uint32_t tl = __hip_atomic_load(&q->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                uint32_t h = __hip_atomic_load(&q->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t tl = __hip_atomic_load(&q->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (h < tl) {
-                    if (__hip_atomic_compare_exchange_strong(&q->head, &h, h + 1, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                    if (__hip_atomic_compare_exchange_strong(&q->head, &h, h + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                              __HIP_MEMORY_SCOPE_AGENT)) {
                         // reserved by a running part 0, published right after: bounded wait
                         uint64_t v;
-                        while (!((v = __hip_atomic_load(&ring[h], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) >> 63) &&
+                        while (!((v = __hip_atomic_load(&ring[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 63) &&
                                gfs::now_ticks() - t0 < g->wait_ticks)
                             __builtin_amdgcn_s_sleep(2);
                         it = v;   // not ready after the wait limit: stop this workgroup
@@ -117,14 +120,15 @@ __global__ __launch_bounds__(1024) void k_grep_batched(const char* names, uint32
                     atomicSub(&q->expanding, 1u);
                     continue;
                 }
-                if (__hip_atomic_load(&q->expanding, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
-                    __hip_atomic_load(&q->head, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >=
-                        __hip_atomic_load(&q->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT))
+                if (__hip_atomic_load(&q->expanding, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
+                    __hip_atomic_load(&q->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
+                        __hip_atomic_load(&q->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
                     break;   // nothing left
                 if (gfs::now_ticks() - t0 > g->wait_ticks) break;
                 if (spins < 64) __builtin_amdgcn_s_sleep(2);
                 else __builtin_amdgcn_s_sleep(32);
             }
+            gfs::fence_acq_agent();   // the part item's publication (a release store)
             item = it;
         }
         __syncthreads();

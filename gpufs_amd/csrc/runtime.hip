@@ -323,7 +323,7 @@ extern "C" int gpufs_cache_drop(gpufs_ctx* c) {
     std::vector<uint64_t> meta(h.n_frames);
     HIP_TRY(hipMemcpy(meta.data(), h.meta, 8ull * h.n_frames, hipMemcpyDeviceToHost));
     for (uint64_t m : meta) {
-        if (m & GFS_FL_DIRTY) return -EBUSY;
+        if (m & (GFS_FL_DIRTY | GFS_FL_WB)) return -EBUSY;
         if (m & GFS_REF_MASK) return -EBUSY;
     }
     uint32_t nf = (uint32_t)std::min<unsigned long long>(cur.files_next, h.max_files);
@@ -425,6 +425,8 @@ extern "C" int gpufs_check_invariants(gpufs_ctx* c) {
     }
     for (uint32_t f = 0; f < h.n_frames; ++f)
         if (meta[f] & GFS_REF_MASK) return -1001;   // a frame is still pinned
+    for (uint32_t f = 0; f < h.n_frames; ++f)
+        if (meta[f] & GFS_FL_WB) return -1015;   // a write-back never ended
     for (uint32_t f = 0; f < h.n_frames; ++f)
         if (((meta[f] >> GFS_ST_SHIFT) & 0xF) == GFS_F_FILLING) return -1002;   // a fill never finished
     {
@@ -549,6 +551,7 @@ extern "C" const char* gpufs_strerror(int code) {
     case -1012: return "invariant: file table and name index disagree";
     case -1013: return "invariant: radix node lost, shared or on the free list twice";
     case -1014: return "invariant: dirty-frame count differs from the frames marked dirty";
+    case -1015: return "invariant: a write-back never ended";
     default: break;
     }
     if (code < 0 && code > -4096) return strerror(-code);

@@ -34,6 +34,7 @@ struct HostPolicy {
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     uint64_t fadd(uint64_t* w, uint64_t v) { return __atomic_fetch_add(w, v, __ATOMIC_ACQ_REL); }
     uint64_t load(uint64_t* w) { return __atomic_load_n(w, __ATOMIC_ACQUIRE); }
+    uint64_t load_rlx(uint64_t* w) { return __atomic_load_n(w, __ATOMIC_RELAXED); }
     bool cas(uint64_t* w, uint64_t& exp, uint64_t v) {
         return __atomic_compare_exchange_n(w, &exp, v, false, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE);
     }

@@ -58,6 +58,7 @@ struct HostFst {
     HostFt f;
     uint64_t fadd(uint64_t* w, uint64_t v) { return f.fadd(w, v); }
     uint64_t load(uint64_t* w) { return f.load(w); }
+    uint64_t load_rlx(uint64_t* w) { return f.load_rlx(w); }
     bool cas(uint64_t* w, uint64_t& exp, uint64_t v) { return f.cas(w, exp, v); }
     bool expired() { return f.expired(); }
     void pause() { sched_yield(); }
