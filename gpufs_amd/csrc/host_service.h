@@ -23,7 +23,12 @@
 #include <cstring>
 #include <thread>
 #include <vector>
+#include <algorithm>
 #include <climits>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <sys/resource.h>
 #include <fcntl.h>
 #include <linux/futex.h>
 #include <sys/prctl.h>
@@ -84,6 +89,171 @@ static inline int64_t pwrite_full(int fd, const uint8_t* buf, uint64_t len, uint
     }
     return (int64_t)done;
 }
+
+// Host descriptors kept across a file's close and reopen.  The device closes
+// a file when its last user leaves and opens it again on the next gopen
+// (libgpufs/fs_calls.cu:152-225: a reopen goes to the host, which opens the
+// path and returns its size and ctime, so a changed file gets a new cache
+// version).  Opening and closing cost 4-7 us of host time when 16 workers do
+// it at once (contention in the kernel), against well under 1 us for a
+// stat(): so a descriptor opened with no flag beyond its access mode is kept
+// after its close, and a reopen of the same path with the same access mode
+// reuses it when stat() of the path still finds the same file (same device
+// and inode); its size and times then come from that stat(), as they would
+// from a fresh open.  Anything else (O_CREAT, O_TRUNC, ..., a file replaced or
+// gone, a descriptor in use) takes a real open.  The table is split into
+// shards by path hash, each under a spin lock held for a lookup only (a
+// blocking mutex parked the workers of a burst of opens in the kernel); at
+// most `cap` descriptors are kept (per shard, the least recently closed eighth
+// of the idle ones goes when a shard is full); all are closed with the service.
+class DescCache {
+public:
+    DescCache() {
+        struct rlimit rl;
+        size_t lim = 1024;
+        if (getrlimit(RLIMIT_NOFILE, &rl) == 0 && rl.rlim_cur != RLIM_INFINITY) lim = (size_t)rl.rlim_cur;
+        const size_t cap = lim / 4 < 4096 ? lim / 4 : 4096;
+        shard_cap_ = cap / kShards;
+        for (auto& o : fd_shard_) o.store(0, std::memory_order_relaxed);
+    }
+    ~DescCache() { clear(); }
+    // A host descriptor for `name` opened with `flags`, with its stat; the
+    // descriptor or -errno.
+    int64_t open(const char* name, int flags, struct stat* st) {
+        const bool cacheable = (flags & ~O_ACCMODE) == 0 && shard_cap_ > 0;
+        std::string key;
+        Shard* sh = nullptr;
+        if (cacheable) {
+            key = key_of(name, flags);
+            sh = &shards_[std::hash<std::string>()(key) % kShards];
+            int fd = -1;
+            dev_t dev = 0;
+            ino_t ino = 0;
+            sh->lock();
+            auto it = sh->by_path.find(key);
+            if (it != sh->by_path.end() && !it->second.busy) {
+                it->second.busy = true;
+                fd = it->second.fd;
+                dev = it->second.dev;
+                ino = it->second.ino;
+            }
+            sh->unlock();
+            if (fd >= 0) {
+                if (::stat(name, st) == 0 && st->st_dev == dev && st->st_ino == ino) return fd;
+                sh->lock();   // replaced or gone: a real open decides
+                sh->by_path.erase(key);
+                sh->unlock();
+                fd_shard_[fd].store(0, std::memory_order_release);
+                ::close(fd);
+            }
+        }
+        const int fd = ::open(name, flags | O_CLOEXEC, 0644);
+        if (fd < 0) return -(int64_t)errno;
+        if (::fstat(fd, st) != 0) {
+            const int64_t e = -(int64_t)errno;
+            ::close(fd);
+            return e;
+        }
+        if (cacheable && fd < kMaxFd) {
+            std::vector<int> evicted;
+            sh->lock();
+            if (!sh->by_path.count(key)) {
+                sh->by_path[key] = Ent{fd, true, st->st_dev, st->st_ino, 0};
+                fd_shard_[fd].store((uint8_t)(sh - shards_ + 1), std::memory_order_release);
+                if (sh->by_path.size() > shard_cap_) evict(*sh, evicted);
+            }
+            sh->unlock();
+            for (int x : evicted) ::close(x);
+        }
+        return fd;
+    }
+    // Close of a descriptor the device got from open(): a kept one stays open.
+    int close(int fd) {
+        const uint32_t k = fd >= 0 && fd < kMaxFd ? fd_shard_[fd].load(std::memory_order_acquire) : 0;
+        if (k) {
+            Shard& sh = shards_[k - 1];
+            sh.lock();
+            for (auto& kv : sh.by_path)
+                if (kv.second.fd == fd && kv.second.busy) {
+                    kv.second.busy = false;
+                    kv.second.lru = ++sh.tick;
+                    sh.unlock();
+                    return 0;
+                }
+            sh.unlock();
+        }
+        return ::close(fd) == 0 ? 0 : -errno;
+    }
+    void clear() {
+        for (auto& sh : shards_) {
+            sh.lock();
+            for (auto& kv : sh.by_path) {
+                fd_shard_[kv.second.fd].store(0, std::memory_order_relaxed);
+                ::close(kv.second.fd);
+            }
+            sh.by_path.clear();
+            sh.unlock();
+        }
+    }
+    size_t size() {
+        size_t n = 0;
+        for (auto& sh : shards_) {
+            sh.lock();
+            n += sh.by_path.size();
+            sh.unlock();
+        }
+        return n;
+    }
+
+private:
+    static constexpr uint32_t kShards = 64;
+    static constexpr int kMaxFd = 1 << 16;
+    struct Ent {
+        int fd;
+        bool busy;
+        dev_t dev;
+        ino_t ino;
+        uint64_t lru;
+    };
+    struct Shard {
+        std::atomic<bool> held{false};
+        std::unordered_map<std::string, Ent> by_path;
+        uint64_t tick = 0;
+        void lock() {
+            while (held.exchange(true, std::memory_order_acquire))
+                while (held.load(std::memory_order_relaxed)) {
+#if defined(__x86_64__)
+                    __builtin_ia32_pause();
+#endif
+                }
+        }
+        void unlock() { held.store(false, std::memory_order_release); }
+    };
+    static std::string key_of(const char* name, int flags) {
+        std::string k(1, (char)('0' + (flags & O_ACCMODE)));
+        k += name;
+        return k;
+    }
+    // A full shard (lock held): the least recently closed eighth of its idle
+    // entries go (their fds into `out`).
+    void evict(Shard& sh, std::vector<int>& out) {
+        std::vector<std::pair<uint64_t, std::string>> idle;
+        for (auto& kv : sh.by_path)
+            if (!kv.second.busy) idle.emplace_back(kv.second.lru, kv.first);
+        size_t n = shard_cap_ / 8 > 0 ? shard_cap_ / 8 : 1;
+        if (n > idle.size()) n = idle.size();
+        std::partial_sort(idle.begin(), idle.begin() + n, idle.end());
+        for (size_t i = 0; i < n; ++i) {
+            auto it = sh.by_path.find(idle[i].second);
+            out.push_back(it->second.fd);
+            fd_shard_[it->second.fd].store(0, std::memory_order_relaxed);
+            sh.by_path.erase(it);
+        }
+    }
+    Shard shards_[kShards];
+    std::atomic<uint8_t> fd_shard_[kMaxFd];   // shard + 1 of a kept descriptor, or 0
+    size_t shard_cap_ = 0;
+};
 
 // Per-worker buffers: the OR-merge's copy of the disk bytes, and the bounce
 // buffer of streamed page reads.
@@ -151,7 +321,8 @@ static inline int64_t diff_merge_write(int fd, uint8_t* page, uint64_t len, uint
 }
 
 // Serve one request in slot s.  Returns the result stored in the slot.
-static inline int64_t serve_slot(const RingView& rv, uint32_t s, ServiceCounters& ctr, WorkerBufs& wb) {
+static inline int64_t serve_slot(const RingView& rv, uint32_t s, ServiceCounters& ctr, WorkerBufs& wb,
+                                 DescCache* dc = nullptr) {
     gfs_rpc_slot* r = &rv.slots[s];
     const uint32_t op = r->op;
     const int fd = r->host_fd;
@@ -179,15 +350,22 @@ static inline int64_t serve_slot(const RingView& rv, uint32_t s, ServiceCounters
         // O_GWRONCE files are opened read-write so the OR-merge can read them
         // (libgpufs/host_loop.h:170-173)
         if ((flags & (O_CREAT | O_ACCMODE)) == (O_CREAT | O_WRONLY)) flags = O_RDWR | O_CREAT;
-        int hfd = ::open(name, flags | O_CLOEXEC, 0644);
-        if (hfd < 0) {
-            res = -(int64_t)errno;
-            break;
-        }
         struct stat st;
-        if (::fstat(hfd, &st) != 0) {
-            res = -(int64_t)errno;
-            ::close(hfd);
+        int64_t hfd;
+        if (dc) {
+            hfd = dc->open(name, flags, &st);
+        } else {
+            hfd = ::open(name, flags | O_CLOEXEC, 0644);
+            if (hfd < 0) {
+                hfd = -(int64_t)errno;
+            } else if (::fstat((int)hfd, &st) != 0) {
+                const int64_t e = -(int64_t)errno;
+                ::close((int)hfd);
+                hfd = e;
+            }
+        }
+        if (hfd < 0) {
+            res = hfd;
             break;
         }
         r->size = (uint64_t)st.st_size;
@@ -198,7 +376,7 @@ static inline int64_t serve_slot(const RingView& rv, uint32_t s, ServiceCounters
         break;
     }
     case GFS_OP_CLOSE:
-        res = ::close(fd) == 0 ? 0 : -(int64_t)errno;
+        res = dc ? dc->close(fd) : (::close(fd) == 0 ? 0 : -(int64_t)errno);
         break;
     case GFS_OP_TRUNC:
         res = ::ftruncate(fd, (off_t)off) == 0 ? 0 : -(int64_t)errno;
@@ -215,27 +393,63 @@ static inline int64_t serve_slot(const RingView& rv, uint32_t s, ServiceCounters
     return res;
 }
 
-// Try to take the next published request.  Returns true and sets *slot.
-// Takes the next published request.  For a close posted without a slot, the
-// host fd is read from closeq before the ticket is taken: while the ticket is
-// still the head, no later ticket can reuse its closeq entry (the device never
-// has more than sq_depth tickets ahead of the head), and a failed take drops
-// the value.
-static inline bool try_take(const RingView& rv, std::atomic<uint64_t>& head, uint32_t* slot, int64_t* close_fd) {
-    uint64_t h = head.load(std::memory_order_relaxed);
-    uint64_t e = __atomic_load_n(&rv.sq[h & (rv.sq_depth - 1)], __ATOMIC_ACQUIRE);
-    if ((e >> 16) != (h & 0xFFFFFFFFFFFFull)) return false;
-    const uint32_t s = (uint32_t)(e & 0xFFFF);
-    const int64_t fd = s == GFS_SQ_NOSLOT ? __atomic_load_n(&rv.closeq[h & (rv.sq_depth - 1)], __ATOMIC_RELAXED) : -1;
-    if (!head.compare_exchange_strong(h, h + 1, std::memory_order_acq_rel)) return false;
-    *slot = s;
-    *close_fd = fd;
-    return true;
+// The host side of the submission queue: `head` is the first ticket not
+// taken yet, and claim[t & (sq_depth - 1)] == t + 1 once ticket t is taken.
+// Workers take tickets out of order: each scans the window of kTakeWindow
+// tickets from the head, in ticket order, and claims the first published one
+// nobody has claimed with a CAS on that ticket's own word.  With one shared
+// head word taken in order, every published request set off a CAS storm of
+// all spinning workers on that word, and a ticket whose request the device
+// had not published yet held up every later one: 64 opens posted within
+// 9 us took the host ~30 us to pick up.  Scanning in ticket order keeps the
+// property that a published request is taken before any request published
+// after it, so a slot-less close is taken before the reopen of its entry (the
+// submission queue's depth counts one outstanding close per entry).
+static constexpr uint32_t kTakeWindow = 64;
+struct TakeQueue {
+    std::atomic<uint64_t> head{0};
+    std::unique_ptr<std::atomic<uint64_t>[]> claim;
+    uint64_t mask = 0;
+    void init(uint32_t sq_depth) {
+        if (claim && mask + 1 == sq_depth) return;
+        claim.reset(new std::atomic<uint64_t>[sq_depth]);
+        for (uint32_t i = 0; i < sq_depth; ++i) claim[i].store(0, std::memory_order_relaxed);
+        mask = sq_depth - 1;
+    }
+};
+
+// Takes a published request that no worker has taken.  Returns true and sets
+// *slot (and, for a close posted without a slot, *close_fd: read before the
+// claim, while the ticket cannot be reused, since the device never has more
+// than sq_depth tickets outstanding and this one is not taken).
+static inline bool try_take(const RingView& rv, TakeQueue& q, uint32_t* slot, int64_t* close_fd) {
+    const uint64_t h = q.head.load(std::memory_order_acquire);
+    const uint32_t win = kTakeWindow < rv.sq_depth ? kTakeWindow : rv.sq_depth;
+    for (uint32_t k = 0; k < win; ++k) {
+        const uint64_t t = h + k;
+        const uint64_t e = __atomic_load_n(&rv.sq[t & q.mask], __ATOMIC_ACQUIRE);
+        if ((e >> 16) != (t & 0xFFFFFFFFFFFFull)) continue;   // not published (yet)
+        std::atomic<uint64_t>& c = q.claim[t & q.mask];
+        uint64_t cv = c.load(std::memory_order_relaxed);
+        if (cv == t + 1) continue;   // taken
+        const uint32_t s = (uint32_t)(e & 0xFFFF);
+        const int64_t fd = s == GFS_SQ_NOSLOT ? __atomic_load_n(&rv.closeq[t & q.mask], __ATOMIC_RELAXED) : -1;
+        if (!c.compare_exchange_strong(cv, t + 1, std::memory_order_acq_rel)) continue;
+        // move the head past the tickets taken in a row
+        uint64_t hh = q.head.load(std::memory_order_relaxed);
+        while (q.claim[hh & q.mask].load(std::memory_order_acquire) == hh + 1) {
+            if (q.head.compare_exchange_weak(hh, hh + 1, std::memory_order_acq_rel)) ++hh;
+        }
+        *slot = s;
+        *close_fd = fd;
+        return true;
+    }
+    return false;
 }
 
 // A close posted without a slot (the device does not wait for it).
-static inline void serve_slotless_close(int64_t fd, ServiceCounters& ctr) {
-    if (fd < 0 || ::close((int)fd) != 0) ctr.errors.fetch_add(1, std::memory_order_relaxed);
+static inline void serve_slotless_close(int64_t fd, ServiceCounters& ctr, DescCache* dc = nullptr) {
+    if (fd < 0 || (dc ? dc->close((int)fd) : ::close((int)fd)) != 0) ctr.errors.fetch_add(1, std::memory_order_relaxed);
     ctr.requests.fetch_add(1, std::memory_order_relaxed);
 }
 
@@ -261,11 +475,16 @@ static inline void wake_all(WakeState* w) {
     syscall(SYS_futex, (uint32_t*)&w->word, FUTEX_WAKE_PRIVATE, INT_MAX, nullptr, nullptr, 0);
 }
 
-// True when the submission queue holds a published request at `head`.
-static inline bool backlog(const RingView& rv, const std::atomic<uint64_t>& head) {
-    const uint64_t h = head.load(std::memory_order_relaxed);
-    const uint64_t e = __atomic_load_n(&rv.sq[h & (rv.sq_depth - 1)], __ATOMIC_ACQUIRE);
-    return (e >> 16) == (h & 0xFFFFFFFFFFFFull);
+// True when one of the first tickets from the head holds a published request
+// that no worker has taken.
+static inline bool backlog(const RingView& rv, const TakeQueue& q) {
+    const uint64_t h = q.head.load(std::memory_order_relaxed);
+    for (uint64_t t = h; t < h + 4; ++t) {
+        const uint64_t e = __atomic_load_n(&rv.sq[t & q.mask], __ATOMIC_ACQUIRE);
+        if ((e >> 16) == (t & 0xFFFFFFFFFFFFull) && q.claim[t & q.mask].load(std::memory_order_relaxed) != t + 1)
+            return true;
+    }
+    return false;
 }
 
 // Worker `index` of the service.  Worker 0 polls the queue without sleeping
@@ -273,8 +492,8 @@ static inline bool backlog(const RingView& rv, const std::atomic<uint64_t>& head
 // is picked up within a poll; the others spin for a while after their last
 // request, then sleep on the wake futex (2 us steps, 50 us after a longer
 // idle; timer slack lowered to 1 us so short sleeps are not stretched).
-static inline void worker_loop(RingView rv, std::atomic<uint64_t>* head, std::atomic<bool>* stop,
-                               ServiceCounters* ctr, WakeState* wk, unsigned index) {
+static inline void worker_loop(RingView rv, TakeQueue* head, std::atomic<bool>* stop,
+                               ServiceCounters* ctr, WakeState* wk, DescCache* dc, unsigned index) {
     prctl(PR_SET_TIMERSLACK, 1000UL, 0, 0, 0);
     WorkerBufs wb;
     uint64_t idle = 0;
@@ -285,8 +504,8 @@ static inline void worker_loop(RingView rv, std::atomic<uint64_t>* head, std::at
         if (try_take(rv, *head, &s, &close_fd)) {
             if (wk->sleepers.load(std::memory_order_relaxed) && backlog(rv, *head)) wake_all(wk);
             const uint64_t t0 = mono_ns();
-            if (s == GFS_SQ_NOSLOT) serve_slotless_close(close_fd, *ctr);
-            else serve_slot(rv, s, *ctr, wb);
+            if (s == GFS_SQ_NOSLOT) serve_slotless_close(close_fd, *ctr, dc);
+            else serve_slot(rv, s, *ctr, wb, dc);
             ctr->busy_ns.fetch_add(mono_ns() - t0, std::memory_order_relaxed);
             idle = 0;
             continue;
@@ -322,8 +541,9 @@ public:
     void start(const RingView& rv, unsigned nthreads) {
         if (running_) return;
         stop_.store(false);
+        head_.init(rv.sq_depth);
         for (unsigned i = 0; i < nthreads; ++i)
-            threads_.emplace_back(worker_loop, rv, &head_, &stop_, &ctr_, &wake_, i);
+            threads_.emplace_back(worker_loop, rv, &head_, &stop_, &ctr_, &wake_, &dcache_, i);
         running_ = true;
     }
     void stop() {
@@ -344,12 +564,13 @@ public:
         wake_all(&wake_);
     }
     ServiceCounters& counters() { return ctr_; }
-    uint64_t head() const { return head_.load(); }
+    uint64_t head() const { return head_.head.load(); }
     ~Service() { stop(); }
 
 private:
     std::vector<std::thread> threads_;
-    std::atomic<uint64_t> head_{0};
+    TakeQueue head_;   // persists across stop/start: the device's tickets continue
+    DescCache dcache_;   // persists across stop/start: reopens come after a restart
     std::atomic<bool> stop_{false};
     ServiceCounters ctr_;
     WakeState wake_;
