@@ -45,7 +45,8 @@ class Stats(ctypes.Structure):
 
 class ServeConfig(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in
-                ("workgroups", "threads", "slots", "chunk_bytes", "n_words", "frame_budget")]
+                ("workgroups", "threads", "slots", "chunk_bytes", "n_words", "frame_budget", "parts_max",
+                 "min_part_kib")]
 
 
 # name -> (restype, argtypes) of every exported entry point
@@ -81,6 +82,7 @@ SIGNATURES = {
     "gpufs_serve_run": (I, [P, P, P, U32, P, P, P, ctypes.POINTER(ctypes.c_double)]),
     "gpufs_serve_stop": (I, [P]),
     "gpufs_serve_pages": (I, [P, P, P]),
+    "gpufs_serve_parts": (I, [P, P, P]),
     "gpufs_launch_mmap_warp_records": (I, [P, P, U64, U64, P, P, U32, U32, P]),
     "gpufs_grep_text": (I, [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, U32, U32, U32, P, P]),
     "gpufs_grep_text_records": (I, [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, U32, U32, U32, P, P]),

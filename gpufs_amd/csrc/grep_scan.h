@@ -824,10 +824,17 @@ __device__ __noinline__ int grep_split_finish(grep_lds& S, int fd, uint32_t nw, 
 // reference's form, one workgroup per file): its counts stay in LDS, the tail
 // is added there and the row is stored at the end, so nothing needs clearing
 // before the launch and no state is shared.
-template <class OnSize>
-__device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t nw, uint32_t chunk, uint32_t* row,
+//
+// `split_cold`: a file that is not cached is split too (the serving form,
+// where a large cold file read by one workgroup sets the tail latency).
+// on_arrive() is called by every thread of each part right before the part
+// counts itself arrived.  Returns true in the workgroup that completed the
+// file (the last to arrive, or the only one) and wrote *status.
+template <class OnSize, class OnArrive>
+__device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t nw, uint32_t chunk, uint32_t* row,
                                        int32_t* status, uint32_t j, uint32_t parts, uint64_t min_part,
-                                       grep_split_state* fs, uint32_t target, uint32_t epoch, OnSize&& on_size) {
+                                       grep_split_state* fs, uint32_t target, uint32_t epoch, bool split_cold,
+                                       OnSize&& on_size, OnArrive&& on_arrive) {
     const uint32_t T = gfs::nthreads(), t = gfs::tid();
     __shared__ uint32_t fin_flag;
     const bool single = fs == nullptr;
@@ -868,7 +875,7 @@ __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t n
             if (fd >= 0 && !single) {
                 const uint64_t sz = S.fsize;
                 warm = sz == 0 || gfs::page_cached(gfs_g, fd, (sz - 1) >> gfs_g->log_page);
-                k = warm ? sz / min_part : 1;
+                k = (warm || split_cold) ? sz / min_part : 1;
                 k = k < 1 ? 1 : (k > parts ? parts : k);
             } else if (fd >= 0) {
                 warm = S.fsize == 0 || gfs::page_cached(gfs_g, fd, (S.fsize - 1) >> gfs_g->log_page);
@@ -969,8 +976,9 @@ __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t n
         const int c = fd >= 0 ? gclose(fd) : 0;
         if (t == 0) *status = st ? st : c;
         __syncthreads();
-        return;
+        return true;
     }
+    on_arrive();
     __syncthreads();
     if (t == 0) {
         if (lr) atomicOr(&fs->longrun, 1u);
@@ -997,10 +1005,11 @@ __device__ inline void grep_file_split(grep_lds& S, const char* name, uint32_t n
         if (t == 0) *status = st ? st : c;
         __syncthreads();
         GFS_TRACE(49);
-        return;
+        return true;
     }
     if (fd >= 0) gclose(fd);
     GFS_TRACE(49);
+    return false;
 }
 
 static int grep_args_ok(uint32_t n_files, uint32_t n_words, uint32_t chunk_bytes, uint32_t threads) {

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(1024) void k_grep(const char* names, uint32_t strid
     GFS_TRACE(41);
     [[clang::always_inline]] grep_file_split(S, names + (size_t)f * stride, nw, chunk, counts_out + (size_t)f * nw,
                                              status + f, j, parts, min_part, fstate ? fstate + f : nullptr, parts,
-                                             epoch, [](uint32_t) {});
+                                             epoch, false, [](uint32_t) {}, [] {});
 }
 
 // ---------------------------------------------------------------------------
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(1024) void k_grep_batched(const char* names, uint32
         if (j == 0 && t == 0 && times) times[2 * (size_t)f] = gfs::now_ticks();
         [[clang::always_inline]] grep_file_split(
             S, names + (size_t)f * stride, nw, chunk, counts_out + (size_t)f * nw, status + f, j, parts, min_part,
-            fs + f, j == 0 ? 0u : kfi, 0u, [&](uint32_t kf) {
+            fs + f, j == 0 ? 0u : kfi, 0u, false, [&](uint32_t kf) {
                 if (j != 0 || t != 0) return;
                 if (kf > 1) {
                     const uint32_t r0 = atomicAdd(&q->tail, kf - 1);
@@ -151,7 +151,8 @@ __global__ __launch_bounds__(1024) void k_grep_batched(const char* names, uint32
                                            __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 }
                 __hip_atomic_fetch_sub(&q->expanding, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            });
+            },
+            [] {});
         if (t == 0 && times) times[2 * (size_t)f + 1] = gfs::now_ticks();   // the last part's end is kept
         // A barrier between this thread-0 region and the next round's keeps the
         // compiler from threading the `tid == 0` test across the back-edge.

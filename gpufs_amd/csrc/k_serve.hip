@@ -39,7 +39,36 @@ struct serve_req {
     uint64_t seq;       // ticket + 1 once published
     uint32_t file_id;   // SERVE_STOP ends the workgroup
     uint32_t slot;
+    uint32_t parts;     // workgroups the request may be split over (1: one)
+    uint32_t pad;
 };
+
+// An extra part of a split request (device memory), handed from the request's
+// part 0 to a waiting workgroup: `ready` = ring position + 1 once published.
+struct serve_part {
+    uint64_t ready;
+    uint64_t seq;       // the request's ticket + 1: its split epoch and completion word
+    uint32_t file_id, slot, kf, part;
+};
+struct serve_part_ctl {
+    uint32_t head, tail;   // ring positions taken / reserved
+    uint32_t pad[62];
+};
+// Device-side state of the requests in flight, per request slot.
+struct serve_dev {
+    grep_split_state* st;   // split state
+    uint32_t* row;          // counts, added into by the parts
+    int32_t* status;        // written by the part that completes the request
+    unsigned int* pages;    // page hits, misses of all its parts
+    serve_part* ring;       // extra parts
+    uint32_t ring_mask;
+    serve_part_ctl* ctl;
+    uint64_t min_part;      // bytes: a split request's parts are at least this long
+};
+
+#define SERVE_KIND_STOP 0u
+#define SERVE_KIND_REQ 1u
+#define SERVE_KIND_PART 2u
 
 __global__ __launch_bounds__(1024) void k_serve(const char* names, uint32_t stride, uint32_t nfiles,
                                                 const char* dict, uint32_t nw, uint32_t chunk,
@@ -47,43 +76,114 @@ __global__ __launch_bounds__(1024) void k_serve(const char* names, uint32_t stri
                                                 int32_t* res_status, uint32_t* res_pages, uint64_t* done,
                                                 unsigned long long* ticket, const uint32_t* halt,
                                                 uint64_t claim_ticks, uint32_t* held, unsigned int* held_n,
-                                                uint32_t held_cap) {
+                                                uint32_t held_cap, serve_dev D, uint32_t* res_parts) {
     __shared__ __attribute__((aligned(16))) grep_lds S;
-    __shared__ uint32_t fid, slot;
-    __shared__ uint64_t myseq;
+    __shared__ uint32_t fid, slot, nparts, partj, kind, have_tk;
+    __shared__ uint64_t myseq, mytk;
     __shared__ uint32_t h0, m0, hold;
     grep_build_dict(S, dict, nw);
     const uint32_t t = gfs::tid();
+    if (t == 0) have_tk = 0;
+    // Completion of a split request by the part that finished it: the row is
+    // stored into the request's result slot, then its status, page counts and
+    // completion word (workgroup-collective).
+    auto complete_split = [&](uint32_t s, uint64_t sq, uint32_t kf) {
+        for (uint32_t q = t; q < nw; q += gfs::nthreads()) results[(size_t)s * nw + q] = D.row[(size_t)s * nw + q];
+        __threadfence_system();
+        __syncthreads();
+        if (t == 0) {
+            gfs::sys_st(&res_pages[2 * (size_t)s], gfs::ld_rlx(&D.pages[2 * (size_t)s]));
+            gfs::sys_st(&res_pages[2 * (size_t)s + 1], gfs::ld_rlx(&D.pages[2 * (size_t)s + 1]));
+            gfs::sys_st(&res_status[s], D.status[s]);
+            gfs::sys_st(&res_parts[s], kf);
+            gfs::fence_rel_sys();
+            gfs::sys_st_rel(&done[s], sq);
+        }
+        __syncthreads();
+    };
+    // the part's page lookups go to the request's totals before it arrives
+    auto add_pages = [&](uint32_t s) {
+        if (t == 0) {
+            atomicAdd(&D.pages[2 * (size_t)s], gfs_ctl.hits - h0);
+            atomicAdd(&D.pages[2 * (size_t)s + 1], gfs_ctl.misses - m0);
+        }
+    };
     for (;;) {
         if (t == 0) {
-            unsigned long long k = atomicAdd(ticket, 1ull);
+            if (!have_tk) {
+                mytk = atomicAdd(ticket, 1ull);
+                have_tk = 1;
+            }
+            const unsigned long long k = mytk;
             serve_req* e = &ring[k & ring_mask];
             uint32_t spins = 0;
-            fid = SERVE_STOP;
+            kind = SERVE_KIND_STOP;
             // Waits as long as the server runs, however long it idles: a ticket
-            // once taken is served.  The wait ends when the request is published
-            // or when the host sets the halt word (gpufs_serve_stop, after it has
-            // published a stop request for every workgroup).  Relaxed polls (a
-            // system-scope acquire load invalidates the caches each time), one
-            // acquire when the request is there.
-            while (gfs::sys_ld(&e->seq) != k + 1) {
+            // once taken is served.  Meanwhile extra parts of split requests
+            // are taken first (they belong to earlier requests).  The wait ends
+            // when the request is published or when the host sets the halt word
+            // (gpufs_serve_stop, after it has published a stop request for every
+            // workgroup).  Relaxed polls (a system-scope acquire load
+            // invalidates the caches each time), one acquire when something is
+            // there.
+            for (;;) {
+                const uint32_t ph = gfs::ld_rlx(&D.ctl->head), pt = gfs::ld_rlx(&D.ctl->tail);
+                if ((int32_t)(pt - ph) > 0) {
+                    uint32_t hh = ph;
+                    if (__hip_atomic_compare_exchange_strong(&D.ctl->head, &hh, ph + 1, __ATOMIC_RELAXED,
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        // reserved by a running part 0, published right after it:
+                        // bounded wait
+                        serve_part* it = &D.ring[ph & D.ring_mask];
+                        const uint64_t c0 = gfs::now_ticks();
+                        while (gfs::ld_rlx(&it->ready) != (uint64_t)ph + 1 && gfs::now_ticks() - c0 < claim_ticks)
+                            __builtin_amdgcn_s_sleep(2);
+                        gfs::fence_acq_agent();
+                        if (gfs::ld_rlx(&it->ready) == (uint64_t)ph + 1) {
+                            kind = SERVE_KIND_PART;
+                            fid = it->file_id;
+                            slot = it->slot;
+                            nparts = it->kf;
+                            partj = it->part;
+                            myseq = it->seq;
+                            break;
+                        }
+                    }
+                    continue;
+                }
+                if (gfs::sys_ld(&e->seq) == k + 1) {
+                    gfs::fence_acq_sys();
+                    kind = SERVE_KIND_REQ;
+                    fid = gfs::sys_ld(&e->file_id);
+                    slot = gfs::sys_ld(&e->slot);
+                    nparts = gfs::sys_ld(&e->parts);
+                    partj = 0;
+                    myseq = k + 1;
+                    have_tk = 0;
+                    break;
+                }
                 if (++spins > 64) __builtin_amdgcn_s_sleep(127);
                 else __builtin_amdgcn_s_sleep(8);
                 if ((spins & 63) == 0 && gfs::sys_ld(halt) != 0u) break;
             }
-            gfs::fence_acq_sys();
-            if (gfs::sys_ld(&e->seq) == k + 1) {
-                fid = gfs::sys_ld(&e->file_id);
-                slot = gfs::sys_ld(&e->slot);
-            }
-            myseq = k + 1;
             h0 = gfs_ctl.hits;
             m0 = gfs_ctl.misses;
         }
         __syncthreads();
-        const uint32_t f = gfs::uni(fid), s = gfs::uni(slot);
+        const uint32_t kd = gfs::uni(kind);
+        const uint32_t f = kd == SERVE_KIND_STOP ? SERVE_STOP : gfs::uni(fid), s = gfs::uni(slot);
+        const uint32_t np = gfs::uni(nparts), pj = gfs::uni(partj);
         const uint64_t sq = gfs::uni(myseq);
         __syncthreads();
+        if (kd == SERVE_KIND_PART) {
+            // an extra part of another workgroup's request
+            const bool fin = grep_file_split(S, names + (size_t)f * stride, nw, chunk, D.row + (size_t)s * nw,
+                                             D.status + s, pj, np, D.min_part, D.st + s, np, (uint32_t)sq, true,
+                                             [](uint32_t) {}, [&] { add_pages(s); });
+            if (fin) complete_split(s, sq, np);
+            __syncthreads();
+            continue;
+        }
         if (f == SERVE_STOP) {
             // drop the server's open references of this workgroup's share of the files
             for (uint32_t x = blockIdx.x; x < nfiles; x += gridDim.x) {
@@ -133,6 +233,38 @@ __global__ __launch_bounds__(1024) void k_serve(const char* names, uint32_t stri
                 }
             }
         }
+        if (f < nfiles && np > 1) {
+            // part 0 of a split request: it hands the file's other parts to
+            // waiting workgroups through the part ring; whichever part
+            // finishes last completes the request
+            if (t == 0) {
+                __hip_atomic_store(&D.pages[2 * (size_t)s], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&D.pages[2 * (size_t)s + 1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            uint32_t kfr = 1;
+            const bool fin = grep_file_split(
+                S, names + (size_t)f * stride, nw, chunk, D.row + (size_t)s * nw, D.status + s, 0, np, D.min_part,
+                D.st + s, 0, (uint32_t)sq, true,
+                [&](uint32_t kf) {
+                    kfr = kf;
+                    if (t != 0 || kf < 2) return;
+                    const uint32_t r0 = atomicAdd(&D.ctl->tail, kf - 1);
+                    for (uint32_t x = 1; x < kf; ++x) {
+                        const uint32_t pos = r0 + x - 1;
+                        serve_part* it = &D.ring[pos & D.ring_mask];
+                        it->seq = sq;
+                        it->file_id = f;
+                        it->slot = s;
+                        it->kf = kf;
+                        it->part = x;
+                        __hip_atomic_store(&it->ready, (uint64_t)pos + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                },
+                [&] { add_pages(s); });
+            if (fin) complete_split(s, sq, kfr);
+            __syncthreads();
+            continue;
+        }
         int st = f < nfiles ? grep_one_file(S, names + (size_t)f * stride, nw, chunk, results + (size_t)s * nw)
                             : -EINVAL;
         __threadfence_system();
@@ -141,6 +273,7 @@ __global__ __launch_bounds__(1024) void k_serve(const char* names, uint32_t stri
             gfs::sys_st(&res_pages[2 * (size_t)s], gfs_ctl.hits - h0);
             gfs::sys_st(&res_pages[2 * (size_t)s + 1], gfs_ctl.misses - m0);
             gfs::sys_st(&res_status[s], st);
+            gfs::sys_st(&res_parts[s], 1u);
             gfs::fence_rel_sys();
             gfs::sys_st_rel(&done[s], sq);
         }
@@ -163,7 +296,12 @@ struct gpufs_server {
     uint32_t* rpages;    // host coherent: slots x 2 (page hits, misses of the request)
     uint64_t* done;      // host coherent: slots
     uint32_t* halt;      // host coherent: set by gpufs_serve_stop, ends every workgroup's request wait
+    uint32_t* rparts;    // host coherent: slots (workgroups the request ran on)
     uint64_t hits, misses;   // summed over completed requests
+    uint64_t split_requests, split_parts;   // requests run on more than one workgroup, and their parts
+    uint32_t parts_max;
+    uint64_t min_part;
+    serve_dev D;         // device: per-slot split state, rows, page counts; the part ring
     unsigned long long* ticket;   // device
     uint32_t* held;               // device, per file: 0, SERVE_HOLD_CLAIM, or the server's open fd + 1
     unsigned int* held_n;         // device: files held open by the server
@@ -211,11 +349,12 @@ static uint64_t serve_now_ns() {
                std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-static void serve_publish(gpufs_server* S, uint32_t file_id, uint32_t slot) {
+static void serve_publish(gpufs_server* S, uint32_t file_id, uint32_t slot, uint32_t parts = 1) {
     uint64_t k = S->next_ticket++;
     serve_req* e = &S->ring[k & S->ring_mask];
     e->file_id = file_id;
     e->slot = slot;
+    e->parts = parts;
     __atomic_store_n(&e->seq, k + 1, __ATOMIC_RELEASE);
 }
 
@@ -226,6 +365,13 @@ static void serve_free(gpufs_server* S) {
     (void)hipHostFree(S->rpages);
     (void)hipHostFree(S->done);
     (void)hipHostFree(S->halt);
+    (void)hipHostFree(S->rparts);
+    (void)hipFree(S->D.st);
+    (void)hipFree(S->D.row);
+    (void)hipFree(S->D.status);
+    (void)hipFree(S->D.pages);
+    (void)hipFree(S->D.ring);
+    (void)hipFree(S->D.ctl);
     (void)hipFree(S->ticket);
     (void)hipFree(S->held);
     (void)hipFree(S->held_n);
@@ -257,9 +403,18 @@ extern "C" int gpufs_serve_start(gpufs_ctx* ctx, const gpufs_serve_config* cfg, 
     const uint64_t frames = cc.cache_bytes / cc.page_size;
     S->frame_budget = cfg->frame_budget ? cfg->frame_budget : frames - frames / 10;
     S->sizes.assign(file_sizes, file_sizes + n_files);
+    S->parts_max = cfg->parts_max ? cfg->parts_max : 8;
+    S->min_part = (uint64_t)(cfg->min_part_kib ? cfg->min_part_kib : 512) << 10;
+    if (S->parts_max > 64 || S->min_part < 4096) {
+        delete S;
+        return -EINVAL;
+    }
     uint32_t rs = 1;
     while (rs < 4 * (cfg->slots + cfg->workgroups)) rs <<= 1;
     S->ring_mask = rs - 1;
+    // the part ring holds every extra part of the requests in flight
+    uint32_t ps = 1;
+    while (ps < cfg->slots * (S->parts_max - 1) + 1) ps <<= 1;
     const unsigned hf = hipHostMallocMapped | hipHostMallocCoherent;
     bool ok = hipHostMalloc((void**)&S->ring, sizeof(serve_req) * rs, hf) == hipSuccess &&
               hipHostMalloc((void**)&S->results, sizeof(uint32_t) * (size_t)cfg->slots * cfg->n_words, hf) == hipSuccess &&
@@ -267,6 +422,13 @@ extern "C" int gpufs_serve_start(gpufs_ctx* ctx, const gpufs_serve_config* cfg, 
               hipHostMalloc((void**)&S->rpages, sizeof(uint32_t) * 2 * (size_t)cfg->slots, hf) == hipSuccess &&
               hipHostMalloc((void**)&S->done, sizeof(uint64_t) * cfg->slots, hf) == hipSuccess &&
               hipHostMalloc((void**)&S->halt, sizeof(uint32_t), hf) == hipSuccess &&
+              hipHostMalloc((void**)&S->rparts, sizeof(uint32_t) * cfg->slots, hf) == hipSuccess &&
+              hipMalloc((void**)&S->D.st, sizeof(grep_split_state) * cfg->slots) == hipSuccess &&
+              hipMalloc((void**)&S->D.row, sizeof(uint32_t) * (size_t)cfg->slots * cfg->n_words) == hipSuccess &&
+              hipMalloc((void**)&S->D.status, sizeof(int32_t) * cfg->slots) == hipSuccess &&
+              hipMalloc((void**)&S->D.pages, sizeof(unsigned int) * 2 * (size_t)cfg->slots) == hipSuccess &&
+              hipMalloc((void**)&S->D.ring, sizeof(serve_part) * ps) == hipSuccess &&
+              hipMalloc((void**)&S->D.ctl, sizeof(serve_part_ctl)) == hipSuccess &&
               hipMalloc((void**)&S->ticket, sizeof(unsigned long long)) == hipSuccess &&
               hipMalloc((void**)&S->held, sizeof(uint32_t) * (size_t)n_files) == hipSuccess &&
               hipMalloc((void**)&S->held_n, sizeof(unsigned int)) == hipSuccess;
@@ -276,9 +438,15 @@ extern "C" int gpufs_serve_start(gpufs_ctx* ctx, const gpufs_serve_config* cfg, 
     }
     memset(S->ring, 0, sizeof(serve_req) * rs);
     memset(S->done, 0, sizeof(uint64_t) * cfg->slots);
+    memset(S->rparts, 0, sizeof(uint32_t) * cfg->slots);
     *S->halt = 0;
     gfs_dev hd;
-    if (hipMemsetAsync(S->ticket, 0, sizeof(unsigned long long), S->stream) != hipSuccess ||
+    S->D.ring_mask = ps - 1;
+    S->D.min_part = S->min_part;
+    if (hipMemsetAsync(S->D.st, 0, sizeof(grep_split_state) * cfg->slots, S->stream) != hipSuccess ||
+        hipMemsetAsync(S->D.ring, 0, sizeof(serve_part) * ps, S->stream) != hipSuccess ||
+        hipMemsetAsync(S->D.ctl, 0, sizeof(serve_part_ctl), S->stream) != hipSuccess ||
+        hipMemsetAsync(S->ticket, 0, sizeof(unsigned long long), S->stream) != hipSuccess ||
         hipMemsetAsync(S->held, 0, sizeof(uint32_t) * (size_t)n_files, S->stream) != hipSuccess ||
         hipMemsetAsync(S->held_n, 0, sizeof(unsigned int), S->stream) != hipSuccess ||
         hipMemcpyAsync(&hd, gpufs_device_state(ctx), sizeof(hd), hipMemcpyDeviceToHost, S->stream) != hipSuccess ||
@@ -295,7 +463,7 @@ extern "C" int gpufs_serve_start(gpufs_ctx* ctx, const gpufs_serve_config* cfg, 
     hipLaunchKernelGGL(k_serve, dim3(S->workgroups), dim3(S->threads), 0, S->stream, names_dev, name_stride,
                        n_files, dict_dev, cfg->n_words, cfg->chunk_bytes, S->ring, S->ring_mask, S->results,
                        S->rstatus, S->rpages, S->done, S->ticket, (const uint32_t*)S->halt, hd.wait_ticks, S->held,
-                       S->held_n, hold_cap);
+                       S->held_n, hold_cap, S->D, S->rparts);
     if (hipGetLastError() != hipSuccess) {
         serve_free(S);
         return -EIO;
@@ -328,6 +496,14 @@ extern "C" int gpufs_serve_run(gpufs_server* S, const uint32_t* file_ids, const 
         uint64_t p = (sz + S->page_size - 1) / S->page_size;
         return p < 1 ? 1 : p;
     };
+    // a file of at least two minimum parts is split over up to parts_max
+    // workgroups (its part 0 decides from the size it opens: the same count
+    // unless the file changed)
+    auto parts_for = [&](uint32_t i) -> uint32_t {
+        uint32_t f = file_ids[i];
+        uint64_t k = (f < S->nfiles ? S->sizes[f] : 0) / S->min_part;
+        return k < 2 ? 1u : (uint32_t)(k < S->parts_max ? k : S->parts_max);
+    };
     uint64_t last_done = t0;
     uint64_t guard_t = t0;
     uint64_t polls = 0;
@@ -344,6 +520,10 @@ extern "C" int gpufs_serve_run(gpufs_server* S, const uint32_t* file_ids, const 
             status_out[i] = S->rstatus[s];
             S->hits += S->rpages[2 * (size_t)s];
             S->misses += S->rpages[2 * (size_t)s + 1];
+            if (S->rparts[s] > 1) {
+                ++S->split_requests;
+                S->split_parts += S->rparts[s];
+            }
             latency_s_out[i] = (double)(tc - t0) * 1e-9 - arrival_s[i];
             last_done = tc;
             reserved -= slot_frames[s];
@@ -365,7 +545,7 @@ extern "C" int gpufs_serve_run(gpufs_server* S, const uint32_t* file_ids, const 
             slot_frames[s] = need;
             reserved += need;
             slot_seq[s] = S->next_ticket + 1;
-            serve_publish(S, file_ids[i], s);
+            serve_publish(S, file_ids[i], s, parts_for(i));
             ++inflight;
         }
         if (inflight > 0 && serve_now_ns() - guard_t > 120ull * 1000000000ull) return -ETIMEDOUT;
@@ -405,5 +585,12 @@ extern "C" int gpufs_serve_pages(gpufs_server* S, uint64_t* page_hits, uint64_t*
     if (!S || !page_hits || !page_misses) return -EINVAL;
     *page_hits = S->hits;
     *page_misses = S->misses;
+    return 0;
+}
+
+extern "C" int gpufs_serve_parts(gpufs_server* S, uint64_t* split_requests, uint64_t* parts) {
+    if (!S || !split_requests || !parts) return -EINVAL;
+    *split_requests = S->split_requests;
+    *parts = S->split_parts;
     return 0;
 }

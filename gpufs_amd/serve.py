@@ -23,7 +23,7 @@ class Server:
     """Resident request loop (gpufs_serve_*): start, run open-loop batches, stop."""
 
     def __init__(self, ctx, paths, sizes, dict_words, workgroups=8, slots=16, threads=1024, chunk=65536,
-                 frame_budget=0):
+                 frame_budget=0, parts_max=0, min_part_kib=0):
         import torch
         names = np.zeros((len(paths), STRIDE), dtype=np.uint8)
         for i, p in enumerate(paths):
@@ -35,7 +35,7 @@ class Server:
         self.dict = _dev(torch, np.ascontiguousarray(dict_words, dtype=np.uint8).reshape(-1))
         self.nw = int(np.asarray(dict_words).shape[0])
         self.sizes = np.ascontiguousarray(sizes, dtype=np.uint64)
-        cfg = _lib.ServeConfig(workgroups, threads, slots, chunk, self.nw, frame_budget)
+        cfg = _lib.ServeConfig(workgroups, threads, slots, chunk, self.nw, frame_budget, parts_max, min_part_kib)
         h = ctypes.c_void_p()
         torch.cuda.synchronize()
         _lib.check("gpufs_serve_start", _lib.lib().gpufs_serve_start(
@@ -61,6 +61,12 @@ class Server:
         h, m = ctypes.c_uint64(), ctypes.c_uint64()
         _lib.check("gpufs_serve_pages", _lib.lib().gpufs_serve_pages(self.h, ctypes.byref(h), ctypes.byref(m)))
         return int(h.value), int(m.value)
+
+    def parts(self):
+        """(requests split over several workgroups, their parts) since start."""
+        r, p = ctypes.c_uint64(), ctypes.c_uint64()
+        _lib.check("gpufs_serve_parts", _lib.lib().gpufs_serve_parts(self.h, ctypes.byref(r), ctypes.byref(p)))
+        return int(r.value), int(p.value)
 
     def stop(self):
         if self.h:

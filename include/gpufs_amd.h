@@ -229,6 +229,8 @@ typedef struct gpufs_serve_config {
     uint32_t chunk_bytes;   /* scan window */
     uint32_t n_words;       /* dictionary words (32-byte records) */
     uint32_t frame_budget;  /* frames in-flight requests may need; 0 = 90% of the pool */
+    uint32_t parts_max;     /* a large file's request runs on up to this many workgroups (<= 64); 0 = 8 */
+    uint32_t min_part_kib;  /* ... each part at least this long (>= 4); 0 = 512: smaller files take one */
 } gpufs_serve_config;
 int gpufs_serve_start(gpufs_ctx* ctx, const gpufs_serve_config* cfg, const char* names_dev,
                       uint32_t name_stride, uint32_t n_files, const uint64_t* file_sizes_host,
@@ -243,6 +245,11 @@ int gpufs_serve_stop(gpufs_server* srv);
  * gpufs_check_invariants return -EBUSY: their device copies cannot run
  * beside workgroups that hold every compute unit. */
 int gpufs_serve_pages(gpufs_server* srv, uint64_t* page_hits, uint64_t* page_misses);
+/* Requests completed since gpufs_serve_start that ran on more than one
+ * workgroup (a file of at least 2 x min_part_kib is split into parts handed to
+ * waiting workgroups), and the parts they ran as.  No counterpart in the
+ * reference, whose samples have no serving loop. */
+int gpufs_serve_parts(gpufs_server* srv, uint64_t* split_requests, uint64_t* parts);
 
 /* Record sums through ActivePointers (FatPointer / gvmmap with a per-workgroup
  * TLB of tlb_lines = 0, 8 or 32 lines; libgpufs/fat_pointer.cu.h, pattern of

@@ -64,6 +64,41 @@ def test_serve_alone_equals_mix_equals_cpu(tmp_path, orc):
         assert ctx.check_invariants() == 0
 
 
+def test_serve_splits_large_files(tmp_path, orc):
+    """Requests for large files run on several workgroups (parts handed to
+    waiting workgroups through the device part ring; the last part to finish
+    completes the request), cold and cached: every result equals the CPU
+    path, alone and in a Poisson mix with small requests in flight."""
+    say("test_serve_splits_large_files")
+    n = 40
+    voc = dg.vocab(4000, seed=91)
+    q = np.ascontiguousarray(voc[dg.query(4000, 1024, seed=92)])
+    sizes = dg.lognormal_sizes(n, seed=93, median=65536.0, sigma=2.0, lo=1, hi=6 << 20)
+    sizes[:6] = [6 << 20, (4 << 20) + 12345, 3 << 20, (1 << 20) + 7, 700001, 300000]   # several parts each
+    paths = [str(tmp_path / f"L{i:03d}.txt") for i in range(n)]
+    dg.text_files(paths, sizes, 94, voc, threads=8)
+    cpu, cst = orc.scan_files(paths, q, 65536, threads=8)
+    assert (cst == 0).all()
+    ids = np.concatenate([np.arange(6), dg.zipf_sample(n, 300, seed=95, s=1.1)]).astype(np.uint32)
+    arr = np.cumsum(dg.poisson_gaps(5000.0, ids.size, seed=96))
+    with make_ctx(page_size=64 << 10, cache_bytes=256 << 20, ring_depth=256) as ctx:
+        srv = Server(ctx, paths, sizes, q, workgroups=16, slots=8, parts_max=8, min_part_kib=128)
+        try:
+            for f in range(6):   # cold, then cached, alone
+                for rnd in ("cold", "warm"):
+                    c, st, lat, _ = srv.run([f], [0.0])
+                    assert st[0] == 0 and (c[0] == cpu[f]).all(), (f, rnd)
+            split, parts = srv.parts()
+            assert split >= 8 and parts > 2 * split, (split, parts)
+            c, st, lat, el = srv.run(ids, arr)
+            assert (st == 0).all(), st[:10]
+            for i, f in enumerate(ids):
+                assert (c[i] == cpu[f]).all(), (i, f)
+        finally:
+            srv.stop()
+        assert ctx.check_invariants() == 0
+
+
 def test_serve_survives_idle(tmp_path, orc, monkeypatch):
     """A server left idle for longer than the device wait limit (1 s here, so
     3 s of idling is three times it) still serves: its workgroups never give
