@@ -130,6 +130,34 @@ def link_peak_gbs(torch, nbytes=1 << 30, trials=5, to_host=False):
     return best
 
 
+def host_dirty_bytes():
+    """Dirty + Writeback bytes of the host page cache (/proc/meminfo), or None."""
+    try:
+        kb = 0
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith(("Dirty:", "Writeback:")):
+                    kb += int(line.split()[1])
+        return kb << 10
+    except (OSError, ValueError, IndexError):
+        return None
+
+
+def settle_host_writeback(limit_s=60.0, below=256 << 20):
+    """Flush the host page cache's dirty pages and wait (bounded) until fewer
+    than `below` bytes are dirty or under write-back, so that the disk
+    write-back a write leg leaves behind does not run under the next leg's
+    host reads.  Returns the seconds waited."""
+    t = time.perf_counter()
+    os.sync()
+    while time.perf_counter() - t < limit_s:
+        d = host_dirty_bytes()
+        if d is None or d < below:
+            break
+        time.sleep(0.2)
+    return time.perf_counter() - t
+
+
 def host_pwrite_gbs(path, size, threads, piece=4 << 20):
     """The host's own write rate into the page cache: `threads` threads pwrite
     `size` bytes in `piece`-byte pieces from one buffer (the host half of
@@ -197,6 +225,7 @@ def main():
         t = time.time()
         dg.random_file(path, size, dg.SEED_C1, threads=16)
         log(f"generated {path} ({size / 2**30:.1f} GiB) in {time.time() - t:.1f}s")
+        log(f"host write-back settled in {settle_host_writeback():.1f} s")
     _warm_host_cache([path])
 
     link = link_peak_gbs(torch)
@@ -415,6 +444,10 @@ def bench_write(args, ctx, torch, L, _lib, src, size, n, rng_bytes, stream, gpu_
     os.unlink(wpath)
     hp = host_pwrite_gbs(os.path.join(args.workdir, "pwrite_probe.bin"), size, args.host_threads)
     log(f"host pwrite ({args.host_threads} threads, 4 MiB pieces): {hp:.1f} GB/s")
+    d0 = host_dirty_bytes()
+    w = settle_host_writeback()
+    log(f"host write-back settled in {w:.1f} s (dirty + writeback {d0 and d0 >> 20} -> "
+        f"{(host_dirty_bytes() or 0) >> 20} MiB)")
     return {"workload": f"gwrite + gfsync + gclose: new {size / 2**30:.0f} GiB file, {n} workgroups x "
                         f"{rng_bytes >> 20} MiB ranges, emptied page cache per pass",
             "GBps_median": round(float(np.median(each)), 2), "GBps_each": [round(x, 2) for x in each],
@@ -505,6 +538,7 @@ def _text_pool(dg, voc, sub, n, seed, workdir):
         dg.text_files(paths, sizes, seed, voc, threads=16)
         open(stamp, "w").write(f"{n} {seed} {int(sizes.sum())}")
         log(f"generated {n} files, {sizes.sum() / 2**30:.2f} GiB in {time.time() - t:.1f}s")
+        log(f"host write-back settled in {settle_host_writeback():.1f} s")
     return paths, sizes
 
 
@@ -623,6 +657,7 @@ def bench_gemv(args, torch, L, _lib, dg):
         t = time.time()
         dg.matrix_file(path, rows, cols, dg.SEED_C2, threads=16)
         log(f"generated {path} ({size / 2**30:.1f} GiB) in {time.time() - t:.1f}s")
+        log(f"host write-back settled in {settle_host_writeback():.1f} s")
     _warm_host_cache([path])
     x = dg.normal_f32(cols, dg.SEED_C2 + 1)
     cache = 8 << 30

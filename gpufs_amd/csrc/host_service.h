@@ -490,13 +490,19 @@ static inline bool backlog(const RingView& rv, const TakeQueue& q) {
 // Worker `index` of the service.  Worker 0 polls the queue without sleeping
 // while the service runs, so a lone request (an open between two launches)
 // is picked up within a poll; the others spin for a while after their last
-// request, then sleep on the wake futex (2 us steps, 50 us after a longer
-// idle; timer slack lowered to 1 us so short sleeps are not stretched).
+// request or a launch's kick, then sleep on the wake futex, woken by a worker
+// that finds requests queued behind the one it took, or by their timeout
+// (50 us for the first millisecond of sleep, then 1 ms).  A sleep is a few
+// microseconds of CPU: idle workers sleeping in 2 us steps each kept a CPU
+// busy, and a process under a CPU quota (a container's cgroup) that spends
+// it all is stopped until the next period, which showed as multi-millisecond
+// stalls of the serving loop.
 static inline void worker_loop(RingView rv, TakeQueue* head, std::atomic<bool>* stop,
                                ServiceCounters* ctr, WakeState* wk, DescCache* dc, unsigned index) {
     prctl(PR_SET_TIMERSLACK, 1000UL, 0, 0, 0);
     WorkerBufs wb;
     uint64_t idle = 0;
+    uint32_t naps = 0;   // futex sleeps since the last request or kick
     uint32_t kicks = wk->kicks.load(std::memory_order_relaxed);
     for (;;) {
         uint32_t s;
@@ -508,6 +514,7 @@ static inline void worker_loop(RingView rv, TakeQueue* head, std::atomic<bool>* 
             else serve_slot(rv, s, *ctr, wb, dc);
             ctr->busy_ns.fetch_add(mono_ns() - t0, std::memory_order_relaxed);
             idle = 0;
+            naps = 0;
             continue;
         }
         // stopping: only once every published request is taken, so closes the
@@ -519,6 +526,7 @@ static inline void worker_loop(RingView rv, TakeQueue* head, std::atomic<bool>* 
         if (k != kicks) {
             kicks = k;
             idle = 0;
+            naps = 0;
         }
         if (index == 0 || idle < 20000) {
 #if defined(__x86_64__)
@@ -528,8 +536,9 @@ static inline void worker_loop(RingView rv, TakeQueue* head, std::atomic<bool>* 
             const uint32_t seen = wk->word.load(std::memory_order_acquire);
             wk->sleepers.fetch_add(1, std::memory_order_acq_rel);
             if (!backlog(rv, *head)) {
-                struct timespec ts = {0, idle < 200000 ? 2000 : 50000};
+                struct timespec ts = {0, naps < 20 ? 50000 : 1000000};
                 syscall(SYS_futex, (uint32_t*)&wk->word, FUTEX_WAIT_PRIVATE, seen, &ts, nullptr, 0);
+                ++naps;
             }
             wk->sleepers.fetch_sub(1, std::memory_order_acq_rel);
         }

@@ -1,6 +1,6 @@
 """Digest of rocprofv3 CSV output for one kernel (kernel trace and PMC runs).
 
-    python -m gpufs_amd.profsum trace   <dir> <kernel-substring> [--skip N]
+    python -m gpufs_amd.profsum trace   <dir> <kernel-substring> [--skip N | --last N]
     python -m gpufs_amd.profsum counter <dir> <kernel-substring> [--skip N]
     python -m gpufs_amd.profsum sq      <dir> <kernel-substring> [--skip N]
 
@@ -30,10 +30,12 @@ def _one(pattern):
     return files[0]
 
 
-def trace(d, kernel, skip):
+def trace(d, kernel, skip, last=0):
     rows = list(csv.DictReader(open(_one(os.path.join(d, "*kernel_trace.csv")))))
     ks = [r for r in rows if kernel in r["Kernel_Name"]]
     ks.sort(key=lambda r: int(r["Start_Timestamp"]))
+    if last:   # only the last N dispatches count (the earlier ones are listed as skipped)
+        skip = max(0, len(ks) - last)
     out = []
     for i, r in enumerate(ks):
         ns = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
@@ -128,8 +130,12 @@ def main(argv=None):
     ap.add_argument("dir")
     ap.add_argument("kernel")
     ap.add_argument("--skip", type=int, default=0)
+    ap.add_argument("--last", type=int, default=0, help="trace: count only the last N dispatches")
     a = ap.parse_args(argv)
-    {"trace": trace, "counter": counter, "sq": sq}[a.mode](a.dir, a.kernel, a.skip)
+    if a.mode == "trace":
+        trace(a.dir, a.kernel, a.skip, a.last)
+    else:
+        {"counter": counter, "sq": sq}[a.mode](a.dir, a.kernel, a.skip)
 
 
 if __name__ == "__main__":
