@@ -705,9 +705,11 @@ __device__ inline int rpc_wait(gfs_dev* g, uint32_t cq, uint64_t want) {
     fence_acq_sys();
     return 0;
 }
-// One synchronous request from a single thread (open / close / truncate / write).
-__device__ inline int64_t rpc_call_single(gfs_dev* g, uint32_t op, int32_t host_fd, int32_t flags,
-                                          uint64_t off, uint64_t len, const char* name, int* slot_out) {
+// One request from a single thread, posted: claims a slot, writes the name
+// (if any) and posts.  Returns the slot, whose answer rpc_finish_single
+// waits for, or -ETIMEDOUT (no slot within the wait limit).
+__device__ inline int rpc_post_single(gfs_dev* g, uint32_t op, int32_t host_fd, int32_t flags, uint64_t off,
+                                      uint64_t len, const char* name) {
     int s = -1;
     GFS_TRACE(23);
     uint64_t t0 = now_ticks();
@@ -731,6 +733,12 @@ __device__ inline int64_t rpc_call_single(gfs_dev* g, uint32_t op, int32_t host_
     GFS_TRACE(20);
     rpc_post(g, s, op, host_fd, flags, off, len, (uint32_t)s);
     GFS_TRACE(21);
+    return s;
+}
+// Waits for the answer to slot s of rpc_post_single; returns the request's
+// result.  *slot_out: the slot, left claimed for the caller to read extra
+// results from and release, or -1 (released, or quarantined on a timeout).
+__device__ inline int64_t rpc_finish_single(gfs_dev* g, int s, int* slot_out) {
     int w = rpc_wait(g, (uint32_t)s, 1ull);
     GFS_TRACE(22);
     if (w) {
@@ -743,6 +751,13 @@ __device__ inline int64_t rpc_call_single(gfs_dev* g, uint32_t op, int32_t host_
     if (slot_out) *slot_out = s;   // caller reads extra results, then releases
     else rpc_release(g, s);
     return res;
+}
+// One synchronous request from a single thread (open / close / truncate / write).
+__device__ inline int64_t rpc_call_single(gfs_dev* g, uint32_t op, int32_t host_fd, int32_t flags,
+                                          uint64_t off, uint64_t len, const char* name, int* slot_out) {
+    const int s = rpc_post_single(g, op, host_fd, flags, off, len, name);
+    if (s < 0) return s;
+    return rpc_finish_single(g, s, slot_out);
 }
 
 // ======================================================================
@@ -839,12 +854,18 @@ __device__ __noinline__ int reclaim_files(gfs_dev* g, uint64_t t0) {
     return ft_reclaim(g, F);
 }
 
-// Host open of entry e (new or reopened), whose state is OPENING.  Fills the
-// host fields and moves the state to OPEN, or to CLOSED dropping this
-// caller's open on failure.
-__device__ inline int open_host(gfs_dev* g, gfs_file* e, int flags, bool reopen) {
+// Host open of entry e (new or reopened), whose state is OPENING, in two
+// halves: open_host_post posts the request (returns its slot or a negative
+// code), open_host_finish waits for the answer, fills the host fields and
+// moves the state to OPEN, or to CLOSED dropping this caller's open on
+// failure.  Between the two the entry stays OPENING: joiners wait, and only
+// this caller may change its fields.
+__device__ inline int open_host_post(gfs_dev* g, gfs_file* e, int flags) {
+    return rpc_post_single(g, GFS_OP_OPEN, -1, flags, 0, 0, e->name);
+}
+__device__ inline int open_host_finish(gfs_dev* g, gfs_file* e, int flags, bool reopen, int posted) {
     int s = -1;
-    int64_t r = rpc_call_single(g, GFS_OP_OPEN, -1, flags, 0, 0, e->name, &s);
+    int64_t r = posted < 0 ? (int64_t)posted : rpc_finish_single(g, posted, &s);
     count(&g->ctr.rpc_opens, 1);
     uint64_t size = 0, ct = 0, mt = 0, ino = 0;
     if (s >= 0) {
@@ -877,12 +898,25 @@ __device__ inline int open_host(gfs_dev* g, gfs_file* e, int flags, bool reopen)
     fst_open_done(&e->st, P, true, changed);
     return 0;
 }
+__device__ inline int open_host(gfs_dev* g, gfs_file* e, int flags, bool reopen) {
+    return open_host_finish(g, e, flags, reopen, open_host_post(g, e, flags));
+}
+
+// A reopen of a closed entry whose host open was posted and not waited for
+// (open_single with a gfs_open_pend): open_finish completes it.
+struct gfs_open_pend {
+    int idx, slot, flags;
+    uint32_t pending;   // 1: posted, open_finish owes the completion
+};
 
 // Single-thread open (reference: single_thread_open, libgpufs/fs_calls.cu:152-225).
 // The name is looked up (or inserted) and joined inside the open gate, so no
 // reclaim pass (ftable.h) can take the entry back in between; a full table
-// runs one and looks again.
-__device__ __noinline__ int open_single(const char* name, int flags) {
+// runs one and looks again.  With `pend`, a reopen of a closed entry returns
+// the entry at once with pend->pending = 1, its host open not yet posted
+// (open_post, then open_finish; the entry stays OPENING until then); every
+// other case completes here.
+__device__ __noinline__ int open_single(const char* name, int flags, gfs_open_pend* pend = nullptr) {
     gfs_dev* g = gfs_g;
     gname nm;
     name_load(name, nm);
@@ -907,6 +941,13 @@ __device__ __noinline__ int open_single(const char* name, int flags) {
             ft_gate_leave(g, F);
             if (o == FST_JOINED) return (int)idx;
             if (o < 0) return o;
+            if (pend) {
+                pend->idx = (int)idx;
+                pend->flags = flags;
+                pend->slot = -ETIMEDOUT;   // until open_post
+                pend->pending = 1;
+                return (int)idx;
+            }
             return open_host(g, &g->files[idx], flags, true) < 0 ? E_IPC_OPEN_ERROR : (int)idx;
         }
         ft_gate_leave(g, F);
@@ -916,6 +957,19 @@ __device__ __noinline__ int open_single(const char* name, int flags) {
         if (c == 0) return E_FSTABLE_FULL;
         if (F.expired()) return -ETIMEDOUT;
     }
+}
+
+// Posts the host open of a pending reopen (one thread).
+__device__ inline void open_post(gfs_open_pend* pend) {
+    pend->slot = open_host_post(gfs_g, &gfs_g->files[pend->idx], pend->flags);
+}
+// Completes a posted reopen (one thread): the entry's index, or
+// E_IPC_OPEN_ERROR / a negative code when the host open failed.
+__device__ __noinline__ int open_finish(gfs_open_pend* pend) {
+    gfs_dev* g = gfs_g;
+    pend->pending = 0;
+    return open_host_finish(g, &g->files[pend->idx], pend->flags, true, pend->slot) < 0 ? E_IPC_OPEN_ERROR
+                                                                                          : pend->idx;
 }
 
 } // namespace gfs
@@ -1561,6 +1615,62 @@ __device__ __noinline__ int pin_pages(gfs_dev* g, int fd, uint64_t pb, uint32_t 
     }
     __syncthreads();
     return 0;
+}
+// Pins pages [pb, pb+cnt) (cnt <= 64) of entry `fd` only if every one of them
+// is cached (VALID in a frame): no frame is claimed, filled or created and no
+// host descriptor is used, so the entry need not be open (workgroup-collective).
+// For reads that run while the file's reopen is still at the host: the entry
+// is then OPENING, its tree is neither grown nor dropped except by the opener
+// when the host finds the file changed, which also bumps the file's version;
+// the caller compares the version after the open and discards what it read if
+// it moved.  Node and frame indexes are bounds-checked, since nodes of a
+// dropped tree may be reused under the walk.  Returns 0 (every page pinned,
+// frames in gfs_ctl.pinf, released by unpin_pages) or 1 (nothing pinned).
+__device__ __noinline__ int pin_cached(gfs_dev* g, int fd, uint64_t pb, uint32_t cnt) {
+    __syncthreads();
+    if (wave() == 0) {
+        const uint32_t ln = lane();
+        bool ok = true;
+        uint32_t f = GFS_NOFRAME;
+        if (ln < cnt) {
+            const uint64_t page = pb + ln;
+            const uint64_t rw = ld_rlx(&g->files[fd].root);
+            const uint32_t h = (uint32_t)(rw >> 40);
+            uint64_t node = rw & ((1ull << 40) - 1);
+            ok = rw != 0 && h >= 1 && h <= GFS_MAX_HEIGHT && page < tree_cap(h) && node != 0 && node < g->node_cap;
+            for (int lvl = (int)h - 1; ok && lvl >= 1; --lvl) {
+                const uint64_t c =
+                    ld_rlx(&g->nodes[node * GFS_FANOUT + ((page >> (GFS_FANOUT_LOG * lvl)) & (GFS_FANOUT - 1))]);
+                ok = c != 0 && c < g->node_cap;
+                node = c;
+            }
+            if (ok) {
+                const uint64_t v = ld_rlx(&g->nodes[node * GFS_FANOUT + (page & (GFS_FANOUT - 1))]);
+                ok = v != 0 && v != GFS_LEAF_RESERVED && leaf_frame(v) < g->n_frames;
+                if (ok) {
+                    f = leaf_frame(v);
+                    const uint64_t old = fadd_rlx(&g->meta[f], 1ull);
+                    ok = meta_gen(old) == leaf_gen(v) && meta_state(old) == GFS_F_VALID;
+                    if (!ok) fadd_rlx(&g->meta[f], ~0ull);
+                }
+            }
+        }
+        const uint64_t bad = __ballot(ln < cnt && !ok);
+        if (ln < cnt) {
+            if (bad) {
+                if (ok) fadd_rlx(&g->meta[f], ~0ull);   // give back the pins taken
+            } else {
+                gfs_ctl.pinf[ln] = f;
+                g->fref[f] = 1;
+            }
+        }
+        if (ln == 0) {
+            gfs_ctl.a = bad;
+            gfs_memo.fd = -1;
+        }
+    }
+    __syncthreads();
+    return uni(gfs_ctl.a) ? 1 : 0;
 }
 // Whether page `page` of open file `fd` is cached (VALID in a frame), looked
 // up without pinning or creating anything (one thread).  Advisory: the answer

@@ -495,11 +495,18 @@ __device__ inline int grep_part_windows(grep_lds& S, int fd, uint32_t chunk, uin
 // instead of a gread per window between the scans.  Same windows, same
 // results.  Needs a workgroup large enough that GREP_PF 16-byte units per
 // thread cover the largest window; returns 1 (nothing done) otherwise.
+// Whether T threads can hold the largest window of grep_part_pinned.
+__device__ __forceinline__ bool grep_pinned_fits(uint32_t T, uint32_t chunk) {
+    return (uint64_t)T * GREP_PF * 16 >= (uint64_t)chunk + 64 + 16 + 16;
+}
+// SPEC: pages are pinned with pin_cached (no host read, the entry need not be
+// open); a page that is not cached ends the part with 1, its counts partial.
+template <bool SPEC = false>
 __device__ inline int grep_part_pinned(grep_lds& S, int fd, uint32_t chunk, uint64_t size, uint64_t p_begin,
                                        uint64_t p_end, uint32_t* longrun, bool held) {
     gfs_dev* g = gfs_g;
     const uint32_t T = gfs::nthreads();
-    if ((uint64_t)T * GREP_PF * 16 < (uint64_t)chunk + 64 + 16 + 16) return 1;
+    if (!grep_pinned_fits(T, chunk)) return 1;
     const uint32_t lp = g->log_page;
     const uint64_t last_page = (size - 1) >> lp;
     // window at cursor c: bytes [c - mis, rend) land at corpus[0, len)
@@ -518,7 +525,7 @@ __device__ inline int grep_part_pinned(grep_lds& S, int fd, uint32_t chunk, uint
     // pinf[i], and stays pinned after the return
     uint64_t pb = held ? 0 : wbase >> lp;
     uint32_t cnt = (uint32_t)(last_page - pb + 1 < 64 ? last_page - pb + 1 : 64);
-    int st = held ? 0 : gfs::pin_pages(g, fd, pb, cnt);
+    int st = held ? 0 : (SPEC ? gfs::pin_cached(g, fd, pb, cnt) : gfs::pin_pages(g, fd, pb, cnt));
     if (st) {
         *longrun = 0;
         return st;
@@ -564,7 +571,7 @@ __device__ inline int grep_part_pinned(grep_lds& S, int fd, uint32_t chunk, uint
                 gfs::unpin_pages(g, cnt);
                 pb = nwbase >> lp;
                 cnt = (uint32_t)(last_page - pb + 1 < 64 ? last_page - pb + 1 : 64);
-                st = gfs::pin_pages(g, fd, pb, cnt);
+                st = SPEC ? gfs::pin_cached(g, fd, pb, cnt) : gfs::pin_pages(g, fd, pb, cnt);
                 if (st) {
                     cnt = 0;
                     break;
@@ -749,7 +756,12 @@ struct grep_split_state {
     int32_t err;        // first error of a part
     uint32_t cut;       // part 0's chain: some reference window cuts a token
     uint64_t c_last;    // part 0's chain: start of the reference's last window
-    uint64_t tag;       // part 0's decision: epoch << 32 | (K_f + 1)
+    uint64_t tag;       // part 0's decision: epoch << 32 | spec << 30 | (K_f + 1)
+    uint64_t spec_size; // spec decision: the file's size as its last open left it
+    uint32_t spec_ver;  // spec decision: the file's version then
+    uint32_t spec_idx;  // spec decision: its file-table entry
+    uint32_t p0_open;   // part 0 left its open for the last part to close
+    uint32_t pad;
 };
 
 // Part boundaries: part 0 also walks the reference's window chain before its
@@ -830,13 +842,28 @@ __device__ __noinline__ int grep_split_finish(grep_lds& S, int fd, uint32_t nw, 
 // on_arrive() is called by every thread of each part right before the part
 // counts itself arrived.  Returns true in the workgroup that completed the
 // file (the last to arrive, or the only one) and wrote *status.
+//
+// `spec` (split form only): the scan of a cached file overlaps its reopen.
+// When part 0's open is a reopen of a closed entry (a host round trip), it
+// posts the open and, if every page is still cached, pins them all and
+// publishes a decision from the size and version the entry's last open left,
+// then walks the chain and scans its part while the host answers; the other
+// parts scan their ranges from the cached pages (pin_cached: no host read)
+// before they join the open.  Each part then checks that the open found the
+// same file (same entry, version and size): a reopen that finds the file
+// changed on the host bumps the version, and that part sets the long-run
+// flag, so the last part recounts the whole file through the open file with
+// the reference's loop.  A part that finds a page no longer cached scans its
+// range again through the open file.  Results are those of opening first.
 template <class OnSize, class OnArrive>
 __device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t nw, uint32_t chunk, uint32_t* row,
                                        int32_t* status, uint32_t j, uint32_t parts, uint64_t min_part,
                                        grep_split_state* fs, uint32_t target, uint32_t epoch, bool split_cold,
-                                       OnSize&& on_size, OnArrive&& on_arrive) {
+                                       bool spec, OnSize&& on_size, OnArrive&& on_arrive) {
     const uint32_t T = gfs::nthreads(), t = gfs::tid();
     __shared__ uint32_t fin_flag;
+    __shared__ uint32_t spec_sh[2];   // spec decision: entry, version
+    __shared__ gfs::gfs_open_pend opend;
     const bool single = fs == nullptr;
     for (uint32_t q = t; q < nw; q += T) S.counts[q] = 0;
     if (!single && j == 0) {
@@ -850,14 +877,70 @@ __device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t n
             fs->err = 0;
             fs->cut = 0;
             fs->c_last = 0;
+            fs->p0_open = 0;
         }
         __builtin_amdgcn_s_waitcnt(0);
     }
-    const int fd = gopen(name, O_GRDONLY);
+    const bool sp = spec && !single;
+    int fd = -1;
+    uint32_t pend = 0;   // part 0: its reopen is posted, not yet completed
+    // part 0 after a posted reopen: the other parts join the open only after
+    // their scans, later than part 0 ends, so part 0's close would be the last
+    // one and close the file under them; it leaves its open to the last part
+    bool defer_close = false;
+    if (sp && j == 0) {
+        __syncthreads();
+        if (t == 0) {
+            gfs_hist.fd = -1;
+            gfs_memo.fd = -1;
+            opend.pending = 0;
+            gfs_ctl.ret = gfs::open_single(name, O_GRDONLY, &opend);
+            fin_flag = opend.pending;
+        }
+        __syncthreads();
+        fd = (int)gfs::uni(gfs_ctl.ret);
+        pend = gfs::uni(fin_flag);
+    } else if (!sp) {
+        fd = gopen(name, O_GRDONLY);
+    }   // the other parts of the split form open after the decision (below)
     GFS_TRACE(42);
     uint64_t size = 0, kf = 1, cl = 0;
-    int st = fd < 0 ? fd : 0;
+    int st = fd < 0 && (!sp || j == 0) ? fd : 0;
     uint32_t lr = 0, cut = 0;
+    uint32_t held = 0;   // part 0: every page of the file pinned (page i's frame in pinf[i])
+    uint32_t sver = 0, sidx = 0;
+    uint32_t maxpin = gfs_g->n_frames / 8;
+    maxpin = maxpin > 64 ? 64 : (maxpin < 2 ? 2 : maxpin);
+    if (pend) {
+        // the entry is OPENING and this workgroup's until open_finish: its
+        // size and version stay as the last open left them
+        if (t == 0) {
+            gfs_file* e = &gfs_g->files[fd];
+            S.fsize = e->size;
+            fin_flag = fst_version(gfs::ld_rlx(&e->st));
+        }
+        __syncthreads();
+        const uint64_t ssz = gfs::uni((uint64_t)S.fsize);
+        sver = gfs::uni(fin_flag);
+        const uint64_t np = (ssz + gfs_g->page_size - 1) >> gfs_g->log_page;
+        // (the chain and the part scan then read only the pinned frames: no
+        // gread, which needs the open completed)
+        if (ssz && np <= maxpin && grep_pinned_fits(T, chunk) && gfs::pin_cached(gfs_g, fd, 0, (uint32_t)np) == 0) {
+            held = (uint32_t)np;
+        } else {
+            // not all cached: the open first, then the usual path
+            __syncthreads();
+            if (t == 0) {
+                gfs::open_post(&opend);
+                gfs_ctl.ret = gfs::open_finish(&opend);
+            }
+            __syncthreads();
+            fd = (int)gfs::uni(gfs_ctl.ret);
+            if (fd < 0) st = fd;
+            pend = 0;
+        }
+    }
+
     // Part 0 decides the split.  A file that is not cached (its last page is
     // not) is read by part 0 alone: one workgroup streams a cold file best
     // (its sequential read-ahead keeps a batch in flight while it scans; cold
@@ -869,7 +952,7 @@ __device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t n
     bool warm = true;
     if (t == 0) {
         uint32_t d = 0;
-        if (fd >= 0) S.fsize = fstat(fd);
+        if (fd >= 0) S.fsize = fstat(fd);   // with a posted reopen: the size the last open left
         if (single || j == 0) {
             uint64_t k = 1;
             if (fd >= 0 && !single) {
@@ -881,8 +964,17 @@ __device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t n
                 warm = S.fsize == 0 || gfs::page_cached(gfs_g, fd, (S.fsize - 1) >> gfs_g->log_page);
             }
             d = (uint32_t)k + 1;
+            if (held) {
+                // a decision from the cached pages; the parts check it after their open
+                d |= 0x40000000u;
+                fs->spec_size = S.fsize;
+                fs->spec_ver = sver;
+                fs->spec_idx = (uint32_t)fd;
+            }
             if (!single)
                 __hip_atomic_store(&fs->tag, ((uint64_t)epoch << 32) | d, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            // the other parts scan while the host opens the file
+            if (held) gfs::open_post(&opend);
         } else {
             const uint64_t w0 = gfs::now_ticks();
             // relaxed polls, one acquire once the decision is seen (an acquire
@@ -896,28 +988,86 @@ __device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t n
             gfs::fence_acq_agent();
             // no decision within the wait limit: this part scans nothing and
             // the file reports -ETIMEDOUT
+            if (d & 0x40000000u) {
+                S.fsize = fs->spec_size;
+                spec_sh[0] = fs->spec_idx;
+                spec_sh[1] = fs->spec_ver;
+            }
         }
         fin_flag = d | (warm ? 0x80000000u : 0u);
     }
     __syncthreads();
+    bool specd = false;   // a part other than 0 reading under a spec decision
     {
         const uint32_t d = gfs::uni(fin_flag);
         warm = (d >> 31) != 0;
-        kf = (d & 0x7FFFFFFFu) ? (d & 0x7FFFFFFFu) - 1 : 0;
-        if ((d & 0x7FFFFFFFu) == 0 && !st) st = -ETIMEDOUT;
-        if (fd >= 0) size = gfs::uni((uint64_t)S.fsize);
+        specd = j > 0 && ((d >> 30) & 1u);
+        kf = (d & 0x3FFFFFFFu) ? (d & 0x3FFFFFFFu) - 1 : 0;
+        if ((d & 0x3FFFFFFFu) == 0 && !st) st = -ETIMEDOUT;
+        if (fd >= 0 || specd) size = gfs::uni((uint64_t)S.fsize);
+        if (specd) {
+            sidx = gfs::uni(spec_sh[0]);
+            sver = gfs::uni(spec_sh[1]);
+        }
     }
     __syncthreads();
+    if (sp && j > 0 && !specd && !st) {
+        // the decision followed part 0's open: join it now
+        fd = gopen(name, O_GRDONLY);
+        if (t == 0 && fd >= 0) S.fsize = fstat(fd);
+        __syncthreads();
+        if (fd < 0) st = fd;
+        else size = gfs::uni((uint64_t)S.fsize);
+    }
     GFS_TRACE(43);
     on_size((uint32_t)kf);
-    if (fd >= 0) {
+    if (specd) {
+        // This part's range from the cached pages, before the open; then the
+        // open, and the check that it found the file read.
+        int sst = 0;
+        const bool mine = j < kf && size && !st;
+        if (mine) {
+            const uint64_t pb = split_point(size, j, kf), pe = split_point(size, j + 1, kf);
+            sst = grep_part_pinned<true>(S, (int)sidx, chunk, size, pb, pe, &lr, false);
+        }
+        fd = gopen(name, O_GRDONLY);
+        if (t == 0) {
+            uint32_t ok = 0;
+            if (fd == (int)sidx) {
+                gfs_file* e = &gfs_g->files[fd];
+                ok = fst_version(gfs::ld_rlx(&e->st)) == sver && e->size == size;
+            }
+            fin_flag = ok;
+        }
+        __syncthreads();
+        const bool same = gfs::uni(fin_flag) != 0;
+        if (fd < 0) {
+            st = fd;
+        } else if (!same) {
+            lr = 1;   // the file changed on the host: the last part recounts it
+        } else if (mine && sst) {
+            // a page left the cache meanwhile: the range again, through the open file
+            for (uint32_t q = t; q < nw; q += T) S.counts[q] = 0;
+            lr = 0;
+            __syncthreads();
+            const uint64_t pb = split_point(size, j, kf), pe = split_point(size, j + 1, kf);
+            st = grep_part_pinned(S, fd, chunk, size, pb, pe, &lr, false);
+            if (st == 1) st = grep_part_windows(S, fd, chunk, size, pb, pe, &lr);
+        }
+        if (mine && fd >= 0) {
+            for (uint32_t q = t; q < nw; q += T) {
+                const uint32_t k = S.canon[q];
+                const uint32_t v = k == 0xFFFF ? 0u : S.counts[k];
+                if (v) atomicAdd(&row[q], v);
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+        }
+        GFS_TRACE(46);
+    } else if (fd >= 0) {
         // Part 0 of a cached file that fits one pin batch pins all of it once
-        // for both the chain and its own part.
-        uint32_t held = 0;
-        if (j == 0 && size && warm && !st) {
+        // for both the chain and its own part (already, with a posted reopen).
+        if (j == 0 && size && warm && !st && !held) {
             const uint64_t np = (size + gfs_g->page_size - 1) >> gfs_g->log_page;
-            uint32_t maxpin = gfs_g->n_frames / 8;
-            maxpin = maxpin > 64 ? 64 : (maxpin < 2 ? 2 : maxpin);
             if (np <= maxpin) {
                 st = gfs::pin_pages(gfs_g, fd, 0, (uint32_t)np);
                 if (!st) held = (uint32_t)np;
@@ -932,7 +1082,35 @@ __device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t n
             }
         }
         GFS_TRACE(45);
-        if (j < kf && size && !st) {
+        bool changed = false;
+        if (pend) {
+            // Part 0: the host's answer (it came during the chain), then the
+            // same check as the other parts.  Completing the open here, before
+            // this part's scan, lets the other parts join it without waiting
+            // when their scans end.
+            __syncthreads();
+            if (t == 0) {
+                const int r = gfs::open_finish(&opend);
+                uint32_t ok = 0;
+                if (r >= 0) {
+                    gfs_file* e = &gfs_g->files[r];
+                    ok = fst_version(gfs::ld_rlx(&e->st)) == sver && e->size == size;
+                }
+                gfs_ctl.ret = r;
+                fin_flag = ok;
+            }
+            __syncthreads();
+            fd = (int)gfs::uni(gfs_ctl.ret);
+            if (fd < 0) {
+                if (!st) st = fd;
+            } else if (!gfs::uni(fin_flag)) {
+                changed = true;
+                lr = 1;   // the file changed on the host: the last part recounts it
+            }
+            defer_close = fd >= 0;
+            pend = 0;
+        }
+        if (j < kf && size && !st && !changed) {
             const uint64_t pb = split_point(size, j, kf), pe = split_point(size, j + 1, kf);
             // a cached file is read from pinned frames, a cold one with gread
             // (whose read-ahead streams it from the host)
@@ -954,7 +1132,7 @@ __device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t n
         }
         GFS_TRACE(46);
         if (held) gfs::unpin_pages(gfs_g, held);
-        if (j == 0 && size && !st && !warm) {
+        if (fd >= 0 && j == 0 && size && !st && !warm) {
             st = grep_chain_last(S, fd, size, chunk, &cl, &cut, false);
             if (t == 0 && !single) {
                 fs->c_last = cl;
@@ -983,6 +1161,7 @@ __device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t n
     if (t == 0) {
         if (lr) atomicOr(&fs->longrun, 1u);
         if (st) atomicCAS(&fs->err, 0, st);
+        if (defer_close) fs->p0_open = 1;
         const uint32_t old = __hip_atomic_fetch_add(&fs->arrived, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         fin_flag = old + 1 == (target ? target : (uint32_t)kf) ? 1u : 0u;
     }
@@ -994,20 +1173,23 @@ __device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t n
             S.status = __hip_atomic_load(&fs->err, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
             fin_flag = __hip_atomic_load(&fs->longrun, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) | fs->cut;
             S.fsize = fs->c_last;
+            spec_sh[0] = j != 0 && fs->p0_open;
         }
         __syncthreads();
         st = gfs::uni(S.status);
         const uint32_t redo = gfs::uni(fin_flag);
         const uint64_t cl = gfs::uni((uint64_t)S.fsize);
+        const bool close_p0 = gfs::uni(spec_sh[0]) != 0;
         if (!st && size) st = grep_split_finish(S, fd, nw, chunk, size, row, redo, cl);
         GFS_TRACE(48);
+        if (close_p0 && fd >= 0) gclose(fd);   // part 0's open (the same entry)
         const int c = fd >= 0 ? gclose(fd) : 0;
         if (t == 0) *status = st ? st : c;
         __syncthreads();
         GFS_TRACE(49);
         return true;
     }
-    if (fd >= 0) gclose(fd);
+    if (fd >= 0 && !defer_close) gclose(fd);
     GFS_TRACE(49);
     return false;
 }

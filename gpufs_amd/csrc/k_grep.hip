@@ -36,12 +36,18 @@ __global__ __launch_bounds__(1024) void k_grep(const char* names, uint32_t strid
                                                grep_split_state* fstate, uint32_t epoch) {
     __shared__ __attribute__((aligned(16))) grep_lds S;
     const uint32_t f = blockIdx.x % n_files, j = blockIdx.x / n_files;
+    // Part 0 builds the dictionary once it has published its decision, which
+    // the other parts wait for; they build theirs meanwhile.
     GFS_TRACE(40);
-    grep_build_dict(S, dict, nw);
+    if (j != 0) grep_build_dict(S, dict, nw);
     GFS_TRACE(41);
     [[clang::always_inline]] grep_file_split(S, names + (size_t)f * stride, nw, chunk, counts_out + (size_t)f * nw,
                                              status + f, j, parts, min_part, fstate ? fstate + f : nullptr, parts,
-                                             epoch, false, [](uint32_t) {}, [] {});
+                                             epoch, false, true,
+                                             [&](uint32_t) {
+                                                 if (j == 0) grep_build_dict(S, dict, nw);
+                                             },
+                                             [] {});
 }
 
 // ---------------------------------------------------------------------------
@@ -141,7 +147,7 @@ __global__ __launch_bounds__(1024) void k_grep_batched(const char* names, uint32
         if (j == 0 && t == 0 && times) times[2 * (size_t)f] = gfs::now_ticks();
         [[clang::always_inline]] grep_file_split(
             S, names + (size_t)f * stride, nw, chunk, counts_out + (size_t)f * nw, status + f, j, parts, min_part,
-            fs + f, j == 0 ? 0u : kfi, 0u, false, [&](uint32_t kf) {
+            fs + f, j == 0 ? 0u : kfi, 0u, false, false, [&](uint32_t kf) {
                 if (j != 0 || t != 0) return;
                 if (kf > 1) {
                     const uint32_t r0 = atomicAdd(&q->tail, kf - 1);

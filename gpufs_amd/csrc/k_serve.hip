@@ -179,7 +179,7 @@ __global__ __launch_bounds__(1024) void k_serve(const char* names, uint32_t stri
             // an extra part of another workgroup's request
             const bool fin = grep_file_split(S, names + (size_t)f * stride, nw, chunk, D.row + (size_t)s * nw,
                                              D.status + s, pj, np, D.min_part, D.st + s, np, (uint32_t)sq, true,
-                                             [](uint32_t) {}, [&] { add_pages(s); });
+                                             false, [](uint32_t) {}, [&] { add_pages(s); });
             if (fin) complete_split(s, sq, np);
             __syncthreads();
             continue;
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(1024) void k_serve(const char* names, uint32_t stri
             uint32_t kfr = 1;
             const bool fin = grep_file_split(
                 S, names + (size_t)f * stride, nw, chunk, D.row + (size_t)s * nw, D.status + s, 0, np, D.min_part,
-                D.st + s, 0, (uint32_t)sq, true,
+                D.st + s, 0, (uint32_t)sq, true, false,
                 [&](uint32_t kf) {
                     kfr = kf;
                     if (t != 0 || kf < 2) return;

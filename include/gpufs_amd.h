@@ -168,7 +168,7 @@ int gpufs_launch_grep_split(gpufs_ctx* ctx, const char* names_dev, uint32_t name
 /* gpufs_launch_grep_split without any allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under
  * hipGraph capture, serving loops, harnesses that forbid hipMalloc).  The
- * per-file state lives in a fixed device array of the library (32 B x 65,536
+ * per-file state lives in a fixed device array of the library (56 B x 65,536
  * files per GPU, held until the process ends); n_files above 65,536 returns
  * -E2BIG.  Calls on one GPU share that array (each call clears what it uses
  * first), so calls that overlap in time on one GPU race. */

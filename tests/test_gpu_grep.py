@@ -205,3 +205,40 @@ def test_grep_split_under_eviction(tmp_path, orc):
         s = ctx.stats()
         assert s["evictions"] > 0, s
         assert ctx.check_invariants() == 0
+
+
+def test_grep_split_sees_changed_file(tmp_path, orc):
+    """A warm split scan reads the cached pages while the files' reopens are
+    at the host; a file changed on the host in between (same size, larger,
+    smaller) must be counted as it is now, through the reopened file."""
+    say("test_grep_split_sees_changed_file")
+    voc = dg.vocab(3000, seed=51)
+    q = voc[dg.query(3000, 1024, seed=52)]
+    paths = [str(tmp_path / f"c{i}.txt") for i in range(5)]
+    dg.text_files(paths, [1 << 20] * 4 + [300000], 53, voc, threads=4)
+    with make_ctx(page_size=65536, cache_bytes=64 << 20) as ctx:
+        for _ in range(2):   # cold, then warm (every page cached, entries closed)
+            counts, status = grep(ctx, paths, q, 65536, parts=4, min_part=65536)
+            assert (status == 0).all(), status
+        cpu, st = orc.scan_files(paths, q, 65536, threads=4)
+        assert (st == 0).all()
+        assert (counts == cpu).all()
+        # rewrite three files in place (same inode): same size, larger, smaller
+        repl = [str(tmp_path / f"n{i}.txt") for i in range(3)]
+        dg.text_files(repl, [1 << 20, (1 << 20) + 70000, 500000], 54, voc, threads=4)
+        for src, dst in zip(repl, paths[:3]):
+            data = open(src, "rb").read()
+            with open(dst, "r+b") as f:
+                f.write(data)
+                f.truncate(len(data))
+            m = os.stat(dst).st_mtime_ns
+            os.utime(dst, ns=(m + 2_000_000_000, m + 2_000_000_000))
+        cpu2, st = orc.scan_files(paths, q, 65536, threads=4)
+        assert (st == 0).all()
+        assert not (cpu2[:3] == cpu[:3]).all()
+        for k in range(2):
+            counts, status = grep(ctx, paths, q, 65536, parts=4, min_part=65536)
+            assert (status == 0).all(), (k, status)
+            for i in range(len(paths)):
+                assert (counts[i] == cpu2[i]).all(), (k, i)
+        assert ctx.check_invariants() == 0
