@@ -429,9 +429,16 @@ static inline bool try_take(const RingView& rv, TakeQueue& q, uint32_t* slot, in
         const uint64_t t = h + k;
         const uint64_t e = __atomic_load_n(&rv.sq[t & q.mask], __ATOMIC_ACQUIRE);
         if ((e >> 16) != (t & 0xFFFFFFFFFFFFull)) continue;   // not published (yet)
+#ifdef GFS_HOST_TAKE_HOOK
+        GFS_HOST_TAKE_HOOK();   // (tests: a worker delayed between the two loads)
+#endif
         std::atomic<uint64_t>& c = q.claim[t & q.mask];
         uint64_t cv = c.load(std::memory_order_relaxed);
-        if (cv == t + 1) continue;   // taken
+        // Taken: by this lap's claim (t + 1), or, when this worker was delayed
+        // since it read the entry, by a later lap's (the ticket was served, its
+        // slot reused and the queue wrapped): a claim word only grows.  Claiming
+        // it again would serve whatever request its slot holds now a second time.
+        if (cv >= t + 1) continue;
         const uint32_t s = (uint32_t)(e & 0xFFFF);
         const int64_t fd = s == GFS_SQ_NOSLOT ? __atomic_load_n(&rv.closeq[t & q.mask], __ATOMIC_RELAXED) : -1;
         if (!c.compare_exchange_strong(cv, t + 1, std::memory_order_acq_rel)) continue;

@@ -21,6 +21,15 @@
 #include <vector>
 #include <fcntl.h>
 #include <unistd.h>
+// Now and then a worker stalls between reading a submission entry and
+// claiming its ticket, long enough for the queue to wrap: it must not take
+// the ticket again once a later lap has claimed the entry.
+static std::atomic<uint32_t> take_hook_calls{0};
+#define GFS_HOST_TAKE_HOOK()                                                                  \
+    do {                                                                                      \
+        if (take_hook_calls.fetch_add(1, std::memory_order_relaxed) % 61 == 0)                \
+            std::this_thread::sleep_for(std::chrono::microseconds(400));                      \
+    } while (0)
 #include "../../gpufs_amd/csrc/host_service.h"
 
 static int failures = 0;
