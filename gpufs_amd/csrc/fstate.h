@@ -49,8 +49,10 @@ template <class P> GFS_HD int fst_open(uint64_t* w, P& p) {
     // Look before adding: when a launch's workgroups all open the same closed
     // file, one of them wins the CAS to OPENING and the others wait on plain
     // loads, then join with one add each (every atomic on the word queues
-    // behind the others at one L2 channel).
-    uint64_t cur = p.load(w);
+    // behind the others at one L2 channel).  The look is relaxed: the add or
+    // CAS that acts on it acquires (an acquire load at agent scope would also
+    // invalidate the opener's L2).
+    uint64_t cur = p.load_rlx(w);
     for (;;) {
         const uint32_t s = fst_state(cur);
         if (s == GFS_FS_OPEN) {
@@ -85,7 +87,7 @@ template <class P> GFS_HD void fst_open_done(uint64_t* w, P& p, bool ok, bool bu
 // remain, or an opener joined), 1 (last: the state is CLOSING; close on the
 // host, then fst_close_done).
 template <class P> GFS_HD int fst_close(uint64_t* w, P& p) {
-    const uint64_t st = p.load(w);
+    const uint64_t st = p.load_rlx(w);   // a check only: the add below orders the close
     if (fst_state(st) != GFS_FS_OPEN || fst_opens(st) == 0) return -EBADF;
     uint64_t cur = p.fadd(w, FST_NEG) + FST_NEG;
     while (fst_state(cur) == GFS_FS_OPEN && fst_opens(cur) == 0) {
