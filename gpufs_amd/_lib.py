@@ -33,6 +33,7 @@ STATS_FIELDS = [
     "rpc_opens", "rpc_closes", "bytes_filled", "bytes_written", "nodes_used", "files_used",
     "host_requests", "host_bytes_read", "host_bytes_written", "frames", "frames_valid", "timeouts",
     "host_busy_ns", "vm_errors", "slots_lost", "files_reclaimed", "nodes_recycled",
+    "spec_parts", "spec_rescans", "spec_changed",
 ]
 
 

@@ -115,6 +115,9 @@ typedef struct gfs_counters {
     unsigned long long slots_lost;   // ring slots quarantined after a timed-out request
     unsigned long long files_reclaimed;  // closed file-table entries taken back for other names
     unsigned long long nodes_recycled;   // radix nodes returned to the free list
+    unsigned long long spec_parts;       // split grep: parts that scanned a file before its reopen completed
+    unsigned long long spec_rescans;     // ... and scanned their range again (a page had left the cache)
+    unsigned long long spec_changed;     // ... and found the file changed by the reopen (recounted)
 } gfs_counters;
 
 typedef struct gfs_dev {

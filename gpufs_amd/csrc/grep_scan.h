@@ -1041,6 +1041,11 @@ __device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t n
         }
         __syncthreads();
         const bool same = gfs::uni(fin_flag) != 0;
+        if (t == 0) {
+            gfs::count(&gfs_g->ctr.spec_parts, 1);
+            if (fd >= 0 && !same) gfs::count(&gfs_g->ctr.spec_changed, 1);
+            else if (fd >= 0 && mine && sst) gfs::count(&gfs_g->ctr.spec_rescans, 1);
+        }
         if (fd < 0) {
             st = fd;
         } else if (!same) {
@@ -1106,6 +1111,10 @@ __device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t n
             } else if (!gfs::uni(fin_flag)) {
                 changed = true;
                 lr = 1;   // the file changed on the host: the last part recounts it
+            }
+            if (t == 0) {
+                gfs::count(&gfs_g->ctr.spec_parts, 1);
+                if (changed) gfs::count(&gfs_g->ctr.spec_changed, 1);
             }
             defer_close = fd >= 0;
             pend = 0;

@@ -383,6 +383,9 @@ extern "C" int gpufs_stats_get(gpufs_ctx* c, gpufs_stats* o) {
     o->slots_lost = cur.ctr.slots_lost;
     o->files_reclaimed = cur.ctr.files_reclaimed;
     o->nodes_recycled = cur.ctr.nodes_recycled;
+    o->spec_parts = cur.ctr.spec_parts;
+    o->spec_rescans = cur.ctr.spec_rescans;
+    o->spec_changed = cur.ctr.spec_changed;
     return 0;
 }
 

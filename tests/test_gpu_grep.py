@@ -236,9 +236,45 @@ def test_grep_split_sees_changed_file(tmp_path, orc):
         cpu2, st = orc.scan_files(paths, q, 65536, threads=4)
         assert (st == 0).all()
         assert not (cpu2[:3] == cpu[:3]).all()
+        ctx.stats_reset()
         for k in range(2):
             counts, status = grep(ctx, paths, q, 65536, parts=4, min_part=65536)
             assert (status == 0).all(), (k, status)
             for i in range(len(paths)):
                 assert (counts[i] == cpu2[i]).all(), (k, i)
+            if k == 0:
+                s = ctx.stats()
+                # every part of the 5 cached files read before the reopen;
+                # the 3 rewritten files were found changed by every part
+                assert s["spec_parts"] == 20 and s["spec_changed"] == 12, s
+        assert ctx.check_invariants() == 0
+
+
+@pytest.mark.parametrize("nfiles,fsize,cache_mib", [(16, 512 << 10, 4), (160, 256 << 10, 8)])
+def test_grep_split_overlapped_reopen_under_eviction(tmp_path, orc, nfiles, fsize, cache_mib):
+    """Files that fit part 0's pin budget in a cache smaller than the data:
+    in every pass some files are still wholly cached (their parts scan
+    before the reopen completes), others are not.  With more workgroups than
+    compute units (160 files x 4 parts), a file's other parts start after its
+    part 0 has ended and other files' fills may have evicted its pages: such
+    a part scans its range again through the open file.  Every pass equals
+    the reference."""
+    say(f"test_grep_split_overlapped_reopen_under_eviction nfiles={nfiles}")
+    voc = dg.vocab(3000, seed=61)
+    q = voc[dg.query(3000, 1024, seed=62)]
+    paths = [str(tmp_path / f"v{i}.txt") for i in range(nfiles)]
+    dg.text_files(paths, [fsize] * (nfiles - 4) + [300000, 200000, 65536, 1], 63, voc, threads=4)
+    cpu, st = orc.scan_files(paths, q, 65536, threads=4)
+    assert (st == 0).all()
+    with make_ctx(page_size=65536, cache_bytes=cache_mib << 20, ring_depth=64, max_files=256) as ctx:
+        for k in range(4):
+            order = paths if k % 2 == 0 else paths[::-1]
+            counts, status = grep(ctx, order, q, 65536, parts=4, min_part=65536)
+            assert (status == 0).all(), (k, status, ctx.stats())
+            want = cpu if k % 2 == 0 else cpu[::-1]
+            bad = [i for i in range(len(order)) if not (counts[i] == want[i]).all()]
+            assert not bad, (k, bad)
+        s = ctx.stats()
+        say(f"  spec_parts {s['spec_parts']} spec_rescans {s['spec_rescans']} evictions {s['evictions']}")
+        assert s["evictions"] > 0 and s["spec_parts"] > 0, s
         assert ctx.check_invariants() == 0
