@@ -44,8 +44,11 @@ GFS_HD inline uint64_t fst_delta(uint32_t from, uint32_t to) { return (uint64_t)
 
 #define FST_JOINED 0     // the file is OPEN and this open is counted
 #define FST_MUST_OPEN 1  // state is OPENING with this open counted: do the host open, then fst_open_done
+#define FST_PEEK 2       // (peek) another caller's host open is in progress; nothing was counted
 
-template <class P> GFS_HD int fst_open(uint64_t* w, P& p) {
+// With `peek`, a caller that finds another caller's host open in progress
+// (OPENING) returns FST_PEEK at once instead of waiting to join.
+template <class P> GFS_HD int fst_open(uint64_t* w, P& p, bool peek = false) {
     // Look before adding: when a launch's workgroups all open the same closed
     // file, one of them wins the CAS to OPENING and the others wait on plain
     // loads, then join with one add each (every atomic on the word queues
@@ -69,6 +72,7 @@ template <class P> GFS_HD int fst_open(uint64_t* w, P& p) {
         // OPENING or CLOSING: another workgroup is at the host.  Relaxed
         // polls: an acquire load at agent scope invalidates the L2 of the
         // poller's XCD each time; the fadd or cas that follows acquires.
+        if (peek && s == GFS_FS_OPENING) return FST_PEEK;
         if (p.expired()) return -ETIMEDOUT;
         p.pause();
         cur = p.load_rlx(w);

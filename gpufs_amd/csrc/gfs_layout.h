@@ -85,6 +85,8 @@ typedef struct gfs_file {
     int32_t  flags;     // open flags of the current open
     uint32_t dirty;     // set when a page was written since the last flush
     uint32_t name_hash;
+    uint32_t opened;    // successful host opens of the entry (incremented after the state turns OPEN)
+    uint32_t pad;
 } gfs_file;
 
 // One RPC slot; lives in coherent pinned host memory.  The GPU writes the
@@ -115,9 +117,9 @@ typedef struct gfs_counters {
     unsigned long long slots_lost;   // ring slots quarantined after a timed-out request
     unsigned long long files_reclaimed;  // closed file-table entries taken back for other names
     unsigned long long nodes_recycled;   // radix nodes returned to the free list
-    unsigned long long spec_parts;       // split grep: parts that scanned a file before its reopen completed
-    unsigned long long spec_rescans;     // ... and scanned their range again (a page had left the cache)
-    unsigned long long spec_changed;     // ... and found the file changed by the reopen (recounted)
+    unsigned long long spec_parts;       // workgroups that read a cached file before its reopen completed
+    unsigned long long spec_rescans;     // ... and read their range again (a page had left the cache)
+    unsigned long long spec_changed;     // ... and found the file changed by the reopen
 } gfs_counters;
 
 typedef struct gfs_dev {

@@ -896,6 +896,8 @@ __device__ inline int open_host_finish(gfs_dev* g, gfs_file* e, int flags, bool 
     e->ino = ino;
     e->dirty = 0;
     fst_open_done(&e->st, P, true, changed);
+    // after the state: a reader that saw OPENING has not seen this increment
+    __hip_atomic_fetch_add(&e->opened, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     return 0;
 }
 __device__ inline int open_host(gfs_dev* g, gfs_file* e, int flags, bool reopen) {
@@ -906,7 +908,9 @@ __device__ inline int open_host(gfs_dev* g, gfs_file* e, int flags, bool reopen)
 // (open_single with a gfs_open_pend): open_finish completes it.
 struct gfs_open_pend {
     int idx, slot, flags;
-    uint32_t pending;   // 1: posted, open_finish owes the completion
+    uint32_t pending;   // 1: this caller's reopen, open_post / open_finish owe it; 2: a peek (below)
+    uint32_t peek_ok;   // in: a caller that finds another caller's reopen at the host may peek
+    uint32_t opened0;   // peek: the entry's `opened` count when its state was OPENING
 };
 
 // Single-thread open (reference: single_thread_open, libgpufs/fs_calls.cu:152-225).
@@ -936,11 +940,28 @@ __device__ __noinline__ int open_single(const char* name, int flags, gfs_open_pe
         if (r == FT_FOUND) {
             fst_dev P{t0, g->wait_ticks};
             GFS_TRACE(25);
-            const int o = fst_open(&g->files[idx].st, P);
+            int o;
+            if (pend && pend->peek_ok) {
+                // Another caller's reopen at the host: take no open and read
+                // the cached pages meanwhile (the caller confirms later that
+                // an open succeeded after this look and the file did not
+                // change).  `opened` is read first, with an acquire, so that a
+                // state still OPENING means the count is from before the open.
+                gfs_file* e = &g->files[idx];
+                pend->opened0 = __hip_atomic_load(&e->opened, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                o = fst_open(&e->st, P, true);
+            } else {
+                o = fst_open(&g->files[idx].st, P);
+            }
             GFS_TRACE(26);
             ft_gate_leave(g, F);
             if (o == FST_JOINED) return (int)idx;
             if (o < 0) return o;
+            if (pend && o == FST_PEEK) {
+                pend->idx = (int)idx;
+                pend->pending = 2;
+                return (int)idx;
+            }
             if (pend) {
                 pend->idx = (int)idx;
                 pend->flags = flags;
@@ -1664,6 +1685,7 @@ __device__ __noinline__ int pin_cached(gfs_dev* g, int fd, uint64_t pb, uint32_t
                 g->fref[f] = 1;
             }
         }
+        wave_count(&g->ctr.page_hits, ln < cnt && !bad);
         if (ln == 0) {
             gfs_ctl.a = bad;
             gfs_memo.fd = -1;
@@ -1701,6 +1723,38 @@ __device__ inline void unpin_pages(gfs_dev* g, uint32_t cnt) {
         fadd_rlx(&g->meta[f], ~0ull);
     }
     __syncthreads();
+}
+// Copies bytes [off, off+n) of entry `fd` into `dst` from cached pages only
+// (pin_cached, at most 64 pages at a time; the entry need not be open), the
+// way gread copies pinned pages.  Returns 0, or 1 when a page was not cached
+// (the bytes before it are copied).  Workgroup-collective.
+__device__ inline int copy_cached(gfs_dev* g, int fd, uint64_t off, uint64_t n, uint8_t* dst) {
+    if (n == 0) return 0;
+    const uint32_t L = g->log_page;
+    const uint64_t PS = g->page_size;
+    const uint64_t p_first = off >> L, p_last = (off + n - 1) >> L;
+    const uint32_t T = nthreads(), t = tid();
+    for (uint64_t pb = p_first; pb <= p_last; pb += 64) {
+        const uint32_t cnt = (uint32_t)(p_last - pb + 1 < 64 ? p_last - pb + 1 : 64);
+        if (pin_cached(g, fd, pb, cnt)) return 1;
+        // the frames may have been filled on another XCD
+        if (t == 0) fence_acq_agent();
+        __syncthreads();
+        uint32_t rot = 0;
+        for (uint32_t i = 0; i < cnt; ++i) {
+            const uint64_t poff = (pb + i) << L;
+            const uint64_t lo = poff > off ? poff : off;
+            const uint64_t hi = poff + PS < off + n ? poff + PS : off + n;
+            copy_seg<true, true, SPACE_GLOBAL, SPACE_GLOBAL>(dst + (lo - off), frame_ptr(g, gfs_ctl.pinf[i]) + (lo - poff),
+                                                             hi - lo, (t + T - rot) % T, T);
+            rot = (rot + (uint32_t)((hi - lo) >> 4)) % T;
+            // as gread: the waves stay on one page
+            if (t == 0) fence_acq_agent();
+            __syncthreads();
+        }
+        unpin_pages(g, cnt);
+    }
+    return 0;
 }
 } // namespace gfs
 

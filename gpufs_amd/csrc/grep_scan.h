@@ -894,6 +894,7 @@ __device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t n
             gfs_hist.fd = -1;
             gfs_memo.fd = -1;
             opend.pending = 0;
+            opend.peek_ok = 0;
             gfs_ctl.ret = gfs::open_single(name, O_GRDONLY, &opend);
             fin_flag = opend.pending;
         }

@@ -16,17 +16,127 @@ static hipStream_t pick_stream(gpufs_ctx* ctx, void* stream) {
     return stream ? (hipStream_t)stream : (hipStream_t)gpufs_stream(ctx);
 }
 
+//
+// The reopen of a cached file overlaps the copy.  The workgroup that takes
+// the closed entry to OPENING posts the host open and copies its range from
+// the cached pages (copy_cached) before completing the open; the others find
+// the entry OPENING, take no open, copy their ranges the same way and then
+// confirm that an open succeeded after they looked and found the file as its
+// last open left it (same version and size: a reopen that finds the file
+// changed bumps the version).  Otherwise, or when a page was not cached, a
+// workgroup reads its range the usual way (gopen, gread, gclose),
+// overwriting what it copied.  So a launch costs one host open and close
+// instead of 256 waits for the open and 512 atomics on one state word.
 __global__ __launch_bounds__(1024) void k_gread_ranges(const char* path, uint64_t range, uint8_t* dst,
                                                        int64_t* out) {
-    int fd = gopen(path, O_GRDONLY);
+    using namespace gfs;
+    __shared__ gfs_open_pend pend;
+    __shared__ uint64_t snap[2];   // the entry's size, version (as its last open left them)
+    __shared__ uint32_t okf;
+    gfs_dev* g = gfs_g;
+    const uint32_t t = tid();
+    const uint64_t off = (uint64_t)blockIdx.x * range;
+    __syncthreads();
+    if (t == 0) {
+        gfs_hist.fd = -1;
+        gfs_memo.fd = -1;
+        pend.pending = 0;
+        pend.peek_ok = 1;
+        const int r0 = open_single(path, O_GRDONLY, &pend);
+        if (r0 >= 0 && pend.pending) {
+            gfs_file* e = &g->files[r0];
+            snap[0] = e->size;   // stable while OPENING: only the opener writes it, at its end
+            snap[1] = fst_version(ld_rlx(&e->st));
+            if (pend.pending == 1) open_post(&pend);
+        }
+        gfs_ctl.ret = r0;
+    }
+    __syncthreads();
+    int fd = (int)uni(gfs_ctl.ret);
+    const uint32_t mode = fd >= 0 ? uni(pend.pending) : 0u;
     if (fd < 0) {
-        if (gfs::tid() == 0) out[blockIdx.x] = fd;
+        if (t == 0) out[blockIdx.x] = fd;
         return;
     }
-    size_t r;
-    [[clang::always_inline]] r = gread(fd, (size_t)blockIdx.x * range, range, dst + (size_t)blockIdx.x * range);
-    int c = gclose(fd);
-    if (gfs::tid() == 0) out[blockIdx.x] = (int64_t)r < 0 ? (int64_t)r : (c < 0 ? (int64_t)c : (int64_t)r);
+    size_t r = 0;
+    bool done = false;
+    if (mode) {
+        const uint64_t S = uni(snap[0]);
+        const uint64_t n = off < S ? (S - off < range ? S - off : range) : 0;
+        // the opener completes its open halfway through its range (the host
+        // has answered by then), so that it ends about when the others do,
+        // which wait for that open at their end
+        const uint64_t n1 = mode == 1 ? ((n >> 1) & ~(g->page_size - 1)) : n;
+        int sst = copy_cached(g, fd, off, n1, dst + off);
+        if (t == 0) {
+            uint32_t ok = 0;
+            gfs_file* e = &g->files[fd];
+            if (mode == 1) {
+                const int rr = open_finish(&pend);
+                gfs_ctl.ret = rr;
+                if (rr >= 0) ok = fst_version(ld_rlx(&e->st)) == (uint32_t)snap[1] && e->size == S;
+            } else {
+                // An open that succeeded after the look (bounded wait).  The
+                // opener counts `opened` right after the state turns OPEN,
+                // and long before its close: OPEN or CLOSING with the old
+                // count is that moment, CLOSED with the old count (read after
+                // the state, with an acquire) is a failed open.
+                const uint64_t w0 = now_ticks();
+                for (;;) {
+                    uint32_t o = __hip_atomic_load(&e->opened, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    bool seen = o != pend.opened0;
+                    if (!seen) {
+                        const uint32_t s = fst_state(ld_acq(&e->st));
+                        if (s == GFS_FS_CLOSED || s == GFS_FS_FREE) {
+                            o = __hip_atomic_load(&e->opened, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            if (o == pend.opened0) break;   // that open failed
+                            seen = true;
+                        }
+                    }
+                    if (seen) {
+                        fence_acq_agent();
+                        ok = fst_version(ld_rlx(&e->st)) == (uint32_t)snap[1] && e->size == S;
+                        break;
+                    }
+                    if (now_ticks() - w0 >= g->wait_ticks) break;
+                    __builtin_amdgcn_s_sleep(8);
+                }
+                gfs_ctl.ret = -1;   // no open is held
+            }
+            okf = ok && sst == 0;
+            count(&g->ctr.spec_parts, 1);
+            if (!ok && (mode == 2 || (int)gfs_ctl.ret >= 0)) count(&g->ctr.spec_changed, 1);
+            else if (ok && sst) count(&g->ctr.spec_rescans, 1);
+        }
+        __syncthreads();
+        fd = (int)uni(gfs_ctl.ret);
+        if (mode == 1 && fd < 0) {
+            if (t == 0) out[blockIdx.x] = fd;
+            return;
+        }
+        if (mode == 1 && uni(okf) && n1 < n) {
+            sst = copy_cached(g, fd, off + n1, n - n1, dst + off + n1);
+            if (t == 0) {
+                okf = sst == 0;
+                if (sst) count(&g->ctr.spec_rescans, 1);
+            }
+            __syncthreads();
+        }
+        if (uni(okf)) {
+            r = n;
+            done = true;
+        }
+    }
+    if (!done) {
+        if (fd < 0) fd = gopen(path, O_GRDONLY);
+        if (fd < 0) {
+            if (t == 0) out[blockIdx.x] = fd;
+            return;
+        }
+        [[clang::always_inline]] r = gread(fd, off, range, dst + off);
+    }
+    const int c = fd >= 0 ? gclose(fd) : 0;
+    if (t == 0) out[blockIdx.x] = (int64_t)r < 0 ? (int64_t)r : (c < 0 ? (int64_t)c : (int64_t)r);
 }
 
 extern "C" int gpufs_launch_gread_ranges(gpufs_ctx* ctx, const char* path_dev, uint32_t n_ranges,

@@ -77,12 +77,12 @@ typedef struct gpufs_stats {
                                   (libgpufs/fs_structures.cu:364-369) */
     uint64_t nodes_recycled;   /* radix nodes returned to the free list (trees of reclaimed files and
                                   of files found changed on reopen) */
-    uint64_t spec_parts;       /* split grep: parts of a cached file that scanned it while its reopen
-                                  was at the host */
-    uint64_t spec_rescans;     /* of those, parts that found a page gone from the cache and scanned
+    uint64_t spec_parts;       /* workgroups (split grep parts, gread_ranges ranges) that read a cached
+                                  file while its reopen was at the host */
+    uint64_t spec_rescans;     /* of those, the ones that found a page gone from the cache and read
                                   their range again through the open file */
-    uint64_t spec_changed;     /* of those, parts whose reopen found the file changed on the host (the
-                                  file is then recounted through the open file) */
+    uint64_t spec_changed;     /* of those, the ones that found the file changed by the reopen (read
+                                  again, or recounted, through the open file) */
 } gpufs_stats;
 
 /* ---- lifecycle: replaces initializer(GPUGlobals_ptr*) (libgpufs/fs_initializer.cu:116)

@@ -196,6 +196,7 @@ def test_gread_ranges_parity_cold_and_warm(tmp_path, orc):
     data = _rand_file(p, n * rb, 11)
     fnv, _ = orc.pread_ranges(str(p), n, rb, threads=4)
     with make_ctx(page_size=256 << 10, cache_bytes=96 << 20, ring_depth=256) as ctx:
+        s0 = None
         for rnd in ("cold", "warm"):
             out, dst = gread_ranges(ctx, str(p), n, rb)
             assert (out == rb).all(), (rnd, out)
@@ -203,8 +204,45 @@ def test_gread_ranges_parity_cold_and_warm(tmp_path, orc):
             assert host.tobytes() == data, rnd
             for r in range(n):
                 assert orc.fnv1a64(host[r * rb:(r + 1) * rb]) == int(fnv[r])
+            if rnd == "cold":
+                s0 = ctx.stats()
         s = ctx.stats()
         assert s["page_misses"] == n * 4 and s["page_hits"] == n * 4
+        # the warm pass read the cached pages while the reopen was at the host
+        d = {k: s[k] - s0[k] for k in ("spec_parts", "spec_rescans", "spec_changed")}
+        assert d["spec_parts"] >= 1 and d["spec_rescans"] == 0 and d["spec_changed"] == 0, (d, s)
+        assert ctx.check_invariants() == 0
+
+
+def test_gread_ranges_sees_changed_file(tmp_path, orc):
+    """A warm gread_ranges copies the cached pages while the file's reopen is
+    at the host; a file rewritten on the host in between (same size, then
+    shorter) must be read as it is now."""
+    say("test_gread_ranges_sees_changed_file")
+    n, rb = 32, 1 << 20
+    p = tmp_path / "chg.bin"
+    _rand_file(p, n * rb, 21)
+    with make_ctx(page_size=256 << 10, cache_bytes=64 << 20, ring_depth=256) as ctx:
+        for _ in range(2):
+            out, dst = gread_ranges(ctx, str(p), n, rb)
+            assert (out == rb).all(), out
+        for k, size in enumerate((n * rb, n * rb - 3 * rb - 12345)):
+            new = np.random.default_rng(30 + k).integers(0, 256, size, dtype=np.uint8).tobytes()
+            with open(p, "r+b") as f:
+                f.write(new)
+                f.truncate(size)
+            m = os.stat(p).st_mtime_ns
+            os.utime(p, ns=(m + 2_000_000_000, m + 2_000_000_000))
+            ctx.stats_reset()
+            out, dst = gread_ranges(ctx, str(p), n, rb)
+            want = [min(rb, max(0, size - r * rb)) for r in range(n)]
+            assert out.tolist() == want, (k, out)
+            host = dst.cpu().numpy().tobytes()
+            for r in range(n):
+                assert host[r * rb:r * rb + want[r]] == new[r * rb:r * rb + want[r]], (k, r)
+            s = ctx.stats()
+            # the workgroups that read before the reopen completed all saw the change
+            assert s["spec_parts"] >= 1 and s["spec_changed"] == s["spec_parts"], (k, s)
         assert ctx.check_invariants() == 0
 
 
