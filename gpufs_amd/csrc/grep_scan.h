@@ -761,7 +761,7 @@ struct grep_split_state {
     uint32_t spec_ver;  // spec decision: the file's version then
     uint32_t spec_idx;  // spec decision: its file-table entry
     uint32_t p0_open;   // part 0 left its open for the last part to close
-    uint32_t pad;
+    uint32_t spec_opened;   // spec decision: the entry's count of successful opens before part 0's reopen
 };
 
 // Part boundaries: part 0 also walks the reference's window chain before its
@@ -862,7 +862,7 @@ __device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t n
                                        bool spec, OnSize&& on_size, OnArrive&& on_arrive) {
     const uint32_t T = gfs::nthreads(), t = gfs::tid();
     __shared__ uint32_t fin_flag;
-    __shared__ uint32_t spec_sh[2];   // spec decision: entry, version
+    __shared__ uint32_t spec_sh[3];   // spec decision: entry, version, open count
     __shared__ gfs::gfs_open_pend opend;
     const bool single = fs == nullptr;
     for (uint32_t q = t; q < nw; q += T) S.counts[q] = 0;
@@ -909,7 +909,7 @@ __device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t n
     int st = fd < 0 && (!sp || j == 0) ? fd : 0;
     uint32_t lr = 0, cut = 0;
     uint32_t held = 0;   // part 0: every page of the file pinned (page i's frame in pinf[i])
-    uint32_t sver = 0, sidx = 0;
+    uint32_t sver = 0, sidx = 0, sop = 0;
     uint32_t maxpin = gfs_g->n_frames / 8;
     maxpin = maxpin > 64 ? 64 : (maxpin < 2 ? 2 : maxpin);
     if (pend) {
@@ -919,10 +919,12 @@ __device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t n
             gfs_file* e = &gfs_g->files[fd];
             S.fsize = e->size;
             fin_flag = fst_version(gfs::ld_rlx(&e->st));
+            spec_sh[2] = __hip_atomic_load(&e->opened, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
         const uint64_t ssz = gfs::uni((uint64_t)S.fsize);
         sver = gfs::uni(fin_flag);
+        sop = gfs::uni(spec_sh[2]);
         const uint64_t np = (ssz + gfs_g->page_size - 1) >> gfs_g->log_page;
         // (the chain and the part scan then read only the pinned frames: no
         // gread, which needs the open completed)
@@ -971,6 +973,7 @@ __device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t n
                 fs->spec_size = S.fsize;
                 fs->spec_ver = sver;
                 fs->spec_idx = (uint32_t)fd;
+                fs->spec_opened = sop;
             }
             if (!single)
                 __hip_atomic_store(&fs->tag, ((uint64_t)epoch << 32) | d, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -993,6 +996,7 @@ __device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t n
                 S.fsize = fs->spec_size;
                 spec_sh[0] = fs->spec_idx;
                 spec_sh[1] = fs->spec_ver;
+                spec_sh[2] = fs->spec_opened;
             }
         }
         fin_flag = d | (warm ? 0x80000000u : 0u);
@@ -1009,6 +1013,7 @@ __device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t n
         if (specd) {
             sidx = gfs::uni(spec_sh[0]);
             sver = gfs::uni(spec_sh[1]);
+            sop = gfs::uni(spec_sh[2]);
         }
     }
     __syncthreads();
@@ -1023,44 +1028,74 @@ __device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t n
     GFS_TRACE(43);
     on_size((uint32_t)kf);
     if (specd) {
-        // This part's range from the cached pages, before the open; then the
-        // open, and the check that it found the file read.
+        // This part's range from the cached pages, before the open; then part
+        // 0's open is confirmed rather than joined: the entry counts its
+        // successful host opens, part 0 published the count from before its
+        // reopen, and part 0's open stays held until the last part closes
+        // it, so this part holds none (no join, no close on the state word).
         int sst = 0;
         const bool mine = j < kf && size && !st;
         if (mine) {
             const uint64_t pb = split_point(size, j, kf), pe = split_point(size, j + 1, kf);
             sst = grep_part_pinned<true>(S, (int)sidx, chunk, size, pb, pe, &lr, false);
         }
-        fd = gopen(name, O_GRDONLY);
         if (t == 0) {
-            uint32_t ok = 0;
-            if (fd == (int)sidx) {
-                gfs_file* e = &gfs_g->files[fd];
-                ok = fst_version(gfs::ld_rlx(&e->st)) == sver && e->size == size;
+            // 0: no open succeeded after the look (it failed); 1: the same
+            // file; 2: the file changed.  OPEN or CLOSING with the old count is
+            // the moment between the state and the count; CLOSED with the old
+            // count (read after the state, with an acquire) is a failed open.
+            gfs_file* e = &gfs_g->files[sidx];
+            uint32_t res = 0;
+            const uint64_t w0 = gfs::now_ticks();
+            for (;;) {
+                uint32_t o = __hip_atomic_load(&e->opened, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                bool seen = o != sop;
+                if (!seen) {
+                    const uint32_t s = fst_state(gfs::ld_acq(&e->st));
+                    if (s == GFS_FS_CLOSED || s == GFS_FS_FREE) {
+                        o = __hip_atomic_load(&e->opened, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (o == sop) break;
+                        seen = true;
+                    }
+                }
+                if (seen) {
+                    gfs::fence_acq_agent();
+                    res = fst_version(gfs::ld_rlx(&e->st)) == sver && e->size == size ? 1u : 2u;
+                    break;
+                }
+                if (gfs::now_ticks() - w0 >= gfs_g->wait_ticks) break;
+                __builtin_amdgcn_s_sleep(8);
             }
-            fin_flag = ok;
+            fin_flag = res;
+            gfs::count(&gfs_g->ctr.spec_parts, 1);
+            if (res == 2) gfs::count(&gfs_g->ctr.spec_changed, 1);
+            else if (res == 1 && mine && sst) gfs::count(&gfs_g->ctr.spec_rescans, 1);
         }
         __syncthreads();
-        const bool same = gfs::uni(fin_flag) != 0;
-        if (t == 0) {
-            gfs::count(&gfs_g->ctr.spec_parts, 1);
-            if (fd >= 0 && !same) gfs::count(&gfs_g->ctr.spec_changed, 1);
-            else if (fd >= 0 && mine && sst) gfs::count(&gfs_g->ctr.spec_rescans, 1);
-        }
-        if (fd < 0) {
-            st = fd;
-        } else if (!same) {
+        const uint32_t res = gfs::uni(fin_flag);
+        if (res == 2) {
             lr = 1;   // the file changed on the host: the last part recounts it
-        } else if (mine && sst) {
-            // a page left the cache meanwhile: the range again, through the open file
-            for (uint32_t q = t; q < nw; q += T) S.counts[q] = 0;
-            lr = 0;
-            __syncthreads();
-            const uint64_t pb = split_point(size, j, kf), pe = split_point(size, j + 1, kf);
-            st = grep_part_pinned(S, fd, chunk, size, pb, pe, &lr, false);
-            if (st == 1) st = grep_part_windows(S, fd, chunk, size, pb, pe, &lr);
+        } else if (res == 0 || (mine && sst)) {
+            // part 0's open failed, or a page left the cache: an open of this
+            // part's own, and the range again through it
+            fd = gopen(name, O_GRDONLY);
+            if (fd < 0) {
+                st = fd;
+            } else if (mine) {
+                for (uint32_t q = t; q < nw; q += T) S.counts[q] = 0;
+                lr = 0;
+                if (t == 0) S.fsize = fstat(fd);
+                __syncthreads();
+                if (gfs::uni((uint64_t)S.fsize) != size) {
+                    lr = 1;   // not the size the split was made for: the last part recounts
+                } else {
+                    const uint64_t pb = split_point(size, j, kf), pe = split_point(size, j + 1, kf);
+                    st = grep_part_pinned(S, fd, chunk, size, pb, pe, &lr, false);
+                    if (st == 1) st = grep_part_windows(S, fd, chunk, size, pb, pe, &lr);
+                }
+            }
         }
-        if (mine && fd >= 0) {
+        if (mine && !st) {
             for (uint32_t q = t; q < nw; q += T) {
                 const uint32_t k = S.canon[q];
                 const uint32_t v = k == 0xFFFF ? 0u : S.counts[k];
@@ -1190,9 +1225,12 @@ __device__ inline bool grep_file_split(grep_lds& S, const char* name, uint32_t n
         const uint32_t redo = gfs::uni(fin_flag);
         const uint64_t cl = gfs::uni((uint64_t)S.fsize);
         const bool close_p0 = gfs::uni(spec_sh[0]) != 0;
-        if (!st && size) st = grep_split_finish(S, fd, nw, chunk, size, row, redo, cl);
+        // a part that confirmed part 0's open holds none: it reads through
+        // part 0's, which it closes after
+        const int ffd = fd >= 0 ? fd : (close_p0 ? (int)sidx : -1);
+        if (!st && size) st = ffd >= 0 ? grep_split_finish(S, ffd, nw, chunk, size, row, redo, cl) : -EBADF;
         GFS_TRACE(48);
-        if (close_p0 && fd >= 0) gclose(fd);   // part 0's open (the same entry)
+        if (close_p0 && ffd >= 0) gclose(ffd);   // part 0's open (the same entry)
         const int c = fd >= 0 ? gclose(fd) : 0;
         if (t == 0) *status = st ? st : c;
         __syncthreads();
