@@ -50,7 +50,12 @@ def parse():
     ap.add_argument("--ring-depth", type=int, default=2048,
                     help="RPC ring slots (pinned staging = slots x page size); 2048 measured 4-5 %% faster "
                          "cold than 1024, 4096 only 1 %% more")
-    ap.add_argument("--host-threads", type=int, default=16)
+    # The host service's workers share the process's CPU quota (16 CPUs on the
+    # GPU boxes this runs on) with the main thread and the HIP runtime's
+    # threads; with 16 workers copying at once the process overran the quota
+    # now and then and was stopped until the next period (a cold 64-file grep
+    # pass, ~6 ms, then took 7-8 ms).  12 leaves room; cold reads are as fast.
+    ap.add_argument("--host-threads", type=int, default=12)
     ap.add_argument("--cold-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grep", action="store_true")
