@@ -434,7 +434,19 @@ __device__ __forceinline__ uint32_t lds_word_run(const uint8_t* corpus, uint32_t
 __device__ inline int grep_part_windows(grep_lds& S, int fd, uint32_t chunk, uint64_t size, uint64_t p_begin,
                                         uint64_t p_end, uint32_t* longrun) {
     const uint32_t t = gfs::tid();
-    if (t == 0) S.status = 0;
+    if (t == 0) {
+        S.status = 0;
+        // The range is read front to back: start gread's read history as if
+        // two reads had led up to it, so that the first window's batch already
+        // takes the read-ahead (otherwise the first two windows of a cold file
+        // wait for the host one page at a time).  The history is advisory.
+        const uint64_t p0 = p_begin >> gfs_g->log_page;
+        gfs_hist.fd = fd;
+        gfs_hist.streak = 1;
+        gfs_hist.first = p0;
+        gfs_hist.last = p0;
+        gfs_hist.fetched = p0;
+    }
     __syncthreads();
     uint64_t cursor = p_begin;
     bool first = p_begin > 0;
