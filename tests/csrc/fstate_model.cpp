@@ -47,7 +47,7 @@ enum { H_CLOSED = 0, H_OPENING = 1, H_OPEN = 2, H_CLOSING = 3 };
 static uint64_t word;
 static std::atomic<int> host{H_CLOSED};
 static std::atomic<int> holders{0};
-static std::atomic<long> host_opens{0}, host_open_fail{0}, host_closes{0}, joins{0}, bumps{0};
+static std::atomic<long> host_opens{0}, host_open_fail{0}, host_closes{0}, joins{0}, bumps{0}, peeks{0};
 
 static int n_threads = 16;
 
@@ -63,8 +63,16 @@ static void worker(int id, int iters) {
     for (int it = 0; it < iters; ++it) {
         HostPolicy P;
         check_bound("before open");
-        int r = fst_open(&word, P);
+        // odd workers sometimes peek: a host open in progress returns at once,
+        // counting nothing (the caller reads cached data and confirms later)
+        const bool peek = (id & 1) && rng() % 2 == 0;
+        int r = fst_open(&word, P, peek);
         check_bound("after open");
+        if (r == FST_PEEK) {
+            CHECK(peek, "FST_PEEK without peek");
+            peeks.fetch_add(1);
+            continue;
+        }
         CHECK(r == FST_JOINED || r == FST_MUST_OPEN, "fst_open returned %d", r);
         if (r == FST_MUST_OPEN) {
             int exp = H_CLOSED;
@@ -127,9 +135,12 @@ int main(int argc, char** argv) {
     CHECK(fst_state(w) == GFS_FS_CLOSED && fst_opens(w) == 0, "final state %u opens %u", fst_state(w), fst_opens(w));
     CHECK(host.load() == H_CLOSED, "final host state %d", host.load());
     CHECK(host_opens.load() == host_closes.load(), "host opens %ld closes %ld", host_opens.load(), host_closes.load());
-    CHECK(host_opens.load() + joins.load() + host_open_fail.load() == (long)threads * iters, "open outcomes do not add up");
-    printf("opens %ld (failed %ld) joins %ld closes %ld version bumps %ld final version %u\n", host_opens.load(),
-           host_open_fail.load(), joins.load(), host_closes.load(), bumps.load(), fst_version(w));
+    CHECK(host_opens.load() + joins.load() + host_open_fail.load() + peeks.load() == (long)threads * iters,
+          "open outcomes do not add up");
+    CHECK(threads < 2 || peeks.load() > 0, "no peek met a host open in progress");
+    printf("opens %ld (failed %ld) joins %ld peeks %ld closes %ld version bumps %ld final version %u\n",
+           host_opens.load(), host_open_fail.load(), joins.load(), peeks.load(), host_closes.load(), bumps.load(),
+           fst_version(w));
     if (failures.load()) {
         printf("fstate model FAILED (%d)\n", failures.load());
         return 1;
